@@ -1,0 +1,88 @@
+"""Seeded synthetic workloads (numpy only; identical bytes on every host).
+
+Shapes follow the benchmark configurations of this engine:
+  * ``lognormal_sphere``: x = rho * z / |z|, z ~ N(0, I_d), rho ~ LogNormal(0, s)
+  * ``gaussian_mixture``: y = c_j + sigma * eps with centres c ~ N(0, I_d),
+    x = rho * y / |y|, rho ~ LogNormal(0, s)
+Rows are points (N x d, C-contiguous float32), i.e. the reference's D x N
+column-major layout (src/FalconnPP.h:30) viewed from numpy.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+DATA_SEED = 20240601
+QUERY_SEED = 20240602
+INDEX_SEED = 42
+
+
+def _normalise_rows(y: np.ndarray, rho: np.ndarray | None) -> np.ndarray:
+    nrm = np.sqrt(np.einsum("ij,ij->i", y, y, dtype=np.float64)).astype(np.float32)
+    nrm[nrm == 0] = 1.0
+    y /= nrm[:, None]
+    if rho is not None:
+        y *= rho[:, None]
+    return y
+
+
+def lognormal_sphere(n: int, d: int, sigma: float = 0.3, seed: int = DATA_SEED, unit: bool = False,
+                     chunk: int = 1 << 20) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    out = np.empty((n, d), np.float32)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        z = rng.standard_normal((e - s, d), dtype=np.float32)
+        rho = None if unit else rng.lognormal(0.0, sigma, e - s).astype(np.float32)
+        out[s:e] = _normalise_rows(z, rho)
+    return out
+
+
+def mixture_centres(n_centres: int, d: int, seed: int = DATA_SEED) -> np.ndarray:
+    rng = np.random.default_rng(seed + 7919)
+    return rng.standard_normal((n_centres, d), dtype=np.float32)
+
+
+def gaussian_mixture(n: int, d: int, n_centres: int, within: float = 0.3, norm_sigma: float = 0.25,
+                     seed: int = DATA_SEED, centres: np.ndarray | None = None, chunk: int = 1 << 20) -> np.ndarray:
+    if centres is None:
+        centres = mixture_centres(n_centres, d)
+    rng = np.random.default_rng(seed)
+    out = np.empty((n, d), np.float32)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        j = rng.integers(0, centres.shape[0], e - s)
+        y = centres[j] + np.float32(within) * rng.standard_normal((e - s, d), dtype=np.float32)
+        rho = rng.lognormal(0.0, norm_sigma, e - s).astype(np.float32)
+        out[s:e] = _normalise_rows(y, rho)
+    return out
+
+
+# name -> (generator kwargs, index params, query params)
+CONFIGS = {
+    # Plain Falconn++ on the CPU reference: N=1e5 x d=100, unit Gaussian queries.
+    "c1": dict(n=100_000, d=100, kind="sphere", n_centres=0, nq=1000, L=50, n_proj=128, iProbes=1,
+               alpha=0.01, bucketLimit=20, qProbes=200, k=20),
+    # Plain FP32 GPU drop-in: N=1e6 x d=200 mixture of 1000 centres.
+    "c2": dict(n=1_000_000, d=200, kind="mixture", n_centres=1000, nq=10_000, L=100, n_proj=256, iProbes=3,
+               alpha=0.005, bucketLimit=20, qProbes=4000, k=20),
+    # Precision variant: N=1e7 x d=128, 5000 centres.
+    "c3": dict(n=10_000_000, d=128, kind="mixture", n_centres=5000, nq=10_000, L=64, n_proj=128, iProbes=2,
+               alpha=0.01, bucketLimit=20, qProbes=2000, k=20),
+}
+
+
+def make_config(name: str, scale: float = 1.0):
+    """Data and queries of a named configuration; ``scale`` shrinks N and nq."""
+    cfg = dict(CONFIGS[name])
+    n = max(1000, int(cfg["n"] * scale))
+    nq = max(10, int(cfg["nq"] * scale)) if scale < 1 else cfg["nq"]
+    d = cfg["d"]
+    if cfg["kind"] == "sphere":
+        X = lognormal_sphere(n, d, 0.3, DATA_SEED)
+        Q = lognormal_sphere(nq, d, 0.3, QUERY_SEED, unit=True)
+    else:
+        C = mixture_centres(cfg["n_centres"], d)
+        X = gaussian_mixture(n, d, cfg["n_centres"], 0.3, 0.25, DATA_SEED, C)
+        Q = gaussian_mixture(nq, d, cfg["n_centres"], 0.3, 0.25, QUERY_SEED, C)
+    cfg["n"], cfg["nq"] = n, nq
+    return X, Q, cfg

@@ -1,0 +1,257 @@
+"""Pins the CPU oracle before anything is checked against it (CPU only).
+
+* FWHT: golden vectors written by the reference's own fht_float
+  (tests/golden/make_fwht_golden.py), the live reference build when present,
+  and the Sylvester closed form H*H = nI on integer inputs.
+* Sign bits: closed form of minstd_rand0 + uniform_int(0,1).
+* Hashing, bucket filter, probing and rerank: brute force and invariants on
+  small inputs (integer-valued data makes every FWHT exact, so values are
+  compared exactly against a float64 Hadamard product).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy.linalg import hadamard
+
+from oracle import pyoracle as P
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_fwht_matches_reference_golden():
+    g = np.load(os.path.join(GOLD, "fwht_golden.npz"))
+    for ln in range(1, 11):
+        assert np.array_equal(P.fwht(g[f"x{ln}"]), g[f"y{ln}"]), ln
+
+
+def test_fwht_matches_live_reference_build():
+    import ctypes as C
+    p = os.path.join(os.path.dirname(P.__file__), "_ref", "libreffht_avx.so")
+    if not os.path.exists(p):
+        pytest.skip("reference FFHT not built here (reference tree absent)")
+    L = C.CDLL(p)
+    L.fht_float.argtypes = [np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS"), C.c_int]
+    rng = np.random.default_rng(5)
+    for ln in (7, 8):
+        x = (rng.standard_normal((200, 1 << ln)) * rng.lognormal(0, 2, (200, 1))).astype(np.float32)
+        ref = x.copy()
+        for row in ref:
+            r = np.ascontiguousarray(row)
+            L.fht_float(r, ln)
+            row[:] = r
+        assert np.array_equal(P.fwht(x), ref)
+
+
+def test_fwht_closed_form_and_involution():
+    rng = np.random.default_rng(1)
+    for ln in (3, 7, 8):
+        n = 1 << ln
+        H = hadamard(n).astype(np.float64)
+        x = rng.integers(-50, 51, (4, n)).astype(np.float32)
+        assert np.array_equal(P.fwht(x).astype(np.float64), x.astype(np.float64) @ H.T)
+        assert np.array_equal(P.fwht(P.fwht(x)), x * n)
+        # a dropped butterfly stage or descending-stride bug would break this:
+        y = rng.standard_normal((3, n)).astype(np.float32)
+        np.testing.assert_allclose(P.fwht(y), y.astype(np.float64) @ H.T, rtol=0, atol=1e-4 * math.sqrt(n))
+
+
+def test_sign_bits_match_closed_form():
+    g = np.load(os.path.join(GOLD, "sign_bits_golden.npz"))
+    for s in (0, 1, 42, 12345):
+        b1, b2 = P.sign_bits(s, 4096)
+        assert np.array_equal(b1, g[f"b1_{s}"]) and np.array_equal(b2, g[f"b2_{s}"])
+    b1, b2 = P.sign_bits(42, 4096)
+    assert 0.45 < b1.mean() < 0.55 and not np.array_equal(b1, b2)
+
+
+def _rot_exact(x, bits1, bits2, l, F, d):
+    """Float64 closed form of the two layers for table l (exact on integers)."""
+    H = hadamard(F).astype(np.float64)
+    out = []
+    for bits in (bits1, bits2):
+        v = np.zeros(F)
+        v[:d] = x
+        for r in range(3):
+            s = 2.0 * bits[l * 3 * F + r * F: l * 3 * F + (r + 1) * F] - 1.0
+            v = H @ (v * s)
+        out.append(v)
+    return out
+
+
+def _codes_vals(v, n_proj):
+    lg = int(math.log2(n_proj))
+    vv = v[:n_proj]
+    codes = np.arange(n_proj) | np.where(vv >= 0, 0, 1 << lg)
+    return codes, np.abs(vv)
+
+
+def test_hash_build_against_closed_form():
+    rng = np.random.default_rng(3)
+    n, d, L, n_proj, p = 64, 20, 3, 32, 2
+    X = rng.integers(-4, 5, (n, d)).astype(np.float32)
+    idx = P.OracleIndex(n, d)
+    idx.setIndexParam(L, n_proj, 5, 0.5, p, 1, 7)
+    idx.set_sign_bits(7)
+    codes, vals = idx.hash_build(X)
+    b1, b2 = P.sign_bits(7, idx.fht_dim * L * 3)
+    F = idx.fht_dim
+    assert F == 32
+    for l in range(L):
+        for j in range(n):
+            r1, r2 = _rot_exact(X[j], b1, b2, l, F, d)
+            c1, a1 = _codes_vals(r1, n_proj)
+            c2, a2 = _codes_vals(r2, n_proj)
+            top1 = np.sort(a1)[::-1][:p]
+            top2 = np.sort(a2)[::-1][:p]
+            # multiset of the top-p pair sums is tie-independent
+            sums = np.sort((top1[:, None] + top2[None, :]).ravel())[::-1][:p]
+            assert np.array_equal(np.sort(vals[l, j])[::-1].astype(np.float64), sums)
+            # every code decodes to the claimed layer values
+            for c, v in zip(codes[l, j], vals[l, j]):
+                i1, i2 = divmod(int(c), 2 * n_proj)
+                k1 = np.nonzero(c1 == i1)[0][0]
+                k2 = np.nonzero(c2 == i2)[0][0]
+                assert a1[k1] + a2[k2] == v
+            assert len(set(codes[l, j].tolist())) == p  # iProbes distinct buckets
+
+
+def test_hash_pop_order_and_strict_replacement():
+    # Gaussian data: values ascending in pop order; on ties, the earliest
+    # coordinate wins under strict '>' replacement for p == 1.
+    rng = np.random.default_rng(4)
+    idx = P.OracleIndex(8, 16)
+    idx.setIndexParam(2, 16, 5, 0.5, 3, 1, 1)
+    idx.set_sign_bits(1)
+    codes, vals = idx.hash_build(rng.standard_normal((8, 16)).astype(np.float32))
+    assert np.all(np.diff(vals, axis=2) >= 0)
+    idx1 = P.OracleIndex(1, 16)
+    idx1.setIndexParam(1, 16, 5, 0.5, 1, 1, 1)
+    idx1.set_sign_bits(1)
+    e = np.zeros((1, 16), np.float32)  # all projections 0: every coordinate ties
+    c, v = idx1.hash_build(e)
+    assert c[0, 0, 0] == 0 and v[0, 0, 0] == 0  # code (0, 0): first coordinate of each layer
+
+
+def _bucket_sets(idx):
+    pos, size, ids = idx.buckets()
+    return pos, size, ids
+
+
+@pytest.mark.parametrize("alpha,blimit,p", [(0.3, 3, 2), (0.01, 2, 1), (1.0, 1, 3)])
+def test_build_filter_rule_and_topvalues(alpha, blimit, p):
+    rng = np.random.default_rng(11)
+    n, d, L, n_proj = 400, 12, 2, 8
+    X = (rng.standard_normal((n, d)) * 3).astype(np.float32)
+    center = X.astype(np.float64).mean(0).astype(np.float32)
+    idx = P.OracleIndex(n, d)
+    idx.setIndexParam(L, n_proj, blimit, alpha, p, 2, 99)
+    idx.build(X, center)
+    codes, vals = idx.hash_build(X - center)
+    pos, size, ids = idx.buckets()
+    nb = 4 * n_proj * n_proj
+    assert len(size) == L * nb
+    kept_total = 0
+    for l in range(L):
+        members = {}
+        for j in range(n):
+            for c, v in zip(codes[l, j], vals[l, j]):
+                members.setdefault(int(c), []).append((float(v), j))
+        for h in range(nb):
+            m = members.get(h, [])
+            B = len(m)
+            if B == 0:
+                assert size[l * nb + h] == 0
+                continue
+            lim = int(math.ceil(np.float32(np.float32(alpha) * np.float32(B)) / np.float32(p)))
+            if lim < blimit:
+                lim = min(blimit, B)
+            lim = min(lim, B)
+            assert size[l * nb + h] == lim, (l, h, B)
+            kept = ids[pos[l * nb + h]: pos[l * nb + h] + lim]
+            byid = {j: v for v, j in m}
+            kv = sorted((byid[int(i)] for i in kept), reverse=True)
+            allv = sorted((v for v, _ in m), reverse=True)
+            assert kv == allv[:lim]  # kept = top-iLimit values (as a multiset)
+            assert len(set(kept.tolist())) == lim
+            kept_total += lim
+    assert kept_total == len(ids)
+
+
+def test_query_probes_candidates_and_rerank():
+    rng = np.random.default_rng(12)
+    n, d, L, n_proj, k = 3000, 24, 8, 16, 10
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    Q = rng.standard_normal((5, d)).astype(np.float32)
+    center = X.astype(np.float64).mean(0).astype(np.float32)
+    idx = P.OracleIndex(n, d)
+    idx.setIndexParam(L, n_proj, 4, 0.2, 2, 2, 5)
+    idx.build(X, center)
+    qp = 37
+    idx.set_qProbes(qp)
+    perTable, perLayer = idx.per_table(), idx.per_layer()
+    assert perTable == math.ceil(2 * qp / L) and perLayer == math.ceil(math.sqrt(perTable))
+    pos, size, ids = idx.buckets()
+    nb = 4 * n_proj * n_proj
+    Xc = X - center
+    for q in Q:
+        dbg = idx.query_debug(q, k)
+        # probe set = global top-qProbes over the per-table top-perTable pair sums
+        pool = []
+        for l in range(L):
+            c1, v1, c2, v2 = idx.hash_query_layers(q, l, perLayer)
+            s = np.sort((v1[:, None] + v2[None, :]).ravel())[::-1][:perTable]
+            pool.extend(s.tolist())
+        want = np.sort(np.array(pool, np.float32))[::-1][:qp]
+        assert np.array_equal(np.sort(dbg["probe_vals"])[::-1], want)
+        assert np.all(np.diff(dbg["probe_vals"]) <= 0)  # probed in descending order
+        assert len(set(dbg["probes"].tolist())) == qp
+        # candidates = union of probed buckets
+        cand = set()
+        ctot = 0
+        for b in dbg["probes"]:
+            cand.update(ids[pos[b]: pos[b] + size[b]].tolist())
+            ctot += int(size[b])
+        assert dbg["ctot"] == ctot
+        assert sorted(cand) == dbg["cands"].tolist()
+        # rerank = exact top-k of q.(x - c) over the candidates
+        cl = np.array(sorted(cand))
+        sc = Xc[cl].astype(np.float64) @ q.astype(np.float64)
+        order = np.argsort(-sc, kind="stable")[:k]
+        m = min(k, len(cl))
+        np.testing.assert_allclose(dbg["scores"][:m], sc[order][:m], rtol=1e-5, atol=1e-5)
+        assert set(dbg["topk"][:m].tolist()) == set(cl[order][:m].tolist())
+
+
+def test_no_filter_keeps_every_entry():
+    # alpha = 1 and iProbes = 1 with a bucket minimum >= n keep whole buckets:
+    # the index then stores each point exactly iProbes times per table.
+    rng = np.random.default_rng(13)
+    n, d, L, n_proj = 300, 8, 3, 4
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    for p in (1, 2, 3):
+        idx = P.OracleIndex(n, d)
+        idx.setIndexParam(L, n_proj, n, 1.0, p, 1, 3)
+        idx.build(X, X.mean(0))
+        pos, size, ids = idx.buckets()
+        assert int(size.sum()) == n * L * p == len(ids)
+        nb = 4 * n_proj * n_proj
+        for l in range(L):
+            cnt = np.bincount(np.concatenate([ids[pos[l * nb + h]: pos[l * nb + h] + size[l * nb + h]]
+                                              for h in range(nb)]), minlength=n)
+            assert np.all(cnt == p)
+
+
+def test_fewer_candidates_than_k_zero_fill():
+    rng = np.random.default_rng(14)
+    n, d = 50, 8
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    idx = P.OracleIndex(n, d)
+    idx.setIndexParam(1, 8, 1, 0.01, 1, 1, 3)
+    idx.build(X, X.mean(0))
+    idx.set_qProbes(1)
+    topk, scores, stats = idx.query(X[:3], 40)
+    for i in range(3):
+        nc = int(np.count_nonzero(scores[i]))
+        assert np.all(topk[i, nc + 1:] == 0)
