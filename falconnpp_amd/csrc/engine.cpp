@@ -1,0 +1,798 @@
+// Host side of the gfx950 Falconn++ engine: index state, the C ABI and the
+// orchestration of the kernels (hash.hip, build.hip, query.hip, bf.hip).
+//
+// Reference mapping: FalconnPP class state src/FalconnPP.h:10-132,
+// build2Layers_1D src/FalconnPP.cpp:21-303, query2Layers_1D :310-707.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/falconnpp_amd.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace fpp {
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // (re)allocate to at least m elements; contents are not preserved
+    void reserve(size_t m) {
+        if (m <= n && p) return;
+        reset();
+        const size_t bytes = std::max<size_t>(m, 1) * sizeof(T);
+        FPP_HIP(hipMalloc(&p, bytes));
+        n = std::max<size_t>(m, 1);
+    }
+    T* get() const { return p; }
+};
+
+struct Index {
+    int n_points = 0, n_features = 0;
+    // src/FalconnPP.h:17-28 defaults
+    int n_tables = 10, n_proj = 512, n_rotate = 3, n_threads = -1;
+    int bucket_minSize = 50;
+    float bucket_scale = 1.0f;
+    int iProbes = 1, qProbes = 1, seed = -1;
+    int fhtDim = 0;
+    bool params = false, built = false;
+
+    hipStream_t stream = nullptr;
+    int64_t ldx = 0;
+    DevBuf<float> Xc;
+    DevBuf<float> center;
+    DevBuf<uint32_t> masks;
+    DevBuf<uint64_t> bucket_off;
+    DevBuf<int32_t> ids;
+    int64_t n_ids = 0;
+    std::vector<uint8_t> bit1, bit2;
+
+    // query scratch
+    DevBuf<float> Qpad;
+    DevBuf<int32_t> qc;
+    DevBuf<float> qv;
+    DevBuf<uint32_t> qflag;
+    DevBuf<uint32_t> probes;
+    DevBuf<uint64_t> ctot;
+    DevBuf<uint64_t> cand_off;
+    DevBuf<int32_t> cand;
+    DevBuf<float> score;
+    DevBuf<uint64_t> ncand;
+    DevBuf<uint32_t> qlist;
+    DevBuf<unsigned char> qwork;
+    DevBuf<int32_t> out_ids;
+    DevBuf<float> out_scores;
+    std::vector<uint64_t> last_cand_off;
+    int last_nq = 0;
+    int gather_threads = 512;
+
+    int64_t stat_hash_ties = 0, stat_bucket_ties = 0, stat_ctot = 0, stat_cuniq = 0, stat_qties = 0;
+
+    // optional per-stage timing of query calls (HIP events on the call's stream)
+    bool profiling = false;
+    hipEvent_t ev[8] = {};
+    double stage_ms[4] = {0, 0, 0, 0};  // hash, probe select, tie replay, gather+rerank
+    void mark(int i, hipStream_t st) {
+        if (!profiling) return;
+        if (!ev[i]) FPP_HIP(hipEventCreate(&ev[i]));
+        FPP_HIP(hipEventRecord(ev[i], st));
+    }
+
+    int log2_D() const { return ilog2_exact(n_proj); }
+    int NB() const { return 4 * n_proj * n_proj; }
+    HashGeom hgeom() const { return HashGeom{fhtDim, n_features, n_proj, log2_D(), n_tables}; }
+    ~Index() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+int fail(const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+}
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return FPP_OK;
+    } catch (const Error& e) {
+        return fail(e);
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return FPP_ENOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return FPP_EINTERNAL;
+    }
+}
+
+hipStream_t pick_stream(Index& I, void* s) {
+    if (s) return reinterpret_cast<hipStream_t>(s);
+    if (!I.stream) FPP_HIP(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
+    return I.stream;
+}
+
+int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// src/FalconnPP.h:52-75 with libstdc++'s own engine and distribution.
+void make_sign_bits(Index& I) {
+    const int nbits = I.fhtDim * I.n_tables * I.n_rotate;
+    unsigned seed = (unsigned)std::chrono::system_clock::now().time_since_epoch().count();
+    if (I.seed > -1) seed = (unsigned)I.seed;
+    std::default_random_engine gen(seed);
+    std::uniform_int_distribution<uint32_t> unif(0, 1);
+    I.bit1.assign(nbits, 0);
+    I.bit2.assign(nbits, 0);
+    for (int d = 0; d < nbits; ++d) {
+        I.bit1[d] = (uint8_t)(unif(gen) & 1);
+        I.bit2[d] = (uint8_t)(unif(gen) & 1);
+    }
+    // packed flip bits: word ((l*2 + layer)*3 + r)*MW + d/32, bit d%32 set when
+    // the reference's sign bit is 0, i.e. factor -1 (src/FalconnPP.cpp:87-88)
+    const int F = I.fhtDim, MW = (F + 31) / 32;
+    std::vector<uint32_t> m((size_t)I.n_tables * 2 * 3 * MW, 0u);
+    for (int l = 0; l < I.n_tables; ++l)
+        for (int layer = 0; layer < 2; ++layer)
+            for (int r = 0; r < 3; ++r)
+                for (int d = 0; d < F; ++d) {
+                    const size_t bit = (size_t)l * 3 * F + (size_t)r * F + d;
+                    const uint8_t b = layer == 0 ? I.bit1[bit] : I.bit2[bit];
+                    if (!b) m[(((size_t)l * 2 + layer) * 3 + r) * MW + d / 32] |= 1u << (d % 32);
+                }
+    I.masks.reserve(m.size());
+    FPP_HIP(hipMemcpy(I.masks.get(), m.data(), m.size() * 4, hipMemcpyHostToDevice));
+}
+
+void check_params(const Index& I) {
+    FPP_REQUIRE(I.params, FPP_ESTATE, "setIndexParam must be called before build");
+    FPP_REQUIRE(I.n_points >= 1 && I.n_features >= 1, FPP_EINVAL, "n_points and n_features must be >= 1");
+    FPP_REQUIRE(I.n_tables >= 1, FPP_EINVAL, "n_tables must be >= 1");
+    FPP_REQUIRE(I.log2_D() >= 0, FPP_EINVAL, "n_proj must be a power of two (src/FalconnPP.cpp:46 truncates log2)");
+    FPP_REQUIRE(I.fhtDim >= 2 && I.fhtDim <= 1024, FPP_EINVAL, "fhtDim = 2^ceil(log2(max(n_proj, n_features))) must be in [2, 1024]");
+    FPP_REQUIRE(I.iProbes >= 1 && I.iProbes <= std::min(15, I.n_proj), FPP_EINVAL,
+                "iProbes must be in [1, min(15, n_proj)]");
+    FPP_REQUIRE((int64_t)I.n_points * I.iProbes < (int64_t)UINT32_MAX, FPP_EINVAL, "n_points * iProbes must be < 2^32");
+    FPP_REQUIRE((int64_t)I.n_tables * I.NB() < (int64_t)INT32_MAX, FPP_EINVAL, "n_tables * 4 * n_proj^2 must be < 2^31");
+}
+
+void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
+    check_params(I);
+    const int64_t N = I.n_points;
+    const int D = I.n_features, L = I.n_tables, p = I.iProbes, NB = I.NB();
+    FPP_REQUIRE(ld >= D, FPP_EINVAL, "leading dimension of X must be >= n_features");
+    I.built = false;
+    make_sign_bits(I);
+    // centre (deterministic FP64 chunk sums) and the centred copy
+    I.ldx = round_up(D, 4);
+    const int64_t chunk = 4096;
+    const int nchunks = (int)ceil_div(N, chunk);
+    {
+        DevBuf<double> partial;
+        partial.reserve((size_t)nchunks * D);
+        I.center.reserve(D);
+        launch_center_partial(dX, ld, N, D, partial.get(), chunk, nchunks, st);
+        launch_center_final(partial.get(), nchunks, D, N, I.center.get(), st);
+        I.Xc.reserve((size_t)N * I.ldx);
+        launch_center_apply(dX, ld, N, D, I.center.get(), I.Xc.get(), I.ldx, st);
+        FPP_HIP(hipStreamSynchronize(st));
+    }
+    const HashGeom g = I.hgeom();
+    const int64_t per_table = N * p;
+    const int64_t total = per_table * L;
+    DevBuf<uint32_t> codes;
+    DevBuf<float> vals;
+    codes.reserve(total);
+    vals.reserve(total);
+    // hashing + exact replay of tie-dependent items
+    {
+        const uint32_t cap = (uint32_t)std::min<int64_t>(N * L, std::max<int64_t>(1 << 16, N * L / 16));
+        DevBuf<uint64_t> flags;
+        DevBuf<uint32_t> fcount;
+        flags.reserve(cap);
+        fcount.reserve(1);
+        FPP_HIP(hipMemsetAsync(fcount.get(), 0, 4, st));
+        launch_hash_build(g, I.Xc.get(), I.ldx, N, p, I.masks.get(), codes.get(), vals.get(), flags.get(),
+                          fcount.get(), cap, st);
+        uint32_t nflag = 0;
+        FPP_HIP(hipMemcpyAsync(&nflag, fcount.get(), 4, hipMemcpyDeviceToHost, st));
+        FPP_HIP(hipStreamSynchronize(st));
+        const bool all = nflag > cap;
+        const uint64_t count = all ? (uint64_t)N * L : nflag;
+        I.stat_hash_ties = (int64_t)nflag;
+        if (count) {
+            // bounded workspace: replay in slices
+            const int64_t per_item = 2 * (int64_t)I.fhtDim + 8 * p + 8;
+            const int64_t slice = std::max<int64_t>(1, std::min<int64_t>(count, (int64_t)(1 << 28) / per_item));
+            DevBuf<float> work;
+            work.reserve((size_t)slice * per_item);
+            for (uint64_t s = 0; s < count; s += slice) {
+                const uint64_t c = std::min<uint64_t>(slice, count - s);
+                if (all) {
+                    // all_items mode walks items 0..c-1: offset via a temporary list
+                    std::vector<uint64_t> lst(c);
+                    for (uint64_t i = 0; i < c; ++i) lst[i] = s + i;
+                    DevBuf<uint64_t> dl;
+                    dl.reserve(c);
+                    FPP_HIP(hipMemcpyAsync(dl.get(), lst.data(), c * 8, hipMemcpyHostToDevice, st));
+                    launch_hash_build_exact(g, I.Xc.get(), I.ldx, N, p, I.masks.get(), codes.get(), vals.get(),
+                                            dl.get(), c, 0, work.get(), st);
+                    FPP_HIP(hipStreamSynchronize(st));
+                } else {
+                    launch_hash_build_exact(g, I.Xc.get(), I.ldx, N, p, I.masks.get(), codes.get(), vals.get(),
+                                            flags.get() + s, c, 0, work.get(), st);
+                }
+            }
+            FPP_HIP(hipStreamSynchronize(st));
+        }
+    }
+    // bucket sizes, limits and CSR offsets over all tables
+    const int64_t nbk = (int64_t)L * NB;
+    DevBuf<uint32_t> cnt, ksz;
+    cnt.reserve(nbk);
+    ksz.reserve(nbk);
+    FPP_HIP(hipMemsetAsync(cnt.get(), 0, nbk * 4, st));
+    launch_bucket_count(codes.get(), per_table, L, NB, cnt.get(), st);
+    launch_bucket_limit(cnt.get(), nbk, I.bucket_scale, p, I.bucket_minSize, ksz.get(), st);
+    const int64_t radix_hist = radix_hist_elems(per_table);
+    DevBuf<uint64_t> scan_tmp;
+    scan_tmp.reserve(std::max(scan_tmp_elems(nbk), std::max(scan_tmp_elems(radix_hist), scan_tmp_elems(NB))));
+    I.bucket_off.reserve(nbk + 1);
+    scan_u32_to_u64(ksz.get(), nbk, I.bucket_off.get(), scan_tmp.get(), st);
+    uint64_t kept = 0;
+    FPP_HIP(hipMemcpyAsync(&kept, I.bucket_off.get() + nbk, 8, hipMemcpyDeviceToHost, st));
+    FPP_HIP(hipStreamSynchronize(st));
+    I.ids.reserve(kept);
+    I.n_ids = (int64_t)kept;
+    // per table: stable sort by bucket, copy / select / exact replay
+    DevBuf<uint32_t> k0, v0, k1, v1, hist, sel, selc, ties, tiec;
+    DevBuf<uint64_t> hist_off, tstart;
+    k0.reserve(per_table);
+    v0.reserve(per_table);
+    k1.reserve(per_table);
+    v1.reserve(per_table);
+    hist.reserve(radix_hist);
+    hist_off.reserve(radix_hist + 1);
+    tstart.reserve(NB + 1);
+    sel.reserve(NB);
+    selc.reserve(2);
+    ties.reserve(NB);
+    tiec.reserve(1);
+    const int key_bits = ilog2_exact(NB);
+    int64_t bucket_ties = 0;
+    for (int l = 0; l < L; ++l) {
+        const uint32_t* cnt_l = cnt.get() + (size_t)l * NB;
+        const uint32_t* ksz_l = ksz.get() + (size_t)l * NB;
+        const uint64_t* off_l = I.bucket_off.get() + (size_t)l * NB;
+        const float* vals_l = vals.get() + (size_t)l * per_table;
+        FPP_HIP(hipMemcpyAsync(k0.get(), codes.get() + (size_t)l * per_table, per_table * 4, hipMemcpyDeviceToDevice,
+                               st));
+        uint32_t *sk = nullptr, *sv = nullptr;
+        radix_sort_pairs(k0.get(), v0.get(), k1.get(), v1.get(), per_table, key_bits, true, hist.get(),
+                         hist_off.get(), scan_tmp.get(), &sk, &sv, st);
+        scan_u32_to_u64(cnt_l, NB, tstart.get(), scan_tmp.get(), st);
+        launch_seg_copy(sk, sv, per_table, p, cnt_l, ksz_l, tstart.get(), off_l, I.ids.get(), st);
+        FPP_HIP(hipMemsetAsync(selc.get(), 0, 8, st));
+        launch_make_select_list(cnt_l, ksz_l, NB, sel.get(), selc.get(), st);
+        uint32_t nsel = 0;
+        FPP_HIP(hipMemcpyAsync(&nsel, selc.get(), 4, hipMemcpyDeviceToHost, st));
+        FPP_HIP(hipStreamSynchronize(st));
+        if (nsel) {
+            FPP_HIP(hipMemsetAsync(tiec.get(), 0, 4, st));
+            launch_seg_select(sel.get(), nsel, sv, vals_l, p, cnt_l, ksz_l, tstart.get(), off_l, I.ids.get(),
+                              ties.get(), tiec.get(), st);
+            uint32_t ntie = 0;
+            FPP_HIP(hipMemcpyAsync(&ntie, tiec.get(), 4, hipMemcpyDeviceToHost, st));
+            FPP_HIP(hipStreamSynchronize(st));
+            if (ntie) {
+                bucket_ties += ntie;
+                std::vector<uint32_t> tl(ntie), bsz(ntie);
+                FPP_HIP(hipMemcpy(tl.data(), ties.get(), ntie * 4, hipMemcpyDeviceToHost));
+                std::vector<uint32_t> cnt_h(NB);
+                FPP_HIP(hipMemcpy(cnt_h.data(), cnt_l, (size_t)NB * 4, hipMemcpyDeviceToHost));
+                std::vector<uint64_t> woff(ntie);
+                uint64_t run = 0;
+                for (uint32_t i = 0; i < ntie; ++i) {
+                    woff[i] = run;
+                    run += round_up((int64_t)cnt_h[tl[i]] * 9, 16);
+                }
+                DevBuf<uint64_t> dwoff;
+                DevBuf<unsigned char> work;
+                dwoff.reserve(ntie);
+                work.reserve(run);
+                FPP_HIP(hipMemcpy(dwoff.get(), woff.data(), ntie * 8, hipMemcpyHostToDevice));
+                launch_seg_exact(ties.get(), ntie, dwoff.get(), sv, vals_l, p, cnt_l, ksz_l, tstart.get(), off_l,
+                                 I.ids.get(), work.get(), st);
+                FPP_HIP(hipStreamSynchronize(st));
+            }
+        }
+    }
+    FPP_HIP(hipStreamSynchronize(st));
+    I.stat_bucket_ties = bucket_ties;
+    I.built = true;
+}
+
+struct QueryPlan {
+    int perTable, perLayer;
+};
+
+// src/FalconnPP.cpp:336-337 in double arithmetic
+QueryPlan plan_query(const Index& I) {
+    QueryPlan pl;
+    pl.perTable = (int)std::ceil(2.0 * I.qProbes / I.n_tables);
+    pl.perLayer = (int)std::ceil(std::sqrt(1.0 * pl.perTable));
+    return pl;
+}
+
+void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_out, float* d_scores,
+              hipStream_t st) {
+    FPP_REQUIRE(I.built, FPP_ESTATE, "query called before build (or after clear)");
+    FPP_REQUIRE(nq >= 0, FPP_EINVAL, "number of queries must be >= 0");
+    FPP_REQUIRE(k >= 1 && k <= 1024, FPP_EINVAL, "n_neighbors must be in [1, 1024]");
+    FPP_REQUIRE(I.qProbes >= 1, FPP_EINVAL, "qProbes must be >= 1");
+    FPP_REQUIRE(ldq >= I.n_features, FPP_EINVAL, "leading dimension of Q must be >= n_features");
+    const QueryPlan pl = plan_query(I);
+    FPP_REQUIRE(pl.perLayer <= I.n_proj, FPP_EINVAL,
+                "qProbes too large: ceil(sqrt(ceil(2*qProbes/n_tables))) exceeds n_proj (the reference pops an empty heap)");
+    FPP_REQUIRE(pl.perLayer <= 31, FPP_EINVAL, "qProbes too large: more than 31 probes per layer");
+    I.stat_ctot = I.stat_cuniq = I.stat_qties = 0;
+    I.last_nq = nq;
+    if (nq == 0) return;
+    const HashGeom hg = I.hgeom();
+    ProbeGeom pg{I.n_tables, I.NB(), pl.perLayer, pl.perTable, I.qProbes, 2 * I.n_proj};
+    const int L = I.n_tables, PL = pl.perLayer;
+    // queries padded to the scored row stride
+    I.Qpad.reserve((size_t)nq * I.ldx);
+    FPP_HIP(hipMemsetAsync(I.Qpad.get(), 0, (size_t)nq * I.ldx * 4, st));
+    FPP_HIP(hipMemcpy2DAsync(I.Qpad.get(), I.ldx * 4, dQ, ldq * 4, (size_t)I.n_features * 4, nq,
+                             hipMemcpyDeviceToDevice, st));
+    I.qc.reserve((size_t)nq * L * 2 * PL);
+    I.qv.reserve((size_t)nq * L * 2 * PL);
+    I.qflag.reserve(nq);
+    I.probes.reserve((size_t)nq * I.qProbes);
+    I.ctot.reserve(nq);
+    FPP_HIP(hipMemsetAsync(I.qflag.get(), 0, (size_t)nq * 4, st));
+    I.mark(0, st);
+    launch_hash_query(hg, I.Qpad.get(), I.ldx, nq, PL, I.masks.get(), I.qc.get(), I.qv.get(), I.qflag.get(), st);
+    I.mark(1, st);
+    launch_probe_select(pg, nq, I.qc.get(), I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(),
+                        I.qflag.get(), st);
+    I.mark(2, st);
+    std::vector<uint32_t> flag(nq);
+    FPP_HIP(hipMemcpyAsync(flag.data(), I.qflag.get(), (size_t)nq * 4, hipMemcpyDeviceToHost, st));
+    FPP_HIP(hipStreamSynchronize(st));
+    std::vector<uint32_t> lst;
+    for (int q = 0; q < nq; ++q)
+        if (flag[q]) lst.push_back((uint32_t)q);
+    I.stat_qties = (int64_t)lst.size();
+    I.mark(3, st);
+    if (!lst.empty()) {
+        const int64_t wpq = probe_exact_work_bytes(hg, pg);
+        I.qlist.reserve(lst.size());
+        I.qwork.reserve((size_t)wpq * lst.size());
+        FPP_HIP(hipMemcpyAsync(I.qlist.get(), lst.data(), lst.size() * 4, hipMemcpyHostToDevice, st));
+        launch_probe_exact(hg, pg, I.Qpad.get(), I.ldx, I.masks.get(), I.qlist.get(), (uint32_t)lst.size(),
+                           I.bucket_off.get(), I.probes.get(), I.ctot.get(), I.qwork.get(), wpq, st);
+    }
+    I.mark(4, st);
+    std::vector<uint64_t> ct(nq);
+    FPP_HIP(hipMemcpyAsync(ct.data(), I.ctot.get(), (size_t)nq * 8, hipMemcpyDeviceToHost, st));
+    FPP_HIP(hipStreamSynchronize(st));
+    std::vector<uint64_t> off(nq + 1);
+    off[0] = 0;
+    for (int q = 0; q < nq; ++q) {
+        off[q + 1] = off[q] + std::min<uint64_t>(ct[q], (uint64_t)I.n_points);
+        I.stat_ctot += (int64_t)ct[q];
+    }
+    I.cand_off.reserve(nq + 1);
+    I.cand.reserve(off[nq]);
+    I.score.reserve(off[nq]);
+    I.ncand.reserve(nq);
+    FPP_HIP(hipMemcpyAsync(I.cand_off.get(), off.data(), (nq + 1) * 8, hipMemcpyHostToDevice, st));
+    GatherArgs ga;
+    ga.N = I.n_points;
+    ga.D = I.n_features;
+    ga.ldx = I.ldx;
+    ga.qProbes = I.qProbes;
+    ga.k = k;
+    ga.window_bits = std::min<int64_t>(round_up(I.n_points, 32), (int64_t)1 << 20);
+    I.mark(5, st);
+    launch_gather_score_f32(ga, nq, I.probes.get(), I.bucket_off.get(), I.ids.get(), I.Xc.get(), I.Qpad.get(),
+                            I.ldx, I.cand_off.get(), I.cand.get(), I.score.get(), I.ncand.get(), d_out, d_scores,
+                            I.gather_threads, st);
+    I.mark(6, st);
+    std::vector<uint64_t> nc(nq);
+    FPP_HIP(hipMemcpyAsync(nc.data(), I.ncand.get(), (size_t)nq * 8, hipMemcpyDeviceToHost, st));
+    FPP_HIP(hipStreamSynchronize(st));
+    if (I.profiling) {
+        float a = 0, b = 0, c = 0, d = 0;
+        FPP_HIP(hipEventElapsedTime(&a, I.ev[0], I.ev[1]));
+        FPP_HIP(hipEventElapsedTime(&b, I.ev[1], I.ev[2]));
+        FPP_HIP(hipEventElapsedTime(&c, I.ev[3], I.ev[4]));
+        FPP_HIP(hipEventElapsedTime(&d, I.ev[5], I.ev[6]));
+        I.stage_ms[0] = a;
+        I.stage_ms[1] = b;
+        I.stage_ms[2] = c;
+        I.stage_ms[3] = d;
+    }
+    for (int q = 0; q < nq; ++q) I.stat_cuniq += (int64_t)nc[q];
+    I.last_cand_off = off;
+}
+
+}  // namespace
+}  // namespace fpp
+
+using namespace fpp;
+
+struct fpp_index {
+    Index I;
+};
+
+extern "C" {
+
+const char* fpp_last_error(void) { return g_last_error.c_str(); }
+
+int fpp_create(int n_points, int n_features, fpp_index** out) {
+    return guard([&] {
+        FPP_REQUIRE(out != nullptr, FPP_EINVAL, "null output handle");
+        FPP_REQUIRE(n_points >= 1 && n_features >= 1, FPP_EINVAL, "n_points and n_features must be >= 1");
+        int ndev = 0;
+        FPP_HIP(hipGetDeviceCount(&ndev));
+        FPP_REQUIRE(ndev > 0, FPP_EHIP, "no HIP device visible");
+        auto* h = new fpp_index();
+        h->I.n_points = n_points;
+        h->I.n_features = n_features;
+        *out = h;
+    });
+}
+
+void fpp_destroy(fpp_index* h) { delete h; }
+
+int fpp_set_index_param(fpp_index* h, int n_tables, int n_proj, int bucket_minSize, float bucket_scale, int iProbes,
+                        int n_threads, int seed) {
+    return guard([&] {
+        FPP_REQUIRE(h, FPP_EINVAL, "null index");
+        Index& I = h->I;
+        I.n_tables = n_tables;
+        I.n_proj = n_proj;
+        I.bucket_minSize = bucket_minSize;
+        I.bucket_scale = bucket_scale;
+        I.iProbes = iProbes;
+        I.n_threads = n_threads;
+        I.seed = seed;
+        FPP_REQUIRE(n_proj >= 1, FPP_EINVAL, "n_proj must be >= 1");
+        // src/FalconnPP.h:94-99
+        if (I.n_proj < I.n_features)
+            I.fhtDim = 1 << (int)std::ceil(std::log2((double)I.n_features));
+        else
+            I.fhtDim = 1 << (int)std::ceil(std::log2((double)I.n_proj));
+        I.params = true;
+    });
+}
+
+int fpp_set_qprobes(fpp_index* h, int qProbes) {
+    return guard([&] {
+        FPP_REQUIRE(h, FPP_EINVAL, "null index");
+        h->I.qProbes = qProbes;
+    });
+}
+
+int fpp_set_threads(fpp_index* h, int n_threads) {
+    return guard([&] {
+        FPP_REQUIRE(h, FPP_EINVAL, "null index");
+        h->I.n_threads = n_threads;
+    });
+}
+
+int fpp_clear(fpp_index* h) {
+    return guard([&] {
+        FPP_REQUIRE(h, FPP_EINVAL, "null index");
+        Index& I = h->I;
+        I.Xc.reset();
+        I.bucket_off.reset();
+        I.ids.reset();
+        I.masks.reset();
+        I.center.reset();
+        I.bit1.clear();
+        I.bit2.clear();
+        I.n_ids = 0;
+        I.built = false;
+    });
+}
+
+int fpp_build_device(fpp_index* h, const float* dX, int64_t ld, void* stream) {
+    return guard([&] {
+        FPP_REQUIRE(h && dX, FPP_EINVAL, "null argument");
+        do_build(h->I, dX, ld, pick_stream(h->I, stream));
+    });
+}
+
+int fpp_build(fpp_index* h, const float* X, int64_t ld) {
+    return guard([&] {
+        FPP_REQUIRE(h && X, FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        FPP_REQUIRE(ld >= I.n_features, FPP_EINVAL, "leading dimension of X must be >= n_features");
+        hipStream_t st = pick_stream(I, nullptr);
+        DevBuf<float> dX;
+        dX.reserve((size_t)I.n_points * I.n_features);
+        FPP_HIP(hipMemcpy2DAsync(dX.get(), (size_t)I.n_features * 4, X, (size_t)ld * 4, (size_t)I.n_features * 4,
+                                 I.n_points, hipMemcpyHostToDevice, st));
+        do_build(I, dX.get(), I.n_features, st);
+    });
+}
+
+int fpp_query_device(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_out, float* d_scores,
+                     void* stream) {
+    return guard([&] {
+        FPP_REQUIRE(h && (nq == 0 || (dQ && d_out)), FPP_EINVAL, "null argument");
+        do_query(h->I, dQ, ldq, nq, k, d_out, d_scores, pick_stream(h->I, stream));
+    });
+}
+
+int fpp_query(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t* out, int verbose) {
+    return guard([&] {
+        FPP_REQUIRE(h && (nq == 0 || (Q && out)), FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        FPP_REQUIRE(ldq >= I.n_features, FPP_EINVAL, "leading dimension of Q must be >= n_features");
+        FPP_REQUIRE(nq >= 0, FPP_EINVAL, "number of queries must be >= 0");
+        hipStream_t st = pick_stream(I, nullptr);
+        auto t0 = std::chrono::high_resolution_clock::now();
+        DevBuf<float> dQ;
+        dQ.reserve((size_t)std::max(nq, 1) * I.n_features);
+        if (nq > 0)
+            FPP_HIP(hipMemcpy2DAsync(dQ.get(), (size_t)I.n_features * 4, Q, (size_t)ldq * 4,
+                                     (size_t)I.n_features * 4, nq, hipMemcpyHostToDevice, st));
+        I.out_ids.reserve((size_t)std::max(nq, 1) * k);
+        do_query(I, dQ.get(), I.n_features, nq, k, I.out_ids.get(), nullptr, st);
+        if (nq > 0)
+            FPP_HIP(hipMemcpyAsync(out, I.out_ids.get(), (size_t)nq * k * 4, hipMemcpyDeviceToHost, st));
+        FPP_HIP(hipStreamSynchronize(st));
+        if (verbose) {
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
+            // src/FalconnPP.cpp:682-692
+            printf("Average number of probes per query: %g\n", (double)I.qProbes);
+            printf("Average number of unique candidates per query: %g\n", nq ? (double)I.stat_cuniq / nq : 0.0);
+            printf("Average number of candidates per query: %g\n", nq ? (double)I.stat_ctot / nq : 0.0);
+            printf("Querying Time: %g ms\n", ms);
+            fflush(stdout);
+        }
+    });
+}
+
+int fpp_get_shape(fpp_index* h, int64_t* out) {
+    return guard([&] {
+        FPP_REQUIRE(h && out, FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        out[0] = I.fhtDim;
+        out[1] = I.params ? I.NB() : 0;
+        out[2] = I.n_tables;
+        out[3] = I.n_ids;
+        out[4] = I.ldx;
+    });
+}
+
+int fpp_get_center(fpp_index* h, float* out) {
+    return guard([&] {
+        FPP_REQUIRE(h && out, FPP_EINVAL, "null argument");
+        FPP_REQUIRE(h->I.built, FPP_ESTATE, "index not built");
+        FPP_HIP(hipMemcpy(out, h->I.center.get(), (size_t)h->I.n_features * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int fpp_get_sign_bits(fpp_index* h, uint8_t* b1, uint8_t* b2) {
+    return guard([&] {
+        FPP_REQUIRE(h && b1 && b2, FPP_EINVAL, "null argument");
+        FPP_REQUIRE(!h->I.bit1.empty(), FPP_ESTATE, "index not built");
+        std::copy(h->I.bit1.begin(), h->I.bit1.end(), b1);
+        std::copy(h->I.bit2.begin(), h->I.bit2.end(), b2);
+    });
+}
+
+int fpp_get_buckets(fpp_index* h, uint64_t* off, int32_t* ids) {
+    return guard([&] {
+        FPP_REQUIRE(h && off && ids, FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        FPP_REQUIRE(I.built, FPP_ESTATE, "index not built");
+        const size_t nbk = (size_t)I.n_tables * I.NB();
+        FPP_HIP(hipMemcpy(off, I.bucket_off.get(), (nbk + 1) * 8, hipMemcpyDeviceToHost));
+        if (I.n_ids) FPP_HIP(hipMemcpy(ids, I.ids.get(), (size_t)I.n_ids * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int fpp_hash_points(fpp_index* h, const float* V, int64_t ld, int m, uint32_t* codes, float* vals) {
+    return guard([&] {
+        FPP_REQUIRE(h && V && codes && vals, FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        FPP_REQUIRE(!I.bit1.empty(), FPP_ESTATE, "sign bits not drawn: build first");
+        FPP_REQUIRE(ld >= I.n_features && m >= 1, FPP_EINVAL, "bad shape");
+        hipStream_t st = pick_stream(I, nullptr);
+        const int L = I.n_tables, p = I.iProbes;
+        const int64_t ldv = round_up(I.n_features, 4);
+        DevBuf<float> dV;
+        dV.reserve((size_t)m * ldv);
+        FPP_HIP(hipMemsetAsync(dV.get(), 0, (size_t)m * ldv * 4, st));
+        FPP_HIP(hipMemcpy2DAsync(dV.get(), ldv * 4, V, ld * 4, (size_t)I.n_features * 4, m, hipMemcpyHostToDevice,
+                                 st));
+        const int64_t total = (int64_t)m * p * L;
+        DevBuf<uint32_t> dc, fcount;
+        DevBuf<float> dv;
+        DevBuf<uint64_t> flags;
+        dc.reserve(total);
+        dv.reserve(total);
+        const uint32_t cap = (uint32_t)((int64_t)m * L);
+        flags.reserve(cap);
+        fcount.reserve(1);
+        FPP_HIP(hipMemsetAsync(fcount.get(), 0, 4, st));
+        const HashGeom g = I.hgeom();
+        launch_hash_build(g, dV.get(), ldv, m, p, I.masks.get(), dc.get(), dv.get(), flags.get(), fcount.get(), cap,
+                          st);
+        uint32_t nflag = 0;
+        FPP_HIP(hipMemcpyAsync(&nflag, fcount.get(), 4, hipMemcpyDeviceToHost, st));
+        FPP_HIP(hipStreamSynchronize(st));
+        if (nflag) {
+            DevBuf<float> work;
+            work.reserve((size_t)nflag * (2 * I.fhtDim + 8 * p + 8));
+            launch_hash_build_exact(g, dV.get(), ldv, m, p, I.masks.get(), dc.get(), dv.get(), flags.get(), nflag, 0,
+                                    work.get(), st);
+        }
+        FPP_HIP(hipMemcpyAsync(codes, dc.get(), total * 4, hipMemcpyDeviceToHost, st));
+        FPP_HIP(hipMemcpyAsync(vals, dv.get(), total * 4, hipMemcpyDeviceToHost, st));
+        FPP_HIP(hipStreamSynchronize(st));
+        I.stat_hash_ties = nflag;
+    });
+}
+
+int fpp_get_stats(fpp_index* h, int64_t* out) {
+    return guard([&] {
+        FPP_REQUIRE(h && out, FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        out[0] = I.stat_hash_ties;
+        out[1] = I.stat_bucket_ties;
+        out[2] = I.stat_ctot;
+        out[3] = I.stat_cuniq;
+        out[4] = I.stat_qties;
+        out[5] = I.last_nq;
+    });
+}
+
+int fpp_set_profiling(fpp_index* h, int on) {
+    return guard([&] {
+        FPP_REQUIRE(h, FPP_EINVAL, "null argument");
+        h->I.profiling = on != 0;
+    });
+}
+
+int fpp_get_timings(fpp_index* h, double* ms) {
+    return guard([&] {
+        FPP_REQUIRE(h && ms, FPP_EINVAL, "null argument");
+        for (int i = 0; i < 4; ++i) ms[i] = h->I.stage_ms[i];
+    });
+}
+
+int fpp_get_last_probes(fpp_index* h, uint32_t* out) {
+    return guard([&] {
+        FPP_REQUIRE(h && out, FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        if (I.last_nq)
+            FPP_HIP(hipMemcpy(out, I.probes.get(), (size_t)I.last_nq * I.qProbes * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int64_t fpp_get_last_candidates(fpp_index* h, int j, int32_t* out, int64_t cap) {
+    int64_t n = -1;
+    const int rc = guard([&] {
+        FPP_REQUIRE(h, FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        FPP_REQUIRE(j >= 0 && j < I.last_nq, FPP_EINVAL, "query index out of range");
+        uint64_t nc = 0;
+        FPP_HIP(hipMemcpy(&nc, I.ncand.get() + j, 8, hipMemcpyDeviceToHost));
+        n = (int64_t)nc;
+        const int64_t c = std::min<int64_t>(cap, n);
+        if (c > 0 && out)
+            FPP_HIP(hipMemcpy(out, I.cand.get() + I.last_cand_off[j], (size_t)c * 4, hipMemcpyDeviceToHost));
+    });
+    return rc == FPP_OK ? n : -(int64_t)rc;
+}
+
+int fpp_bruteforce_f64(const float* dX, int64_t ldx, int64_t N, int D, const float* dQ, int64_t ldq, int nq, int k,
+                       const int32_t* seed_ids, int32_t* out_ids, double* out_scores) {
+    return guard([&] {
+        FPP_REQUIRE(dX && dQ && out_ids && out_scores, FPP_EINVAL, "null argument");
+        FPP_REQUIRE(k >= 1 && N >= k && nq >= 0 && ldx >= D && ldq >= D, FPP_EINVAL, "bad shape");
+        if (nq == 0) return;
+        hipStream_t st;
+        FPP_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        struct SG {
+            hipStream_t s;
+            ~SG() { (void)hipStreamDestroy(s); }
+        } sg{st};
+        // seed rows: given ids (deduplicated per query) or 0..k-1
+        std::vector<int32_t> seeds((size_t)nq * k);
+        for (int q = 0; q < nq; ++q) {
+            std::vector<int32_t> s;
+            if (seed_ids)
+                for (int j = 0; j < k; ++j) {
+                    const int32_t id = seed_ids[(size_t)q * k + j];
+                    if (id >= 0 && id < N && std::find(s.begin(), s.end(), id) == s.end()) s.push_back(id);
+                }
+            for (int32_t id = 0; (int)s.size() < k; ++id)
+                if (std::find(s.begin(), s.end(), id) == s.end()) s.push_back(id);
+            std::copy(s.begin(), s.end(), seeds.begin() + (size_t)q * k);
+        }
+        DevBuf<int32_t> dseeds;
+        DevBuf<double> dsc, dth;
+        dseeds.reserve(seeds.size());
+        dsc.reserve(seeds.size());
+        dth.reserve(nq);
+        FPP_HIP(hipMemcpyAsync(dseeds.get(), seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice, st));
+        launch_dot64_rows(dX, ldx, D, dQ, ldq, nq, k, dseeds.get(), dsc.get(), st);
+        std::vector<double> sc(seeds.size()), th(nq);
+        FPP_HIP(hipMemcpyAsync(sc.data(), dsc.get(), sc.size() * 8, hipMemcpyDeviceToHost, st));
+        FPP_HIP(hipStreamSynchronize(st));
+        for (int q = 0; q < nq; ++q) th[q] = *std::min_element(sc.begin() + (size_t)q * k, sc.begin() + (size_t)(q + 1) * k);
+        FPP_HIP(hipMemcpyAsync(dth.get(), th.data(), nq * 8, hipMemcpyHostToDevice, st));
+        uint32_t cap = (uint32_t)std::min<int64_t>((int64_t)nq * k * 16 + (1 << 16), (int64_t)1 << 28);
+        for (int attempt = 0; attempt < 3; ++attempt) {
+            DevBuf<uint32_t> hq, hid, hc;
+            DevBuf<double> hs;
+            hq.reserve(cap);
+            hid.reserve(cap);
+            hs.reserve(cap);
+            hc.reserve(1);
+            FPP_HIP(hipMemsetAsync(hc.get(), 0, 4, st));
+            launch_bf64(dX, ldx, N, D, dQ, ldq, nq, k, dth.get(), hq.get(), hid.get(), hs.get(), hc.get(), cap, st);
+            uint32_t nh = 0;
+            FPP_HIP(hipMemcpyAsync(&nh, hc.get(), 4, hipMemcpyDeviceToHost, st));
+            FPP_HIP(hipStreamSynchronize(st));
+            if (nh > cap) {
+                cap = (uint32_t)std::min<int64_t>((int64_t)nh + 1024, (int64_t)UINT32_MAX / 2);
+                continue;
+            }
+            std::vector<uint32_t> vq(nh), vid(nh);
+            std::vector<double> vs(nh);
+            FPP_HIP(hipMemcpy(vq.data(), hq.get(), (size_t)nh * 4, hipMemcpyDeviceToHost));
+            FPP_HIP(hipMemcpy(vid.data(), hid.get(), (size_t)nh * 4, hipMemcpyDeviceToHost));
+            FPP_HIP(hipMemcpy(vs.data(), hs.get(), (size_t)nh * 8, hipMemcpyDeviceToHost));
+            std::vector<std::vector<std::pair<double, int32_t>>> per(nq);
+            for (uint32_t i = 0; i < nh; ++i) per[vq[i]].emplace_back(vs[i], (int32_t)vid[i]);
+            for (int q = 0; q < nq; ++q) {
+                auto& v = per[q];
+                std::sort(v.begin(), v.end(), [](const std::pair<double, int32_t>& a, const std::pair<double, int32_t>& b) {
+                    return a.first > b.first || (a.first == b.first && a.second < b.second);
+                });
+                FPP_REQUIRE((int)v.size() >= k, FPP_EINTERNAL, "brute force lost rows above the seed threshold");
+                for (int j = 0; j < k; ++j) {
+                    out_ids[(size_t)q * k + j] = v[j].second;
+                    out_scores[(size_t)q * k + j] = v[j].first;
+                }
+            }
+            return;
+        }
+        throw Error(FPP_ENOMEM, "brute-force hit list overflow");
+    });
+}
+
+}  // extern "C"

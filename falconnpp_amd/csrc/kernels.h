@@ -1,0 +1,110 @@
+// Launchers of the gfx950 kernels, called by the host engine (engine.cpp).
+// All pointers are device pointers; every launcher is asynchronous on `st`.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace fpp {
+
+// ---- hashing (hash.hip) ---------------------------------------------------
+// Sign flips as packed bits: masks[((l*2 + layer)*3 + r)*MW + d/32] with
+// MW = ceil(F/32); bit d%32 is set when the reference's sign bit
+// (l*3*F + r*F + d) is 0, i.e. the factor is -1.
+struct HashGeom {
+    int F;          // fhtDim (power of two, 2..1024)
+    int D;          // n_features
+    int n_proj;     // power of two, <= F
+    int log2_nproj;
+    int L;          // n_tables
+};
+
+// Build side: codes/vals[l*N*p + n*p + i] (i in pop order: ascending value);
+// (n, l) pairs whose result depends on an exact float tie are appended to
+// flag_list (as n*L + l) for the exact replay.
+void launch_hash_build(const HashGeom& g, const float* Xc, int64_t ldx, int64_t N, int p, const uint32_t* masks,
+                       uint32_t* codes, float* vals, uint64_t* flag_list, uint32_t* flag_count, uint32_t flag_cap,
+                       hipStream_t st);
+// Exact replay of the build hash for `count` flagged items (or all N*L
+// items when all_items != 0), emulating the reference's heaps.
+void launch_hash_build_exact(const HashGeom& g, const float* Xc, int64_t ldx, int64_t N, int p,
+                             const uint32_t* masks, uint32_t* codes, float* vals, const uint64_t* flag_list,
+                             uint64_t count, int all_items, float* work, hipStream_t st);
+
+// Query side: per (q, l, layer) the top-PL (code, |value|) largest first:
+// qc/qv[((q*L + l)*2 + layer)*PL + j]; qflag[q] |= 1 when a tie at or above
+// the PL boundary makes the selection depend on heap internals.
+void launch_hash_query(const HashGeom& g, const float* Q, int64_t ldq, int nq, int PL, const uint32_t* masks,
+                       int32_t* qc, float* qv, uint32_t* qflag, hipStream_t st);
+
+// ---- build (build.hip) ----------------------------------------------------
+void launch_center_partial(const float* X, int64_t ld, int64_t N, int D, double* partial, int64_t chunk,
+                           int nchunks, hipStream_t st);
+void launch_center_final(const double* partial, int nchunks, int D, int64_t N, float* center, hipStream_t st);
+void launch_center_apply(const float* X, int64_t ld, int64_t N, int D, const float* center, float* Xc,
+                         int64_t ldx, hipStream_t st);
+void launch_bucket_count(const uint32_t* codes, int64_t per_table, int L, int NB, uint32_t* cnt, hipStream_t st);
+void launch_bucket_limit(const uint32_t* cnt, int64_t nbk, float alpha, int p, int bucket_min, uint32_t* ksz,
+                         hipStream_t st);
+// exclusive scan of n u32 values into n+1 u64 offsets (out[n] = total)
+void scan_u32_to_u64(const uint32_t* in, int64_t n, uint64_t* out, uint64_t* tmp, hipStream_t st);
+int64_t scan_tmp_elems(int64_t n);
+// stable LSD radix sort of (key, val) by the low `bits` key bits; if vals_in
+// is null the values are 0..n-1.  Returns through (*keys_out, *vals_out)
+// which of the two buffer pairs holds the result.
+void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int bits, bool iota_vals,
+                      uint32_t* hist, uint64_t* hist_off, uint64_t* scan_tmp, uint32_t** keys_out,
+                      uint32_t** vals_out, hipStream_t st);
+int64_t radix_hist_elems(int64_t n);
+void launch_make_select_list(const uint32_t* cnt, const uint32_t* ksz, int NB, uint32_t* list, uint32_t* count,
+                             hipStream_t st);
+void launch_seg_copy(const uint32_t* skeys, const uint32_t* svals, int64_t n, int p, const uint32_t* cnt,
+                     const uint32_t* ksz, const uint64_t* tstart, const uint64_t* bucket_off, int32_t* ids,
+                     hipStream_t st);
+void launch_seg_select(const uint32_t* list, uint32_t nlist, const uint32_t* svals, const float* vals, int p,
+                       const uint32_t* cnt, const uint32_t* ksz, const uint64_t* tstart,
+                       const uint64_t* bucket_off, int32_t* ids, uint32_t* tie_list, uint32_t* tie_count,
+                       hipStream_t st);
+void launch_seg_exact(const uint32_t* tie_list, uint32_t ntie, const uint64_t* work_off, const uint32_t* svals,
+                      const float* vals, int p, const uint32_t* cnt, const uint32_t* ksz, const uint64_t* tstart,
+                      const uint64_t* bucket_off, int32_t* ids, void* work, hipStream_t st);
+
+// ---- query (query.hip) ----------------------------------------------------
+struct ProbeGeom {
+    int L, NB, PL, perTable, qProbes;
+    int two_np;  // 2 * n_proj: codes per layer
+};
+// Global top-qProbes bucket set per query; ctot[q] = sum of probed sizes;
+// qflag[q] |= 2 when a tie straddles a per-table or the global boundary.
+void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const float* qv,
+                         const uint64_t* bucket_off, uint32_t* probes, uint64_t* ctot, uint32_t* qflag,
+                         hipStream_t st);
+// Exact replay of the probing sequence for flagged queries.
+void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, int64_t ldq,
+                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, const uint64_t* bucket_off,
+                        uint32_t* probes, uint64_t* ctot, void* work, int64_t work_per_query, hipStream_t st);
+int64_t probe_exact_work_bytes(const HashGeom& hg, const ProbeGeom& g);
+
+struct GatherArgs {
+    int64_t N;
+    int D;
+    int64_t ldx;          // row stride of the scored copy (elements)
+    int qProbes;
+    int k;
+    int64_t window_bits;  // LDS bitset window
+};
+// Dedup (LDS bitset windows), FP32 rerank and top-k of each query.
+// cand/score: per-query segments at cand_off[q]; ncand[q] = unique count.
+void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,
+                             const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
+                             const uint64_t* cand_off, int32_t* cand, float* score, uint64_t* ncand,
+                             int32_t* out_ids, float* out_scores, int threads, hipStream_t st);
+
+// ---- FP64 brute force (bf.hip), verification only -------------------------
+void launch_bf64(const float* X, int64_t ldx, int64_t N, int D, const float* Q, int64_t ldq, int nq, int k,
+                 const double* thresh, uint32_t* hit_q, uint32_t* hit_id, double* hit_s, uint32_t* hit_count,
+                 uint32_t hit_cap, hipStream_t st);
+void launch_dot64_rows(const float* X, int64_t ldx, int D, const float* Q, int64_t ldq, int nq, int k,
+                       const int32_t* ids, double* out, hipStream_t st);
+
+}  // namespace fpp

@@ -1,0 +1,588 @@
+// Query path on gfx950 (reference: src/FalconnPP.cpp:310-707).
+//
+//  k_probe_select   per query: top-perTable pair sums of every table, then the
+//                   global top-qProbes of that pool.  The reference gets the
+//                   same set from its k-way heap merge (:540-582): a merge of
+//                   sorted lists pops the largest values first.  Ties on a cut
+//                   are flagged for the exact replay (k_probe_exact).
+//  k_gather_score   per query: bucket gather + dedup in an LDS bitset window
+//                   (the reference's boost::dynamic_bitset, :552-611), the
+//                   ascending candidate list, FP32 scores q.(x - c) (:647) and
+//                   the top-k (:650-673), ties broken towards smaller ids.
+#include "common.h"
+#include "heap.h"
+#include "kernels.h"
+
+namespace fpp {
+namespace {
+
+// ---- block / wave helpers -------------------------------------------------
+template <int T>
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* sh, uint32_t& total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {
+        const uint32_t a = t >= off ? sh[t - off] : 0u;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    total = sh[T - 1];
+    const uint32_t incl = sh[t];
+    __syncthreads();
+    return incl - v;
+}
+
+// Digit holding the k-th smallest key (1-based) of a 256-bin histogram.
+template <int T>
+__device__ __forceinline__ void block_find_digit(const uint32_t* hist, uint32_t k, uint32_t* sh, int& digit,
+                                                 uint32_t& below, uint32_t& eq, int* s_digit, uint32_t* s_below) {
+    static_assert(T >= 256, "needs >= 256 threads");
+    if (threadIdx.x == 0) {
+        *s_digit = 255;
+        *s_below = 0;
+    }
+    const uint32_t h = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan_u32<T>(h, sh, total);
+    if (threadIdx.x < 256 && ex < k && k <= ex + h) {
+        *s_digit = threadIdx.x;
+        *s_below = ex;
+    }
+    __syncthreads();
+    digit = *s_digit;
+    below = *s_below;
+    eq = hist[digit];
+    __syncthreads();
+}
+
+// Wave variant: lane t owns bins 4t..4t+3.
+__device__ __forceinline__ void wave_find_digit(const uint32_t* hist, uint32_t k, int& digit, uint32_t& below,
+                                                uint32_t& eq) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    uint32_t b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b[i] = hist[lane * 4 + i];
+    const uint32_t s = b[0] + b[1] + b[2] + b[3];
+    uint32_t incl = s;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t a = __shfl_up(incl, off, WAVE);
+        if (lane >= off) incl += a;
+    }
+    const uint32_t ex = incl - s;
+    int d = -1;
+    uint32_t bl = 0, e = 0;
+    if (ex < k && k <= incl) {
+        uint32_t run = ex;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (d < 0 && run + b[i] >= k) {
+                d = lane * 4 + i;
+                bl = run;
+                e = b[i];
+            }
+            run += b[i];
+        }
+    }
+    const uint64_t who = __ballot(d >= 0);
+    const int src = who ? (int)__ffsll((long long)who) - 1 : WAVE - 1;
+    if (!who && lane == WAVE - 1) d = 255;
+    digit = __shfl(d, src, WAVE);
+    below = __shfl(bl, src, WAVE);
+    eq = __shfl(e, src, WAVE);
+}
+
+__device__ __forceinline__ uint64_t lanes_below_mask() {
+    const int lane = threadIdx.x & (WAVE - 1);
+    return lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+}
+
+constexpr int PS_T = 256;
+constexpr int PS_WAVES = PS_T / WAVE;
+
+// LDS (dynamic): pool keys + buckets (L*perTable each), per-wave histograms,
+// block histogram, scan scratch.
+__global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_t* __restrict__ qc,
+                                                       const float* __restrict__ qv,
+                                                       const uint64_t* __restrict__ bucket_off,
+                                                       uint32_t* __restrict__ probes, uint64_t* __restrict__ ctot,
+                                                       uint32_t* __restrict__ qflag) {
+    extern __shared__ uint32_t lds[];
+    const int pool_n = g.L * g.perTable;
+    uint32_t* pkey = lds;
+    uint32_t* pbkt = lds + pool_n;
+    uint32_t* whist = pbkt + pool_n;           // PS_WAVES * 256
+    uint32_t* bhist = whist + PS_WAVES * 256;  // 256
+    uint32_t* sh = bhist + 256;                // PS_T
+    __shared__ int s_digit;
+    __shared__ uint32_t s_below;
+    __shared__ uint32_t s_tie;
+    __shared__ unsigned long long s_ctot;
+    const int q = blockIdx.x;
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    const int PL = g.PL, np = PL * PL;
+    const uint64_t bmask = lanes_below_mask();
+    if (threadIdx.x == 0) {
+        s_tie = 0;
+        s_ctot = 0;
+    }
+    __syncthreads();
+    uint32_t* hw = whist + w * 256;
+    bool tie = false;
+    for (int l = w; l < g.L; l += PS_WAVES) {
+        const size_t base = ((size_t)q * g.L + l) * 2 * PL;
+        const float* v1 = qv + base;
+        const float* v2 = qv + base + PL;
+        const int32_t* c1 = qc + base;
+        const int32_t* c2 = qc + base + PL;
+        // perTable-th largest pair sum (src/FalconnPP.cpp:457-488: fAbsHashValue1 + fAbsHashValue2)
+        uint32_t prefix = 0, pmask = 0, k = (uint32_t)g.perTable, eqc = 0;
+        for (int shift = 24; shift >= 0; shift -= 8) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) hw[lane * 4 + i] = 0;
+            __builtin_amdgcn_wave_barrier();
+            for (int t = lane; t < np; t += WAVE) {
+                const int i = t / PL, j = t - i * PL;
+                const uint32_t key = desc_key(v1[i] + v2[j]);
+                if ((key & pmask) == prefix) atomicAdd(&hw[(key >> shift) & 255u], 1u);
+            }
+            __builtin_amdgcn_wave_barrier();
+            int d;
+            uint32_t below;
+            wave_find_digit(hw, k, d, below, eqc);
+            prefix |= (uint32_t)d << shift;
+            pmask |= 255u << shift;
+            k -= below;
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (np > g.perTable && eqc != k) tie = true;
+        const uint32_t T = prefix;
+        uint32_t written = 0, eq_seen = 0;
+        for (int t0 = 0; t0 < np; t0 += WAVE) {
+            const int t = t0 + lane;
+            const bool valid = t < np;
+            const int i = valid ? t / PL : 0, j = valid ? t - i * PL : 0;
+            const uint32_t key = valid ? desc_key(v1[i] + v2[j]) : 0xffffffffu;
+            const bool lt = valid && key < T;
+            const bool eqk = valid && key == T;
+            const uint64_t meq = __ballot(eqk);
+            const bool take_eq = eqk && (eq_seen + (uint32_t)__popcll(meq & bmask)) < k;
+            const bool take = lt || take_eq;
+            const uint64_t mtake = __ballot(take);
+            const uint32_t slot = written + (uint32_t)__popcll(mtake & bmask);
+            if (take && slot < (uint32_t)g.perTable) {
+                const int idx = l * g.perTable + (int)slot;
+                pkey[idx] = key;
+                pbkt[idx] = (uint32_t)l * (uint32_t)g.NB + (uint32_t)c1[i] * (uint32_t)g.two_np + (uint32_t)c2[j];
+            }
+            written += (uint32_t)__popcll(mtake);
+            eq_seen += (uint32_t)__popcll(meq);
+        }
+    }
+    if (tie) atomicOr(&s_tie, 1u);
+    __syncthreads();
+    // global top-qProbes of the pool
+    uint32_t prefix = 0, pmask = 0, k = (uint32_t)g.qProbes, eqc = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        bhist[threadIdx.x] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < pool_n; i += PS_T) {
+            const uint32_t key = pkey[i];
+            if ((key & pmask) == prefix) atomicAdd(&bhist[(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        int d;
+        uint32_t below;
+        block_find_digit<PS_T>(bhist, k, sh, d, below, eqc, &s_digit, &s_below);
+        prefix |= (uint32_t)d << shift;
+        pmask |= 255u << shift;
+        k -= below;
+    }
+    if (eqc != k && threadIdx.x == 0) atomicOr(&s_tie, 2u);
+    const uint32_t T = prefix;
+    uint32_t written = 0, eq_taken = 0;
+    unsigned long long my_ctot = 0;
+    uint32_t* out = probes + (size_t)q * g.qProbes;
+    for (int c0 = 0; c0 < pool_n; c0 += PS_T) {
+        const int i = c0 + threadIdx.x;
+        const uint32_t key = i < pool_n ? pkey[i] : 0xffffffffu;
+        const uint32_t lt = (i < pool_n && key < T) ? 1u : 0u;
+        const uint32_t eqk = (i < pool_n && key == T) ? 1u : 0u;
+        uint32_t tot_eq;
+        const uint32_t ex_eq = block_excl_scan_u32<PS_T>(eqk, sh, tot_eq);
+        const uint32_t take = lt | ((eqk && eq_taken + ex_eq < k) ? 1u : 0u);
+        uint32_t tot_take;
+        const uint32_t ex_take = block_excl_scan_u32<PS_T>(take, sh, tot_take);
+        if (take) {
+            const uint32_t pos = written + ex_take;
+            if (pos < (uint32_t)g.qProbes) {
+                const uint32_t b = pbkt[i];
+                out[pos] = b;
+                my_ctot += bucket_off[b + 1] - bucket_off[b];
+            }
+        }
+        written += tot_take;
+        eq_taken += min(tot_eq, k - min(k, eq_taken));
+    }
+    atomicAdd(&s_ctot, my_ctot);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ctot[q] = s_ctot;
+        if (s_tie) atomicOr(qflag + q, 2u);
+    }
+}
+
+// ---- exact replay of a flagged query's probing sequence -------------------
+__device__ void fwht_serial(float* a, int F) {
+    for (int h = 1; h < F; h <<= 1)
+        for (int i = 0; i < F; i += 2 * h)
+            for (int j = i; j < i + h; ++j) {
+                const float u = a[j], v = a[j + h];
+                a[j] = u + v;
+                a[j + h] = u - v;
+            }
+}
+
+__device__ void rotate_serial(const float* src, int D, int F, const uint32_t* masks_layer, float* x) {
+    const int MW = (F + 31) / 32;
+    for (int d = 0; d < F; ++d) x[d] = d < D ? src[d] : 0.0f;
+    for (int r = 0; r < 3; ++r) {
+        const uint32_t* mb = masks_layer + r * MW;
+        for (int d = 0; d < F; ++d)
+            x[d] = __uint_as_float(__float_as_uint(x[d]) ^ (((mb[d >> 5] >> (d & 31)) & 1u) << 31));
+        fwht_serial(x, F);
+    }
+}
+
+__device__ void layer_heap(const float* x, int n_proj, int log2_np, int p, IFPair* hs, IFPair* vec) {
+    MinHeap q;
+    q.init(hs);
+    for (int r = 0; r < n_proj; ++r) {
+        const int s = x[r] >= 0.0f ? 1 : -1;
+        const float v = s * x[r];
+        int code = r;
+        if (s < 0) code |= 1 << log2_np;
+        if (q.size() < p)
+            q.push(IFPair{code, v});
+        else if (v > q.top().val) {
+            q.pop();
+            q.push(IFPair{code, v});
+        }
+    }
+    for (int i = p - 1; i >= 0; --i) {
+        vec[i] = q.top();
+        q.pop();
+    }
+}
+
+__host__ __device__ inline int64_t exact_work_bytes(int F, int L, int PL, int perTable) {
+    return (int64_t)2 * F * 4 + (int64_t)(3 * PL + perTable + (int64_t)L * perTable + L) * 8 + (int64_t)L * 4 + 64;
+}
+
+// src/FalconnPP.cpp:349-582 replayed literally with the emulated heaps.
+__global__ void k_probe_exact(HashGeom hg, ProbeGeom g, const float* __restrict__ Q, int64_t ldq,
+                              const uint32_t* __restrict__ masks, const uint32_t* __restrict__ qlist, uint32_t nlist,
+                              const uint64_t* __restrict__ bucket_off, uint32_t* __restrict__ probes,
+                              uint64_t* __restrict__ ctot, unsigned char* __restrict__ work, int64_t wpq) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nlist) return;
+    const uint32_t q = qlist[t];
+    const int F = hg.F, L = g.L, PL = g.PL, PT = g.perTable;
+    unsigned char* wp = work + (int64_t)t * wpq;
+    float* x1 = reinterpret_cast<float*>(wp);
+    float* x2 = x1 + F;
+    IFPair* hs = reinterpret_cast<IFPair*>(x2 + F);
+    IFPair* v1 = hs + PL;
+    IFPair* v2 = v1 + PL;
+    IFPair* tq = v2 + PL;
+    IFPair* list = tq + PT;
+    IFPair* mq = list + (int64_t)L * PT;
+    int* track = reinterpret_cast<int*>(mq + L);
+    const float* qv = Q + (int64_t)q * ldq;
+    for (int l = 0; l < L; ++l) {
+        const int MW = (F + 31) / 32;
+        rotate_serial(qv, hg.D, F, masks + (size_t)(l * 2 + 0) * 3 * MW, x1);
+        rotate_serial(qv, hg.D, F, masks + (size_t)(l * 2 + 1) * 3 * MW, x2);
+        layer_heap(x1, hg.n_proj, hg.log2_nproj, PL, hs, v1);
+        layer_heap(x2, hg.n_proj, hg.log2_nproj, PL, hs, v2);
+        MinHeap th;
+        th.init(tq);
+        for (int i = 0; i < PL; ++i)
+            for (int j = 0; j < PL; ++j) {
+                const int b = v1[i].idx * g.two_np + v2[j].idx;
+                const float s = v1[i].val + v2[j].val;
+                if (th.size() < PT)
+                    th.push(IFPair{b, s});
+                else if (s > th.top().val) {
+                    th.pop();
+                    th.push(IFPair{b, s});
+                }
+            }
+        int idx = PT - 1;
+        while (!th.empty()) {
+            IFPair pr = th.top();
+            th.pop();
+            pr.idx = l * g.NB + pr.idx;
+            list[(int64_t)l * PT + idx] = pr;
+            idx--;
+        }
+        track[l] = 0;
+    }
+    MaxHeap mh;
+    mh.init(mq);
+    for (int l = 0; l < L; ++l) mh.push(list[(int64_t)l * PT]);
+    uint64_t ct = 0;
+    uint32_t* out = probes + (size_t)q * g.qProbes;
+    for (int c = 0; c < g.qProbes; ++c) {
+        const IFPair pr = mh.top();
+        mh.pop();
+        out[c] = (uint32_t)pr.idx;
+        ct += bucket_off[pr.idx + 1] - bucket_off[pr.idx];
+        const int tb = pr.idx / g.NB;
+        track[tb]++;
+        if (track[tb] < PT) mh.push(list[(int64_t)tb * PT + track[tb]]);
+    }
+    ctot[q] = ct;
+}
+
+// ---- gather + dedup + FP32 rerank + top-k ---------------------------------
+template <int TB, int CPL>
+__global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uint32_t* __restrict__ probes,
+                                                          const uint64_t* __restrict__ bucket_off,
+                                                          const int32_t* __restrict__ ids,
+                                                          const float* __restrict__ Xc, const float* __restrict__ Q,
+                                                          int64_t ldq, const uint64_t* __restrict__ cand_off,
+                                                          int32_t* __restrict__ cand, float* __restrict__ score,
+                                                          uint64_t* __restrict__ ncand, int32_t* __restrict__ out_ids,
+                                                          float* __restrict__ out_scores) {
+    extern __shared__ uint32_t lds[];
+    const int64_t W = a.window_bits;
+    const int Wwords = (int)(W / 32);
+    uint32_t* bits = lds;
+    uint32_t* sh = bits + Wwords;       // TB
+    uint32_t* hist = sh + TB;           // 256
+    uint32_t* tk_key = hist + 256;      // kcap
+    uint32_t* tk_id = tk_key + 1024;    // kcap
+    __shared__ int s_digit;
+    __shared__ uint32_t s_below;
+    const int q = blockIdx.x;
+    const uint32_t* pr = probes + (size_t)q * a.qProbes;
+    const uint64_t cbase = cand_off[q];
+    const uint64_t ccap = cand_off[q + 1] - cbase;
+    int32_t* cq = cand + cbase;
+    float* sq = score + cbase;
+    uint32_t nc = 0;
+    for (int64_t w0 = 0; w0 < a.N; w0 += W) {
+        for (int i = threadIdx.x; i < Wwords; i += TB) bits[i] = 0u;
+        __syncthreads();
+        for (int i = threadIdx.x; i < a.qProbes; i += TB) {
+            const uint32_t b = pr[i];
+            const uint64_t s = bucket_off[b], e = bucket_off[b + 1];
+            for (uint64_t j = s; j < e; ++j) {
+                const int64_t rel = (int64_t)ids[j] - w0;
+                if (rel >= 0 && rel < W) atomicOr(&bits[rel >> 5], 1u << (rel & 31));
+            }
+        }
+        __syncthreads();
+        // ordered compaction: thread t scans a contiguous run of words
+        const int per = (Wwords + TB - 1) / TB;
+        const int wb = threadIdx.x * per;
+        uint32_t c = 0;
+        for (int i = 0; i < per; ++i)
+            if (wb + i < Wwords) c += __popc(bits[wb + i]);
+        uint32_t tot;
+        uint32_t pos = nc + block_excl_scan_u32<TB>(c, sh, tot);
+        for (int i = 0; i < per; ++i) {
+            if (wb + i >= Wwords) break;
+            uint32_t word = bits[wb + i];
+            while (word) {
+                const int bit = __ffs(word) - 1;
+                word &= word - 1;
+                if (pos < ccap) cq[pos] = (int32_t)(w0 + (int64_t)(wb + i) * 32 + bit);
+                pos++;
+            }
+        }
+        nc += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ncand[q] = nc;
+    // FP32 scores: groups of 8 lanes per candidate row, CPL float4 chunks per lane
+    const int g8 = threadIdx.x & 7;
+    const int grp = threadIdx.x >> 3;
+    constexpr int NG = TB / 8;
+    const int64_t nchunk = a.ldx / 4;
+    const float4* q4 = reinterpret_cast<const float4*>(Q + (int64_t)q * ldq);
+    float4 qr[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+        const int64_t ch = g8 + 8 * t;
+        qr[t] = ch < nchunk ? q4[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    for (uint32_t i0 = 0; i0 < nc; i0 += 2 * NG) {
+        const uint32_t ia = i0 + grp, ib = i0 + NG + grp;
+        const bool va = ia < nc, vb = ib < nc;
+        const float4* ra = reinterpret_cast<const float4*>(Xc + (int64_t)(va ? cq[ia] : 0) * a.ldx);
+        const float4* rb = reinterpret_cast<const float4*>(Xc + (int64_t)(vb ? cq[ib] : 0) * a.ldx);
+        float sa = 0.f, sb = 0.f;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int64_t ch = g8 + 8 * t;
+            if (ch < nchunk) {
+                const float4 xa = ra[ch];
+                const float4 xb = rb[ch];
+                sa = fmaf(qr[t].x, xa.x, sa);
+                sa = fmaf(qr[t].y, xa.y, sa);
+                sa = fmaf(qr[t].z, xa.z, sa);
+                sa = fmaf(qr[t].w, xa.w, sa);
+                sb = fmaf(qr[t].x, xb.x, sb);
+                sb = fmaf(qr[t].y, xb.y, sb);
+                sb = fmaf(qr[t].z, xb.z, sb);
+                sb = fmaf(qr[t].w, xb.w, sb);
+            }
+        }
+        sa += __shfl_xor(sa, 1, WAVE);
+        sb += __shfl_xor(sb, 1, WAVE);
+        sa += __shfl_xor(sa, 2, WAVE);
+        sb += __shfl_xor(sb, 2, WAVE);
+        sa += __shfl_xor(sa, 4, WAVE);
+        sb += __shfl_xor(sb, 4, WAVE);
+        if (g8 == 0) {
+            if (va) sq[ia] = sa;
+            if (vb) sq[ib] = sb;
+        }
+    }
+    __syncthreads();
+    // top-k: radix select of the k-th smallest descending key, ties -> smaller id
+    const uint32_t kk = min((uint32_t)a.k, nc);
+    uint32_t prefix = 0, pmask = 0, k = kk, eqc = 0;
+    if (kk > 0) {
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            for (int i = threadIdx.x; i < 256; i += TB) hist[i] = 0;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < nc; i += TB) {
+                const uint32_t key = desc_key_any(sq[i]);
+                if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+            }
+            __syncthreads();
+            int d;
+            uint32_t below;
+            block_find_digit<TB>(hist, k, sh, d, below, eqc, &s_digit, &s_below);
+            prefix |= (uint32_t)d << shift;
+            pmask |= 255u << shift;
+            k -= below;
+        }
+    }
+    const uint32_t T = prefix;
+    uint32_t written = 0, eq_taken = 0;
+    for (uint32_t c0 = 0; c0 < nc && kk > 0; c0 += TB) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t key = i < nc ? desc_key_any(sq[i]) : 0xffffffffu;
+        const uint32_t lt = (i < nc && key < T) ? 1u : 0u;
+        const uint32_t eqk = (i < nc && key == T) ? 1u : 0u;
+        uint32_t tot_eq;
+        const uint32_t ex_eq = block_excl_scan_u32<TB>(eqk, sh, tot_eq);
+        const uint32_t take = lt | ((eqk && eq_taken + ex_eq < k) ? 1u : 0u);
+        uint32_t tot_take;
+        const uint32_t ex_take = block_excl_scan_u32<TB>(take, sh, tot_take);
+        if (take && written + ex_take < kk) {
+            tk_key[written + ex_take] = key;
+            tk_id[written + ex_take] = (uint32_t)cq[i];
+        }
+        written += tot_take;
+        eq_taken += min(tot_eq, k - min(k, eq_taken));
+    }
+    __syncthreads();
+    // order the kept entries by (key, id): rank by counting (kk <= 1024)
+    int32_t* oq = out_ids + (size_t)q * a.k;
+    float* osq = out_scores ? out_scores + (size_t)q * a.k : nullptr;
+    for (uint32_t i = threadIdx.x; i < kk; i += TB) {
+        const uint32_t ki = tk_key[i], ii = tk_id[i];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < kk; ++j) {
+            const uint32_t kj = tk_key[j], ij = tk_id[j];
+            r += (kj < ki || (kj == ki && ij < ii)) ? 1u : 0u;
+        }
+        oq[r] = (int32_t)ii;
+        if (osq) osq[r] = desc_key_any_value(ki);
+    }
+    for (uint32_t i = kk + threadIdx.x; i < (uint32_t)a.k; i += TB) {
+        oq[i] = 0;
+        if (osq) osq[i] = 0.0f;
+    }
+}
+
+template <int TB>
+void dispatch_gather_cpl(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,
+                         const int32_t* ids, const float* Xc, const float* Q, int64_t ldq, const uint64_t* cand_off,
+                         int32_t* cand, float* score, uint64_t* ncand, int32_t* out_ids, float* out_scores,
+                         hipStream_t st) {
+    const int64_t nchunk = a.ldx / 4;
+    const int cpl = (int)ceil_div(nchunk, 8);
+    const size_t lds = (size_t)(a.window_bits / 32 + TB + 256 + 2048) * 4;
+#define FPP_G(C) \
+    k_gather_score_f32<TB, C><<<nq, TB, lds, st>>>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids, out_scores)
+    if (cpl <= 1)
+        FPP_G(1);
+    else if (cpl <= 2)
+        FPP_G(2);
+    else if (cpl <= 4)
+        FPP_G(4);
+    else if (cpl <= 8)
+        FPP_G(8);
+    else if (cpl <= 16)
+        FPP_G(16);
+    else
+        FPP_G(32);
+#undef FPP_G
+    FPP_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const float* qv,
+                         const uint64_t* bucket_off, uint32_t* probes, uint64_t* ctot, uint32_t* qflag,
+                         hipStream_t st) {
+    const size_t lds = ((size_t)2 * g.L * g.perTable + PS_WAVES * 256 + 256 + PS_T) * 4;
+    FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "probe pool (n_tables * perTable) too large for one workgroup");
+    if (nq == 0) return;
+    k_probe_select<<<nq, PS_T, lds, st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
+    FPP_HIP(hipGetLastError());
+}
+
+int64_t probe_exact_work_bytes(const HashGeom& hg, const ProbeGeom& g) {
+    return (exact_work_bytes(hg.F, g.L, g.PL, g.perTable) + 255) / 256 * 256;
+}
+
+void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, int64_t ldq,
+                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, const uint64_t* bucket_off,
+                        uint32_t* probes, uint64_t* ctot, void* work, int64_t work_per_query, hipStream_t st) {
+    if (nlist == 0) return;
+    k_probe_exact<<<(unsigned)ceil_div(nlist, 64), 64, 0, st>>>(hg, g, Q, ldq, masks, qlist, nlist, bucket_off,
+                                                                probes, ctot,
+                                                                reinterpret_cast<unsigned char*>(work),
+                                                                work_per_query);
+    FPP_HIP(hipGetLastError());
+}
+
+void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,
+                             const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
+                             const uint64_t* cand_off, int32_t* cand, float* score, uint64_t* ncand,
+                             int32_t* out_ids, float* out_scores, int threads, hipStream_t st) {
+    FPP_REQUIRE(a.k >= 1 && a.k <= 1024, FPP_EINVAL, "n_neighbors must be in [1, 1024]");
+    FPP_REQUIRE(a.ldx % 4 == 0, FPP_EINTERNAL, "row stride must be a multiple of 4");
+    FPP_REQUIRE(a.ldx <= 1024, FPP_EINVAL, "n_features must be <= 1024");
+    if (nq == 0) return;
+    if (threads <= 256)
+        dispatch_gather_cpl<256>(a, nq, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids,
+                                 out_scores, st);
+    else if (threads <= 512)
+        dispatch_gather_cpl<512>(a, nq, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids,
+                                 out_scores, st);
+    else
+        dispatch_gather_cpl<1024>(a, nq, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand,
+                                  out_ids, out_scores, st);
+}
+
+}  // namespace fpp

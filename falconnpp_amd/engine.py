@@ -1,0 +1,225 @@
+"""ctypes binding of the engine's C ABI (include/falconnpp_amd.h).
+
+This is how tests and the benchmark drive the HIP library directly, including
+the device-pointer entry points.  The drop-in Python surface of the reference
+(``FalconnPP.FalconnPP``) is the pybind11 module built beside this file.
+There is no fallback: if the native library is missing, loading fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfalconnpp_amd.so")
+
+# Every symbol include/falconnpp_amd.h declares.
+EXPORTS = (
+    "fpp_last_error", "fpp_create", "fpp_destroy", "fpp_set_index_param", "fpp_set_qprobes", "fpp_set_threads",
+    "fpp_clear", "fpp_build", "fpp_build_device", "fpp_query", "fpp_query_device", "fpp_get_shape",
+    "fpp_get_center", "fpp_get_sign_bits", "fpp_get_buckets", "fpp_hash_points", "fpp_get_stats",
+    "fpp_get_last_probes", "fpp_get_last_candidates", "fpp_bruteforce_f64", "fpp_set_profiling",
+    "fpp_get_timings",
+)
+
+FPP_EINVAL, FPP_ESTATE, FPP_EHIP, FPP_ENOMEM, FPP_EINTERNAL = 1, 2, 3, 4, 5
+
+_lib = None
+
+
+class FppError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def load():
+    """Load libfalconnpp_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C falconnpp_amd/csrc` "
+                          "(or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int
+    L.fpp_last_error.restype = C.c_char_p
+    L.fpp_create.argtypes = [i32, i32, C.POINTER(vp)]
+    L.fpp_destroy.argtypes = [vp]
+    L.fpp_destroy.restype = None
+    L.fpp_set_index_param.argtypes = [vp, i32, i32, i32, C.c_float, i32, i32, i32]
+    L.fpp_set_qprobes.argtypes = [vp, i32]
+    L.fpp_set_threads.argtypes = [vp, i32]
+    L.fpp_clear.argtypes = [vp]
+    L.fpp_build.argtypes = [vp, vp, i64]
+    L.fpp_build_device.argtypes = [vp, vp, i64, vp]
+    L.fpp_query.argtypes = [vp, vp, i64, i32, i32, vp, i32]
+    L.fpp_query_device.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
+    L.fpp_get_shape.argtypes = [vp, vp]
+    L.fpp_get_center.argtypes = [vp, vp]
+    L.fpp_get_sign_bits.argtypes = [vp, vp, vp]
+    L.fpp_get_buckets.argtypes = [vp, vp, vp]
+    L.fpp_hash_points.argtypes = [vp, vp, i64, i32, vp, vp]
+    L.fpp_get_stats.argtypes = [vp, vp]
+    L.fpp_get_last_probes.argtypes = [vp, vp]
+    L.fpp_set_profiling.argtypes = [vp, i32]
+    L.fpp_get_timings.argtypes = [vp, vp]
+    L.fpp_get_last_candidates.restype = i64
+    L.fpp_get_last_candidates.argtypes = [vp, i32, vp, i64]
+    L.fpp_bruteforce_f64.argtypes = [vp, i64, i64, i32, vp, i64, i32, i32, vp, vp, vp]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise FppError(rc, load().fpp_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _points(a) -> np.ndarray:
+    """N x D row-per-point float32 (== the reference's D x N column-major)."""
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Index:
+    """One engine index, driven through the C ABI."""
+
+    def __init__(self, n_points: int, n_features: int):
+        L = load()
+        h = C.c_void_p()
+        _check(L.fpp_create(n_points, n_features, C.byref(h)))
+        self.h = h
+        self.n, self.d = n_points, n_features
+        self.qprobes = 1
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fpp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def setIndexParam(self, n_tables, n_proj, bucket_minSize, bucket_scale, iProbes, n_threads=8, random_seed=-1):
+        self.L, self.n_proj, self.iProbes = n_tables, n_proj, iProbes
+        _check(load().fpp_set_index_param(self.h, n_tables, n_proj, bucket_minSize, bucket_scale, iProbes,
+                                          n_threads, random_seed))
+
+    def set_qProbes(self, p: int):
+        self.qprobes = p
+        _check(load().fpp_set_qprobes(self.h, p))
+
+    def set_threads(self, t: int):
+        _check(load().fpp_set_threads(self.h, t))
+
+    def clear(self):
+        _check(load().fpp_clear(self.h))
+
+    def build(self, X_nd):
+        X = _points(X_nd)
+        assert X.shape == (self.n, self.d)
+        _check(load().fpp_build(self.h, _ptr(X), self.d))
+
+    def build_device(self, dX_ptr: int, ld: int, stream: int = 0):
+        _check(load().fpp_build_device(self.h, C.c_void_p(dX_ptr), ld, C.c_void_p(stream or None)))
+
+    def query(self, Q_nd, k: int, verbose: bool = False) -> np.ndarray:
+        Q = _points(Q_nd)
+        nq = Q.shape[0]
+        out = np.zeros((nq, k), np.int32)
+        _check(load().fpp_query(self.h, _ptr(Q), self.d, nq, k, _ptr(out), 1 if verbose else 0))
+        return out
+
+    def query_device(self, dQ_ptr: int, ldq: int, nq: int, k: int, dout_ptr: int, dscores_ptr: int = 0,
+                     stream: int = 0):
+        _check(load().fpp_query_device(self.h, C.c_void_p(dQ_ptr), ldq, nq, k, C.c_void_p(dout_ptr),
+                                       C.c_void_p(dscores_ptr or None), C.c_void_p(stream or None)))
+
+    # ---- inspection ----
+    def shape(self):
+        s = np.zeros(5, np.int64)
+        _check(load().fpp_get_shape(self.h, _ptr(s)))
+        return dict(fht_dim=int(s[0]), buckets_per_table=int(s[1]), n_tables=int(s[2]), n_ids=int(s[3]),
+                    ldx=int(s[4]))
+
+    def center(self) -> np.ndarray:
+        c = np.zeros(self.d, np.float32)
+        _check(load().fpp_get_center(self.h, _ptr(c)))
+        return c
+
+    def sign_bits(self):
+        sh = self.shape()
+        n = sh["fht_dim"] * sh["n_tables"] * 3
+        b1 = np.zeros(n, np.uint8)
+        b2 = np.zeros(n, np.uint8)
+        _check(load().fpp_get_sign_bits(self.h, _ptr(b1), _ptr(b2)))
+        return b1, b2
+
+    def buckets(self):
+        sh = self.shape()
+        nb = sh["buckets_per_table"] * sh["n_tables"]
+        off = np.zeros(nb + 1, np.uint64)
+        ids = np.zeros(max(sh["n_ids"], 1), np.int32)
+        _check(load().fpp_get_buckets(self.h, _ptr(off), _ptr(ids)))
+        return off, ids[: sh["n_ids"]]
+
+    def hash_points(self, V_md):
+        V = _points(V_md)
+        m = V.shape[0]
+        codes = np.zeros(self.L * m * self.iProbes, np.uint32)
+        vals = np.zeros(self.L * m * self.iProbes, np.float32)
+        _check(load().fpp_hash_points(self.h, _ptr(V), self.d, m, _ptr(codes), _ptr(vals)))
+        return codes.reshape(self.L, m, self.iProbes), vals.reshape(self.L, m, self.iProbes)
+
+    def stats(self):
+        s = np.zeros(6, np.int64)
+        _check(load().fpp_get_stats(self.h, _ptr(s)))
+        return dict(hash_ties=int(s[0]), bucket_ties=int(s[1]), ctot=int(s[2]), cuniq=int(s[3]),
+                    query_ties=int(s[4]), nq=int(s[5]))
+
+    def set_profiling(self, on: bool):
+        _check(load().fpp_set_profiling(self.h, 1 if on else 0))
+
+    def timings(self):
+        """Device ms of the last query: hash, probe select, tie replay, gather+rerank."""
+        t = np.zeros(4, np.float64)
+        _check(load().fpp_get_timings(self.h, _ptr(t)))
+        return dict(hash=t[0], probe=t[1], replay=t[2], gather=t[3])
+
+    def last_probes(self, nq: int) -> np.ndarray:
+        out = np.zeros((nq, self.qprobes), np.uint32)
+        _check(load().fpp_get_last_probes(self.h, _ptr(out)))
+        return out
+
+    def last_candidates(self, j: int) -> np.ndarray:
+        n = load().fpp_get_last_candidates(self.h, j, None, 0)
+        if n < 0:
+            _check(int(-n))
+        out = np.zeros(max(n, 1), np.int32)
+        load().fpp_get_last_candidates(self.h, j, _ptr(out), n)
+        return out[:n]
+
+
+def bruteforce_f64_device(dX_ptr: int, ldx: int, N: int, D: int, dQ_ptr: int, ldq: int, nq: int, k: int,
+                          seed_ids: np.ndarray | None = None):
+    """Exact FP64 top-k on the GPU (verification only)."""
+    ids = np.zeros((nq, k), np.int32)
+    sc = np.zeros((nq, k), np.float64)
+    seeds = None if seed_ids is None else np.ascontiguousarray(seed_ids, np.int32)
+    _check(load().fpp_bruteforce_f64(C.c_void_p(dX_ptr), ldx, N, D, C.c_void_p(dQ_ptr), ldq, nq, k,
+                                     None if seeds is None else _ptr(seeds), _ptr(ids), _ptr(sc)))
+    return ids, sc
+
+
+def recall(truth: np.ndarray, answer: np.ndarray, k: int) -> float:
+    """Mean over queries of |truth[:k] & answer| / k (test/falconnpp.py:181-186)."""
+    s = 0.0
+    for t, a in zip(truth, answer):
+        s += len(set(t[:k].tolist()) & set(a.tolist())) / k
+    return s / max(len(truth), 1)

@@ -1,0 +1,116 @@
+/*
+ * C ABI of the MI355X (gfx950) Falconn++ engine.
+ *
+ * The entry points mirror the reference's FalconnPP class one to one: its
+ * only foreign-function boundary is the pybind11 module that binds that class
+ * (python/python_wrapper.cpp:14-30).  Each declaration names the reference
+ * member it replaces.  Plain pointers and sizes only; every call returns 0 on
+ * success or an FPP_E* status, with the message in fpp_last_error().
+ *
+ * Matrices follow the reference layout: X is D x N column-major (each point's
+ * D floats contiguous, consecutive points `ld` floats apart); Q is D x nq the
+ * same way.  Results are nq x k row-major int32 ids, best first; slots with no
+ * candidate stay 0 (src/FalconnPP.cpp:615-673).
+ *
+ * An index is not thread-safe: like the reference (mutable qProbes, global
+ * OpenMP thread count) one index serves one caller at a time.
+ */
+#ifndef FALCONNPP_AMD_H
+#define FALCONNPP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FPP_OK = 0,
+    FPP_EINVAL = 1,    /* bad argument */
+    FPP_ESTATE = 2,    /* call out of order (query before build, ...) */
+    FPP_EHIP = 3,      /* HIP runtime error */
+    FPP_ENOMEM = 4,    /* allocation failed */
+    FPP_EINTERNAL = 5  /* engine invariant broken */
+};
+
+typedef struct fpp_index fpp_index;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fpp_last_error(void);
+
+/* FalconnPP(int n, int d) -- src/FalconnPP.h:80-83 */
+int fpp_create(int n_points, int n_features, fpp_index** out);
+/* ~FalconnPP() -- src/FalconnPP.h:131 */
+void fpp_destroy(fpp_index* h);
+
+/* Index2Layers(L, D, bucketLimit, alpha, p, t, s) -- src/FalconnPP.h:85-100,
+ * bound as setIndexParam(n_tables, n_proj, bucket_minSize, bucket_scale,
+ * iProbes, n_threads=8, random_seed=-1) at python/python_wrapper.cpp:19-22.
+ * n_threads sizes host-side work only.  seed <= -1 draws from the clock. */
+int fpp_set_index_param(fpp_index* h, int n_tables, int n_proj, int bucket_minSize, float bucket_scale,
+                        int iProbes, int n_threads, int seed);
+/* set_qProbes(p) -- src/FalconnPP.h:113-115 */
+int fpp_set_qprobes(fpp_index* h, int qProbes);
+/* set_threads(t) -- src/FalconnPP.h:117-123 */
+int fpp_set_threads(fpp_index* h, int n_threads);
+/* clear() -- src/FalconnPP.h:102-111: drops data, tables and sign bits. */
+int fpp_clear(fpp_index* h);
+
+/* build2Layers_1D(X) -- src/FalconnPP.cpp:21-303, bound as build(dataset) at
+ * python/python_wrapper.cpp:28.  X: host D x N col-major, leading dim ld >= D.
+ * The index keeps its own centred copy in HBM (src/FalconnPP.cpp:33-34). */
+int fpp_build(fpp_index* h, const float* X, int64_t ld);
+/* Same with X already in device memory (HBM); stream may be NULL. */
+int fpp_build_device(fpp_index* h, const float* dX, int64_t ld, void* stream);
+
+/* query2Layers_1D(Q, k, verbose) -- src/FalconnPP.cpp:310-707, bound as
+ * query(queries, n_neighbors, verbose=False) at python/python_wrapper.cpp:29-30.
+ * Q: host D x nq col-major (ldq >= D); out: host nq x k. */
+int fpp_query(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t* out, int verbose);
+/* Same with Q and outputs in device memory; scores (nq x k, FP32 q.(x-c)) may
+ * be NULL. */
+int fpp_query_device(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_out, float* d_scores,
+                     void* stream);
+
+/* ---- inspection (parity tests and tools; no reference counterpart) ------ */
+/* index shape: out[0]=fhtDim, [1]=4*n_proj^2 buckets per table, [2]=n_tables,
+ * [3]=number of stored ids, [4]=row stride of the centred copy */
+int fpp_get_shape(fpp_index* h, int64_t* out);
+/* centre of the data (n_features floats) */
+int fpp_get_center(fpp_index* h, float* out);
+/* the two sign-bit vectors of src/FalconnPP.h:52-75 (fhtDim*n_tables*3 each) */
+int fpp_get_sign_bits(fpp_index* h, uint8_t* bit1, uint8_t* bit2);
+/* CSR of the tables: off[n_tables*4*n_proj^2 + 1] and ids[off[last]];
+ * bucket b (global id = table*4*n_proj^2 + code) holds ids[off[b]..off[b+1])
+ * in ascending id order. */
+int fpp_get_buckets(fpp_index* h, uint64_t* off, int32_t* ids);
+/* build-side hash of m host vectors (D x m col-major, taken as already
+ * centred): codes/vals[(l*m + j)*iProbes + i], i in ascending value order.
+ * Needs the sign bits of a previous build. */
+int fpp_hash_points(fpp_index* h, const float* V, int64_t ld, int m, uint32_t* codes, float* vals);
+/* statistics: out[0]=build hash items replayed for ties, [1]=buckets replayed
+ * for ties, [2]=sum C_tot of the last query call, [3]=sum C_uniq, [4]=queries
+ * replayed for ties, [5]=queries in the last call */
+int fpp_get_stats(fpp_index* h, int64_t* out);
+/* per-stage device time of the last query call when profiling is on (HIP
+ * events on the call's stream): ms[0]=query hashing, [1]=probe selection,
+ * [2]=tie replay, [3]=gather + dedup + rerank + top-k */
+int fpp_set_profiling(fpp_index* h, int on);
+int fpp_get_timings(fpp_index* h, double* ms);
+/* probes of the last query call (nq x qProbes global bucket ids, as a set per
+ * query) and, for query j, its unique candidates (ascending; returns count) */
+int fpp_get_last_probes(fpp_index* h, uint32_t* out);
+int64_t fpp_get_last_candidates(fpp_index* h, int j, int32_t* out, int64_t cap);
+
+/* FP64 brute-force top-k over all N (src/BF.cpp:17-87 in FP64), device
+ * inputs: X raw (not centred) D x N col-major, Q D x nq.  seed_ids (host,
+ * nq x k, may be NULL) are known rows used to bound each query's k-th score.
+ * Outputs on the host, best first.  For recall checks only. */
+int fpp_bruteforce_f64(const float* dX, int64_t ldx, int64_t N, int D, const float* dQ, int64_t ldq, int nq, int k,
+                       const int32_t* seed_ids, int32_t* out_ids, double* out_scores);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FALCONNPP_AMD_H */

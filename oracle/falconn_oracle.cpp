@@ -436,6 +436,30 @@ void orc_hash_build(void* h, const float* V, int m, int32_t* codes, float* vals)
     }
 }
 
+// Adopt an index built elsewhere (the engine's CSR: off[L*NB+1], ids) so the
+// oracle's query path can be timed or checked on identical tables.  X raw
+// (col-major D x N, leading dimension ld), centre given; sign bits from seed.
+int orc_load_index(void* h, const float* X, int64_t ld, const float* center, const uint64_t* off,
+                   const int32_t* ids) {
+    Index& I = *(Index*)h;
+    const int N = I.n_points, D = I.n_features;
+    I.center.assign(center, center + D);
+    I.X.assign((size_t)D * N, 0.0f);
+    for (int n = 0; n < N; ++n)
+        for (int d = 0; d < D; ++d) I.X[(size_t)n * D + d] = X[(size_t)n * ld + d] - I.center[d];
+    const int nbits = I.fhtDim * I.n_tables * I.n_rotate;
+    I.bit1.assign(nbits, 0);
+    I.bit2.assign(nbits, 0);
+    sign_bits((unsigned)I.seed, nbits, I.bit1.data(), I.bit2.data());
+    const size_t nbk = (size_t)I.n_tables * I.nb();
+    I.bucketPos.assign(off, off + nbk);
+    I.bucketSize.resize(nbk);
+    for (size_t b = 0; b < nbk; ++b) I.bucketSize[b] = (uint32_t)(off[b + 1] - off[b]);
+    I.ids.assign(ids, ids + off[nbk]);
+    I.built = true;
+    return 0;
+}
+
 // Install sign bits without building (for hashing tests).
 void orc_set_sign_bits(void* h, unsigned seed) {
     Index& I = *(Index*)h;

@@ -67,6 +67,7 @@ def lib():
         L.orc_query.argtypes = [C.c_void_p, f32p, C.c_int, C.c_int, i32p, f32p, i64p]
         L.orc_query_debug.restype = C.c_int64
         L.orc_query_debug.argtypes = [C.c_void_p, f32p, C.c_int, i32p, f32p, i32p, f32p, i32p, C.c_int64, i64p]
+        L.orc_load_index.argtypes = [C.c_void_p, f32p, C.c_int64, f32p, u64p, i32p]
         L.orc_bruteforce_f64.argtypes = [f32p, C.c_int, C.c_int, f32p, C.c_int, C.c_int, i32p, f64p, C.c_int]
         _lib = L
     return _lib
@@ -150,6 +151,12 @@ class OracleIndex:
         ids = np.zeros(max(ni, 1), np.int32)
         lib().orc_get_buckets(self.h, pos, size, ids)
         return pos, size, ids[:ni]
+
+    def load_index(self, X_nd: np.ndarray, center: np.ndarray, off: np.ndarray, ids: np.ndarray):
+        """Adopt CSR tables built elsewhere (uses the seed given to setIndexParam)."""
+        X = _cm(X_nd)
+        lib().orc_load_index(self.h, X, X.shape[1], np.ascontiguousarray(center, np.float32),
+                             np.ascontiguousarray(off, np.uint64), np.ascontiguousarray(ids, np.int32))
 
     def set_sign_bits(self, seed: int):
         lib().orc_set_sign_bits(self.h, seed)
