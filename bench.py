@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--cpu-queries", type=int, default=1000)
     ap.add_argument("--cpu-tables", type=int, default=16)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine is single-GPU (multi-GPU sharding is design-only); "
@@ -56,6 +58,16 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_mib": args.calibrate_copy, "copies": 3}))
+        return
 
     from falconnpp_amd import engine as E
     from falconnpp_amd import synth
@@ -86,7 +98,7 @@ def main():
     idx.set_qProbes(cfg["qProbes"])
     out = torch.zeros((nq, k), dtype=torch.int32, device="cuda")
     scores = torch.zeros((nq, k), dtype=torch.float32, device="cuda")
-    idx.set_profiling(True)
+    idx.set_profiling(1)
     for _ in range(args.warmup):
         idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
     torch.cuda.synchronize()
@@ -114,6 +126,13 @@ def main():
                 "kernel": "k_gather_score_f32", "kernel_ms": round(gather_ms, 3),
                 "alg_bytes_per_launch": int(alg_bytes)}
     stages = {key: round(float(np.mean([s[key] for s in stage])), 3) for key in ("hash", "probe", "replay", "gather")}
+    # one more call, outside the timed region, with per-query phase stamps
+    idx.set_profiling(2)
+    idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
+    ph = idx.timings()
+    idx.set_profiling(1)
+    stages.update({key: round(float(ph[key]), 2) for key in ("ph_gather_us", "ph_compact_us", "ph_score_us",
+                                                             "ph_topk_us")})
 
     res = {}
     if not args.no_recall:

@@ -284,16 +284,21 @@ __global__ __launch_bounds__(SEL_T) void k_seg_select(const uint32_t* __restrict
             if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t run = 0;
-            int d = 0;
-            for (; d < 256; ++d) {
-                if (run + hist[d] >= k) break;
-                run += hist[d];
-            }
-            s_prefix = prefix | ((uint32_t)d << shift);
-            s_k = k - run;
-            s_eqcnt = hist[d];
+        // digit holding the k-th smallest key: exclusive scan of the 256 bins
+        const uint32_t h = hist[threadIdx.x];
+        scan[threadIdx.x] = h;
+        __syncthreads();
+        for (int off = 1; off < SEL_T; off <<= 1) {
+            const uint32_t a = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 0u;
+            __syncthreads();
+            scan[threadIdx.x] += a;
+            __syncthreads();
+        }
+        const uint32_t ex = scan[threadIdx.x] - h;
+        if (ex < k && k <= ex + h) {
+            s_prefix = prefix | ((uint32_t)threadIdx.x << shift);
+            s_k = k - ex;
+            s_eqcnt = h;
         }
         __syncthreads();
         prefix = s_prefix;

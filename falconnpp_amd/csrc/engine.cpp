@@ -84,13 +84,22 @@ struct Index {
     std::vector<uint64_t> last_cand_off;
     int last_nq = 0;
     int gather_threads = 512;
+    bool order_queries = true;
+    int wall_khz = 0;
+    DevBuf<uint32_t> okey, oval, okey2, oval2, ohist, order;
+    DevBuf<uint64_t> ohist_off, oscan;
+    DevBuf<uint64_t> phase_ts;
 
     int64_t stat_hash_ties = 0, stat_bucket_ties = 0, stat_ctot = 0, stat_cuniq = 0, stat_qties = 0;
 
     // optional per-stage timing of query calls (HIP events on the call's stream)
-    bool profiling = false;
+    bool profiling = false;   // HIP events around the query stages
+    bool phase_stamps = false;  // plus per-query phase stamps in the gather kernel
     hipEvent_t ev[8] = {};
-    double stage_ms[4] = {0, 0, 0, 0};  // hash, probe select, tie replay, gather+rerank
+    // hash, probe select, tie replay, gather+rerank (ms); then mean per-query
+    // microseconds of the gather kernel's phases: bucket gather + dedup,
+    // candidate compaction, scoring, top-k
+    double stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     void mark(int i, hipStream_t st) {
         if (!profiling) return;
         if (!ev[i]) FPP_HIP(hipEventCreate(&ev[i]));
@@ -394,7 +403,8 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         I.qwork.reserve((size_t)wpq * lst.size());
         FPP_HIP(hipMemcpyAsync(I.qlist.get(), lst.data(), lst.size() * 4, hipMemcpyHostToDevice, st));
         launch_probe_exact(hg, pg, I.Qpad.get(), I.ldx, I.masks.get(), I.qlist.get(), (uint32_t)lst.size(),
-                           I.bucket_off.get(), I.probes.get(), I.ctot.get(), I.qwork.get(), wpq, st);
+                           I.qflag.get(), I.qc.get(), I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(),
+                           I.qwork.get(), wpq, st);
     }
     I.mark(4, st);
     std::vector<uint64_t> ct(nq);
@@ -418,6 +428,28 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     ga.qProbes = I.qProbes;
     ga.k = k;
     ga.window_bits = std::min<int64_t>(round_up(I.n_points, 32), (int64_t)1 << 20);
+    if (I.order_queries && nq > 1) {
+        const int bits = ilog2_exact(I.NB());
+        const int64_t rh = radix_hist_elems(nq);
+        I.okey.reserve(nq);
+        I.oval.reserve(nq);
+        I.okey2.reserve(nq);
+        I.oval2.reserve(nq);
+        I.ohist.reserve(rh);
+        I.ohist_off.reserve(rh + 1);
+        I.oscan.reserve(scan_tmp_elems(rh));
+        I.order.reserve(nq);
+        launch_order_keys(I.qc.get(), nq, L, PL, 2 * I.n_proj, I.okey.get(), st);
+        uint32_t *sk = nullptr, *sv = nullptr;
+        radix_sort_pairs(I.okey.get(), I.oval.get(), I.okey2.get(), I.oval2.get(), nq, bits, true, I.ohist.get(),
+                         I.ohist_off.get(), I.oscan.get(), &sk, &sv, st);
+        launch_xcd_order(sv, nq, 8, I.order.get(), st);
+        ga.order = I.order.get();
+    }
+    if (I.phase_stamps) {
+        I.phase_ts.reserve((size_t)nq * 5);
+        ga.phase_ts = I.phase_ts.get();
+    }
     I.mark(5, st);
     launch_gather_score_f32(ga, nq, I.probes.get(), I.bucket_off.get(), I.ids.get(), I.Xc.get(), I.Qpad.get(),
                             I.ldx, I.cand_off.get(), I.cand.get(), I.score.get(), I.ncand.get(), d_out, d_scores,
@@ -436,6 +468,19 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         I.stage_ms[1] = b;
         I.stage_ms[2] = c;
         I.stage_ms[3] = d;
+    }
+    if (I.phase_stamps) {
+        if (!I.wall_khz) {
+            int dev = 0;
+            FPP_HIP(hipGetDevice(&dev));
+            FPP_HIP(hipDeviceGetAttribute(&I.wall_khz, hipDeviceAttributeWallClockRate, dev));
+        }
+        std::vector<uint64_t> ts((size_t)nq * 5);
+        FPP_HIP(hipMemcpy(ts.data(), I.phase_ts.get(), ts.size() * 8, hipMemcpyDeviceToHost));
+        double acc[4] = {0, 0, 0, 0};
+        for (int q = 0; q < nq; ++q)
+            for (int i = 0; i < 4; ++i) acc[i] += (double)(ts[(size_t)q * 5 + i + 1] - ts[(size_t)q * 5 + i]);
+        for (int i = 0; i < 4; ++i) I.stage_ms[4 + i] = acc[i] / nq * 1000.0 / std::max(1, I.wall_khz);
     }
     for (int q = 0; q < nq; ++q) I.stat_cuniq += (int64_t)nc[q];
     I.last_cand_off = off;
@@ -682,13 +727,32 @@ int fpp_set_profiling(fpp_index* h, int on) {
     return guard([&] {
         FPP_REQUIRE(h, FPP_EINVAL, "null argument");
         h->I.profiling = on != 0;
+        h->I.phase_stamps = on >= 2;
     });
 }
 
 int fpp_get_timings(fpp_index* h, double* ms) {
     return guard([&] {
         FPP_REQUIRE(h && ms, FPP_EINVAL, "null argument");
-        for (int i = 0; i < 4; ++i) ms[i] = h->I.stage_ms[i];
+        for (int i = 0; i < 8; ++i) ms[i] = h->I.stage_ms[i];
+    });
+}
+
+int fpp_set_tuning(fpp_index* h, int which, int value) {
+    return guard([&] {
+        FPP_REQUIRE(h, FPP_EINVAL, "null argument");
+        switch (which) {
+            case FPP_TUNE_QUERY_ORDER:
+                h->I.order_queries = value != 0;
+                break;
+            case FPP_TUNE_GATHER_THREADS:
+                FPP_REQUIRE(value == 256 || value == 512 || value == 1024, FPP_EINVAL,
+                            "gather threads must be 256, 512 or 1024");
+                h->I.gather_threads = value;
+                break;
+            default:
+                throw Error(FPP_EINVAL, "unknown tuning knob");
+        }
     });
 }
 

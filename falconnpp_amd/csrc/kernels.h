@@ -79,10 +79,12 @@ struct ProbeGeom {
 void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const float* qv,
                          const uint64_t* bucket_off, uint32_t* probes, uint64_t* ctot, uint32_t* qflag,
                          hipStream_t st);
-// Exact replay of the probing sequence for flagged queries.
+// Exact replay of the probing sequence for flagged queries: per (query,
+// table) lists in parallel, then the global heap merge per query.
 void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, int64_t ldq,
-                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, const uint64_t* bucket_off,
-                        uint32_t* probes, uint64_t* ctot, void* work, int64_t work_per_query, hipStream_t st);
+                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, const uint32_t* qflag,
+                        const int32_t* qc, const float* qv, const uint64_t* bucket_off, uint32_t* probes,
+                        uint64_t* ctot, void* work, int64_t work_per_query, hipStream_t st);
 int64_t probe_exact_work_bytes(const HashGeom& hg, const ProbeGeom& g);
 
 struct GatherArgs {
@@ -92,7 +94,12 @@ struct GatherArgs {
     int qProbes;
     int k;
     int64_t window_bits;  // LDS bitset window
+    const uint32_t* order = nullptr;  // block -> query (null: identity)
+    uint64_t* phase_ts = nullptr;     // per query 5 wall-clock stamps (null: off)
 };
+// Query order for cache reuse: sort key per query, then the XCD-aware map.
+void launch_order_keys(const int32_t* qc, int nq, int L, int PL, int two_np, uint32_t* key, hipStream_t st);
+void launch_xcd_order(const uint32_t* sorted, int nq, int nxcd, uint32_t* order, hipStream_t st);
 // Dedup (LDS bitset windows), FP32 rerank and top-k of each query.
 // cand/score: per-query segments at cand_off[q]; ncand[q] = unique count.
 void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,

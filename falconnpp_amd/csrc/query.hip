@@ -4,7 +4,7 @@
 //                   global top-qProbes of that pool.  The reference gets the
 //                   same set from its k-way heap merge (:540-582): a merge of
 //                   sorted lists pops the largest values first.  Ties on a cut
-//                   are flagged for the exact replay (k_probe_exact).
+//                   are flagged for the exact replay (k_probe_exact_*).
 //  k_gather_score   per query: bucket gather + dedup in an LDS bitset window
 //                   (the reference's boost::dynamic_bitset, :552-611), the
 //                   ascending candidate list, FP32 scores q.(x - c) (:647) and
@@ -277,62 +277,98 @@ __device__ void layer_heap(const float* x, int n_proj, int log2_np, int p, IFPai
     }
 }
 
-__host__ __device__ inline int64_t exact_work_bytes(int F, int L, int PL, int perTable) {
-    return (int64_t)2 * F * 4 + (int64_t)(3 * PL + perTable + (int64_t)L * perTable + L) * 8 + (int64_t)L * 4 + 64;
+// Work area of one flagged query: the per-table probe lists, the merge heap
+// and table cursors, then one scratch slot per table.
+__host__ __device__ inline int64_t exact_table_bytes(int F, int PL, int PT) {
+    return ((int64_t)2 * F * 4 + (int64_t)(3 * PL + PT) * 8 + 15) / 16 * 16;
+}
+__host__ __device__ inline int64_t exact_head_bytes(int L, int PT) {
+    return ((int64_t)L * PT * 8 + (int64_t)L * 8 + (int64_t)L * 4 + 15) / 16 * 16;
+}
+__host__ __device__ inline int64_t exact_work_bytes(int F, int L, int PL, int PT) {
+    return exact_head_bytes(L, PT) + (int64_t)L * exact_table_bytes(F, PL, PT);
 }
 
-// src/FalconnPP.cpp:349-582 replayed literally with the emulated heaps.
-__global__ void k_probe_exact(HashGeom hg, ProbeGeom g, const float* __restrict__ Q, int64_t ldq,
-                              const uint32_t* __restrict__ masks, const uint32_t* __restrict__ qlist, uint32_t nlist,
-                              const uint64_t* __restrict__ bucket_off, uint32_t* __restrict__ probes,
-                              uint64_t* __restrict__ ctot, unsigned char* __restrict__ work, int64_t wpq) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nlist) return;
-    const uint32_t q = qlist[t];
-    const int F = hg.F, L = g.L, PL = g.PL, PT = g.perTable;
-    unsigned char* wp = work + (int64_t)t * wpq;
-    float* x1 = reinterpret_cast<float*>(wp);
+// Per (flagged query, table): the table's probe list, src/FalconnPP.cpp:358-531
+// replayed with the emulated heaps.  When no per-layer tie was seen (qflag bit
+// 0 clear), the per-layer top lists of the fast kernel are exact (distinct
+// values, largest first, as the reference pops them) and are reused;
+// otherwise the table is rotated again and its layer heaps replayed.
+__global__ void k_probe_exact_tables(HashGeom hg, ProbeGeom g, const float* __restrict__ Q, int64_t ldq,
+                                     const uint32_t* __restrict__ masks, const uint32_t* __restrict__ qlist,
+                                     uint32_t nlist, const uint32_t* __restrict__ qflag,
+                                     const int32_t* __restrict__ qc, const float* __restrict__ qvals,
+                                     unsigned char* __restrict__ work, int64_t wpq) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nlist * g.L) return;
+    const uint32_t f = (uint32_t)(t / g.L);
+    const int l = (int)(t % g.L);
+    const uint32_t q = qlist[f];
+    const int F = hg.F, PL = g.PL, PT = g.perTable;
+    unsigned char* wp = work + (int64_t)f * wpq;
+    IFPair* list = reinterpret_cast<IFPair*>(wp);
+    unsigned char* ts = wp + exact_head_bytes(g.L, PT) + (int64_t)l * exact_table_bytes(F, PL, PT);
+    float* x1 = reinterpret_cast<float*>(ts);
     float* x2 = x1 + F;
     IFPair* hs = reinterpret_cast<IFPair*>(x2 + F);
     IFPair* v1 = hs + PL;
     IFPair* v2 = v1 + PL;
     IFPair* tq = v2 + PL;
-    IFPair* list = tq + PT;
-    IFPair* mq = list + (int64_t)L * PT;
-    int* track = reinterpret_cast<int*>(mq + L);
-    const float* qv = Q + (int64_t)q * ldq;
-    for (int l = 0; l < L; ++l) {
+    if (qflag[q] & 1u) {
         const int MW = (F + 31) / 32;
+        const float* qv = Q + (int64_t)q * ldq;
         rotate_serial(qv, hg.D, F, masks + (size_t)(l * 2 + 0) * 3 * MW, x1);
         rotate_serial(qv, hg.D, F, masks + (size_t)(l * 2 + 1) * 3 * MW, x2);
         layer_heap(x1, hg.n_proj, hg.log2_nproj, PL, hs, v1);
         layer_heap(x2, hg.n_proj, hg.log2_nproj, PL, hs, v2);
-        MinHeap th;
-        th.init(tq);
-        for (int i = 0; i < PL; ++i)
-            for (int j = 0; j < PL; ++j) {
-                const int b = v1[i].idx * g.two_np + v2[j].idx;
-                const float s = v1[i].val + v2[j].val;
-                if (th.size() < PT)
-                    th.push(IFPair{b, s});
-                else if (s > th.top().val) {
-                    th.pop();
-                    th.push(IFPair{b, s});
-                }
-            }
-        int idx = PT - 1;
-        while (!th.empty()) {
-            IFPair pr = th.top();
-            th.pop();
-            pr.idx = l * g.NB + pr.idx;
-            list[(int64_t)l * PT + idx] = pr;
-            idx--;
+    } else {
+        const size_t base = ((size_t)q * g.L + l) * 2 * PL;
+        for (int i = 0; i < PL; ++i) {
+            v1[i] = IFPair{qc[base + i], qvals[base + i]};
+            v2[i] = IFPair{qc[base + PL + i], qvals[base + PL + i]};
         }
-        track[l] = 0;
     }
+    MinHeap th;
+    th.init(tq);
+    for (int i = 0; i < PL; ++i)
+        for (int j = 0; j < PL; ++j) {
+            const int b = v1[i].idx * g.two_np + v2[j].idx;
+            const float s = v1[i].val + v2[j].val;
+            if (th.size() < PT)
+                th.push(IFPair{b, s});
+            else if (s > th.top().val) {
+                th.pop();
+                th.push(IFPair{b, s});
+            }
+        }
+    int idx = PT - 1;
+    while (!th.empty()) {
+        IFPair pr = th.top();
+        th.pop();
+        pr.idx = l * g.NB + pr.idx;
+        list[(int64_t)l * PT + idx] = pr;
+        idx--;
+    }
+}
+
+// Per flagged query: the global merge of src/FalconnPP.cpp:540-582.
+__global__ void k_probe_exact_merge(ProbeGeom g, const uint32_t* __restrict__ qlist, uint32_t nlist,
+                                    const uint64_t* __restrict__ bucket_off, uint32_t* __restrict__ probes,
+                                    uint64_t* __restrict__ ctot, unsigned char* __restrict__ work, int64_t wpq) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nlist) return;
+    const uint32_t q = qlist[f];
+    const int L = g.L, PT = g.perTable;
+    unsigned char* wp = work + (int64_t)f * wpq;
+    const IFPair* list = reinterpret_cast<const IFPair*>(wp);
+    IFPair* mq = reinterpret_cast<IFPair*>(wp) + (int64_t)L * PT;
+    int* track = reinterpret_cast<int*>(mq + L);
     MaxHeap mh;
     mh.init(mq);
-    for (int l = 0; l < L; ++l) mh.push(list[(int64_t)l * PT]);
+    for (int l = 0; l < L; ++l) {
+        track[l] = 0;
+        mh.push(list[(int64_t)l * PT]);
+    }
     uint64_t ct = 0;
     uint32_t* out = probes + (size_t)q * g.qProbes;
     for (int c = 0; c < g.qProbes; ++c) {
@@ -367,7 +403,9 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
     uint32_t* tk_id = tk_key + 1024;    // kcap
     __shared__ int s_digit;
     __shared__ uint32_t s_below;
-    const int q = blockIdx.x;
+    const int q = a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
+    uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * 5 : nullptr;
+    if (ts && threadIdx.x == 0) ts[0] = wall_clock64();
     const uint32_t* pr = probes + (size_t)q * a.qProbes;
     const uint64_t cbase = cand_off[q];
     const uint64_t ccap = cand_off[q + 1] - cbase;
@@ -380,12 +418,20 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
         for (int i = threadIdx.x; i < a.qProbes; i += TB) {
             const uint32_t b = pr[i];
             const uint64_t s = bucket_off[b], e = bucket_off[b + 1];
-            for (uint64_t j = s; j < e; ++j) {
-                const int64_t rel = (int64_t)ids[j] - w0;
-                if (rel >= 0 && rel < W) atomicOr(&bits[rel >> 5], 1u << (rel & 31));
+            // four independent id loads in flight per step
+            for (uint64_t j = s; j < e; j += 4) {
+                int32_t v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = (j + u < e) ? ids[j + u] : -1;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t rel = (int64_t)v[u] - w0;
+                    if (v[u] >= 0 && rel >= 0 && rel < W) atomicOr(&bits[rel >> 5], 1u << (rel & 31));
+                }
             }
         }
         __syncthreads();
+        if (ts && threadIdx.x == 0 && w0 == 0) ts[1] = wall_clock64();
         // ordered compaction: thread t scans a contiguous run of words
         const int per = (Wwords + TB - 1) / TB;
         const int wb = threadIdx.x * per;
@@ -408,6 +454,7 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
         __syncthreads();
     }
     if (threadIdx.x == 0) ncand[q] = nc;
+    if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
     // FP32 scores: groups of 8 lanes per candidate row, CPL float4 chunks per lane
     const int g8 = threadIdx.x & 7;
     const int grp = threadIdx.x >> 3;
@@ -454,6 +501,7 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
         }
     }
     __syncthreads();
+    if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
     // top-k: radix select of the k-th smallest descending key, ties -> smaller id
     const uint32_t kk = min((uint32_t)a.k, nc);
     uint32_t prefix = 0, pmask = 0, k = kk, eqc = 0;
@@ -511,6 +559,28 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
         oq[i] = 0;
         if (osq) osq[i] = 0.0f;
     }
+    if (ts && threadIdx.x == 0) ts[4] = wall_clock64();
+}
+
+// Sort key of a query: its best bucket of table 0 (layer-1 and layer-2 top
+// codes).  Queries with equal keys share most candidates.
+__global__ void k_order_keys(const int32_t* __restrict__ qc, int nq, int L, int PL, int two_np,
+                             uint32_t* __restrict__ key) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const size_t base = (size_t)q * L * 2 * PL;
+    key[q] = (uint32_t)(qc[base] * two_np + qc[base + PL]);
+}
+
+// Block -> query map: XCD x (blocks x, x+8, ...) walks a contiguous run of the
+// sorted queries, so similar queries share that XCD's L2.
+__global__ void k_xcd_order(const uint32_t* __restrict__ sorted, int nq, int nxcd, uint32_t* __restrict__ order) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nq) return;
+    const int x = b % nxcd, j = b / nxcd;
+    int start = 0;
+    for (int y = 0; y < x; ++y) start += (nq - y + nxcd - 1) / nxcd;
+    order[b] = sorted[start + j];
 }
 
 template <int TB>
@@ -551,18 +621,34 @@ void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const fl
     FPP_HIP(hipGetLastError());
 }
 
+void launch_order_keys(const int32_t* qc, int nq, int L, int PL, int two_np, uint32_t* key, hipStream_t st) {
+    if (nq == 0) return;
+    k_order_keys<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(qc, nq, L, PL, two_np, key);
+    FPP_HIP(hipGetLastError());
+}
+
+void launch_xcd_order(const uint32_t* sorted, int nq, int nxcd, uint32_t* order, hipStream_t st) {
+    if (nq == 0) return;
+    k_xcd_order<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(sorted, nq, nxcd, order);
+    FPP_HIP(hipGetLastError());
+}
+
 int64_t probe_exact_work_bytes(const HashGeom& hg, const ProbeGeom& g) {
     return (exact_work_bytes(hg.F, g.L, g.PL, g.perTable) + 255) / 256 * 256;
 }
 
 void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, int64_t ldq,
-                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, const uint64_t* bucket_off,
-                        uint32_t* probes, uint64_t* ctot, void* work, int64_t work_per_query, hipStream_t st) {
+                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, const uint32_t* qflag,
+                        const int32_t* qc, const float* qv, const uint64_t* bucket_off, uint32_t* probes,
+                        uint64_t* ctot, void* work, int64_t work_per_query, hipStream_t st) {
     if (nlist == 0) return;
-    k_probe_exact<<<(unsigned)ceil_div(nlist, 64), 64, 0, st>>>(hg, g, Q, ldq, masks, qlist, nlist, bucket_off,
-                                                                probes, ctot,
-                                                                reinterpret_cast<unsigned char*>(work),
-                                                                work_per_query);
+    unsigned char* w = reinterpret_cast<unsigned char*>(work);
+    const int64_t nt = (int64_t)nlist * g.L;
+    k_probe_exact_tables<<<(unsigned)ceil_div(nt, 64), 64, 0, st>>>(hg, g, Q, ldq, masks, qlist, nlist, qflag, qc, qv,
+                                                                   w, work_per_query);
+    FPP_HIP(hipGetLastError());
+    k_probe_exact_merge<<<(unsigned)ceil_div(nlist, 64), 64, 0, st>>>(g, qlist, nlist, bucket_off, probes, ctot, w,
+                                                                    work_per_query);
     FPP_HIP(hipGetLastError());
 }
 

@@ -21,8 +21,9 @@ EXPORTS = (
     "fpp_clear", "fpp_build", "fpp_build_device", "fpp_query", "fpp_query_device", "fpp_get_shape",
     "fpp_get_center", "fpp_get_sign_bits", "fpp_get_buckets", "fpp_hash_points", "fpp_get_stats",
     "fpp_get_last_probes", "fpp_get_last_candidates", "fpp_bruteforce_f64", "fpp_set_profiling",
-    "fpp_get_timings",
+    "fpp_get_timings", "fpp_set_tuning",
 )
+TUNE_QUERY_ORDER, TUNE_GATHER_THREADS = 1, 2
 
 FPP_EINVAL, FPP_ESTATE, FPP_EHIP, FPP_ENOMEM, FPP_EINTERNAL = 1, 2, 3, 4, 5
 
@@ -66,6 +67,7 @@ def load():
     L.fpp_get_last_probes.argtypes = [vp, vp]
     L.fpp_set_profiling.argtypes = [vp, i32]
     L.fpp_get_timings.argtypes = [vp, vp]
+    L.fpp_set_tuning.argtypes = [vp, i32, i32]
     L.fpp_get_last_candidates.restype = i64
     L.fpp_get_last_candidates.argtypes = [vp, i32, vp, i64]
     L.fpp_bruteforce_f64.argtypes = [vp, i64, i64, i32, vp, i64, i32, i32, vp, vp, vp]
@@ -183,14 +185,20 @@ class Index:
         return dict(hash_ties=int(s[0]), bucket_ties=int(s[1]), ctot=int(s[2]), cuniq=int(s[3]),
                     query_ties=int(s[4]), nq=int(s[5]))
 
-    def set_profiling(self, on: bool):
-        _check(load().fpp_set_profiling(self.h, 1 if on else 0))
+    def set_profiling(self, level: int):
+        """0 off, 1 HIP events per query stage, 2 also per-query phase stamps."""
+        _check(load().fpp_set_profiling(self.h, int(level)))
 
     def timings(self):
-        """Device ms of the last query: hash, probe select, tie replay, gather+rerank."""
-        t = np.zeros(4, np.float64)
+        """Device ms of the last query (hash, probe select, tie replay, gather+rerank)
+        and mean per-query us of the gather kernel's phases."""
+        t = np.zeros(8, np.float64)
         _check(load().fpp_get_timings(self.h, _ptr(t)))
-        return dict(hash=t[0], probe=t[1], replay=t[2], gather=t[3])
+        return dict(hash=t[0], probe=t[1], replay=t[2], gather=t[3], ph_gather_us=t[4], ph_compact_us=t[5],
+                    ph_score_us=t[6], ph_topk_us=t[7])
+
+    def set_tuning(self, which: int, value: int):
+        _check(load().fpp_set_tuning(self.h, which, value))
 
     def last_probes(self, nq: int) -> np.ndarray:
         out = np.zeros((nq, self.qprobes), np.uint32)

@@ -92,11 +92,19 @@ int fpp_hash_points(fpp_index* h, const float* V, int64_t ld, int m, uint32_t* c
  * for ties, [2]=sum C_tot of the last query call, [3]=sum C_uniq, [4]=queries
  * replayed for ties, [5]=queries in the last call */
 int fpp_get_stats(fpp_index* h, int64_t* out);
-/* per-stage device time of the last query call when profiling is on (HIP
- * events on the call's stream): ms[0]=query hashing, [1]=probe selection,
- * [2]=tie replay, [3]=gather + dedup + rerank + top-k */
+/* per-stage device time of the last query call when profiling is on
+ * (on >= 1: HIP events on the call's stream): ms[0]=query hashing,
+ * [1]=probe selection, [2]=tie replay, [3]=gather + dedup + rerank + top-k;
+ * with on >= 2 also the mean per-query microseconds of that last kernel's
+ * phases (device wall clock): [4]=bucket gather + dedup, [5]=candidate
+ * compaction, [6]=scoring, [7]=top-k.  ms must hold 8 doubles. */
 int fpp_set_profiling(fpp_index* h, int on);
 int fpp_get_timings(fpp_index* h, double* ms);
+/* launch-shape knobs (no reference counterpart): FPP_TUNE_QUERY_ORDER 0/1
+ * (process queries grouped by their best table-0 bucket, XCD-aware),
+ * FPP_TUNE_GATHER_THREADS 256/512/1024 (workgroup of the gather kernel) */
+enum { FPP_TUNE_QUERY_ORDER = 1, FPP_TUNE_GATHER_THREADS = 2 };
+int fpp_set_tuning(fpp_index* h, int which, int value);
 /* probes of the last query call (nq x qProbes global bucket ids, as a set per
  * query) and, for query j, its unique candidates (ascending; returns count) */
 int fpp_get_last_probes(fpp_index* h, uint32_t* out);

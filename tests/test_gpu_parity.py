@@ -231,3 +231,23 @@ def test_bruteforce_f64_matches_oracle():
     assert np.array_equal(ids, oi)
     r = E.recall(ids, seeds, k)
     assert 0.0 <= r <= 1.0
+
+
+def test_query_order_and_launch_shape_do_not_change_results():
+    X, Q, g, o = _built("mix_d200_p3")
+    g.set_qProbes(300)
+    ref = g.query(Q, 20)
+    g.set_tuning(E.TUNE_QUERY_ORDER, 0)
+    try:
+        assert np.array_equal(g.query(Q, 20), ref)
+    finally:
+        g.set_tuning(E.TUNE_QUERY_ORDER, 1)
+    for t in (256, 1024):
+        g.set_tuning(E.TUNE_GATHER_THREADS, t)
+        assert np.array_equal(g.query(Q, 20), ref), t
+    g.set_tuning(E.TUNE_GATHER_THREADS, 512)
+    g.set_profiling(2)
+    assert np.array_equal(g.query(Q, 20), ref)
+    t = g.timings()
+    assert t["gather"] > 0 and t["ph_score_us"] > 0
+    g.set_profiling(0)
