@@ -351,36 +351,56 @@ __global__ void k_probe_exact_tables(HashGeom hg, ProbeGeom g, const float* __re
     }
 }
 
-// Per flagged query: the global merge of src/FalconnPP.cpp:540-582.
-__global__ void k_probe_exact_merge(ProbeGeom g, const uint32_t* __restrict__ qlist, uint32_t nlist,
-                                    const uint64_t* __restrict__ bucket_off, uint32_t* __restrict__ probes,
-                                    uint64_t* __restrict__ ctot, unsigned char* __restrict__ work, int64_t wpq) {
-    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nlist) return;
+// Per flagged query: the global merge of src/FalconnPP.cpp:540-582.  The
+// per-table lists are staged in LDS by the workgroup; one lane then replays
+// the max-heap merge out of LDS (heap and cursors in LDS too).
+constexpr int MG_T = 256;
+__global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const uint32_t* __restrict__ qlist,
+                                                            const uint64_t* __restrict__ bucket_off,
+                                                            uint32_t* __restrict__ probes, uint64_t* __restrict__ ctot,
+                                                            const unsigned char* __restrict__ work, int64_t wpq) {
+    extern __shared__ IFPair mlds[];
+    const uint32_t f = blockIdx.x;
     const uint32_t q = qlist[f];
     const int L = g.L, PT = g.perTable;
-    unsigned char* wp = work + (int64_t)f * wpq;
-    const IFPair* list = reinterpret_cast<const IFPair*>(wp);
-    IFPair* mq = reinterpret_cast<IFPair*>(wp) + (int64_t)L * PT;
+    const IFPair* list_g = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
+    IFPair* list = mlds;
+    IFPair* mq = list + (int64_t)L * PT;
     int* track = reinterpret_cast<int*>(mq + L);
-    MaxHeap mh;
-    mh.init(mq);
-    for (int l = 0; l < L; ++l) {
-        track[l] = 0;
-        mh.push(list[(int64_t)l * PT]);
-    }
-    uint64_t ct = 0;
+    for (int64_t i = threadIdx.x; i < (int64_t)L * PT; i += MG_T) list[i] = list_g[i];
+    for (int l = threadIdx.x; l < L; l += MG_T) track[l] = 0;
+    __syncthreads();
     uint32_t* out = probes + (size_t)q * g.qProbes;
-    for (int c = 0; c < g.qProbes; ++c) {
-        const IFPair pr = mh.top();
-        mh.pop();
-        out[c] = (uint32_t)pr.idx;
-        ct += bucket_off[pr.idx + 1] - bucket_off[pr.idx];
-        const int tb = pr.idx / g.NB;
-        track[tb]++;
-        if (track[tb] < PT) mh.push(list[(int64_t)tb * PT + track[tb]]);
+    if (threadIdx.x == 0) {
+        MaxHeap mh;
+        mh.init(mq);
+        for (int l = 0; l < L; ++l) mh.push(list[(int64_t)l * PT]);
+        for (int c = 0; c < g.qProbes; ++c) {
+            const IFPair pr = mh.top();
+            mh.pop();
+            out[c] = (uint32_t)pr.idx;
+            const int tb = pr.idx / g.NB;
+            const int tk = ++track[tb];
+            if (tk < PT) mh.push(list[(int64_t)tb * PT + tk]);
+        }
     }
-    ctot[q] = ct;
+    __syncthreads();
+    // C_tot of the probed set
+    __shared__ unsigned long long s_ct;
+    if (threadIdx.x == 0) s_ct = 0;
+    __syncthreads();
+    unsigned long long ct = 0;
+    for (int c = threadIdx.x; c < g.qProbes; c += MG_T) {
+        const uint32_t b = out[c];
+        ct += bucket_off[b + 1] - bucket_off[b];
+    }
+    atomicAdd(&s_ct, ct);
+    __syncthreads();
+    if (threadIdx.x == 0) ctot[q] = s_ct;
+}
+
+size_t merge_lds_bytes(const ProbeGeom& g) {
+    return (size_t)g.L * g.perTable * sizeof(IFPair) + (size_t)g.L * sizeof(IFPair) + (size_t)g.L * sizeof(int);
 }
 
 // ---- gather + dedup + FP32 rerank + top-k ---------------------------------
@@ -397,7 +417,7 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
     const int64_t W = a.window_bits;
     const int Wwords = (int)(W / 32);
     uint32_t* bits = lds;
-    uint32_t* sh = bits + Wwords;       // TB
+    uint32_t* sh = bits + ((Wwords + 1) & ~1);  // TB (8-byte aligned from here on)
     uint32_t* hist = sh + TB;           // 256
     uint32_t* tk_key = hist + 256;      // kcap
     uint32_t* tk_id = tk_key + 1024;    // kcap
@@ -455,10 +475,15 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
     }
     if (threadIdx.x == 0) ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
-    // FP32 scores: groups of 8 lanes per candidate row, CPL float4 chunks per lane
+    // FP32 scores: groups of 8 lanes per candidate row, CPL float4 chunks per
+    // lane, U rows per group in flight, candidate ids fetched one step ahead.
+    constexpr int U = 4;
+    const int lane = threadIdx.x & (WAVE - 1);
     const int g8 = threadIdx.x & 7;
     const int grp = threadIdx.x >> 3;
+    const int wv = threadIdx.x / WAVE;
     constexpr int NG = TB / 8;
+    constexpr int STEP = U * NG;
     const int64_t nchunk = a.ldx / 4;
     const float4* q4 = reinterpret_cast<const float4*>(Q + (int64_t)q * ldq);
     float4 qr[CPL];
@@ -467,93 +492,147 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
         const int64_t ch = g8 + 8 * t;
         qr[t] = ch < nchunk ? q4[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    for (uint32_t i0 = 0; i0 < nc; i0 += 2 * NG) {
-        const uint32_t ia = i0 + grp, ib = i0 + NG + grp;
-        const bool va = ia < nc, vb = ib < nc;
-        const float4* ra = reinterpret_cast<const float4*>(Xc + (int64_t)(va ? cq[ia] : 0) * a.ldx);
-        const float4* rb = reinterpret_cast<const float4*>(Xc + (int64_t)(vb ? cq[ib] : 0) * a.ldx);
-        float sa = 0.f, sb = 0.f;
+    // wave-level running top-k (k <= 64): lane j holds the wave's j-th best
+    // key (desc_key(score) << 32 | id), ascending; misses cost one compare
+    const bool wsel = a.k <= WAVE;
+    uint64_t wk = ~0ull;
+    uint64_t thr = ~0ull;
+    int idn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t ii = (uint32_t)(u * NG + grp);
+        idn[u] = ii < nc ? cq[ii] : -1;
+    }
+    for (uint32_t i0 = 0; i0 < nc; i0 += STEP) {
+        int id[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            id[u] = idn[u];
+            const uint32_t ii = i0 + STEP + (uint32_t)(u * NG + grp);
+            idn[u] = ii < nc ? cq[ii] : -1;
+        }
+        float sc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) sc[u] = 0.f;
 #pragma unroll
         for (int t = 0; t < CPL; ++t) {
             const int64_t ch = g8 + 8 * t;
             if (ch < nchunk) {
-                const float4 xa = ra[ch];
-                const float4 xb = rb[ch];
-                sa = fmaf(qr[t].x, xa.x, sa);
-                sa = fmaf(qr[t].y, xa.y, sa);
-                sa = fmaf(qr[t].z, xa.z, sa);
-                sa = fmaf(qr[t].w, xa.w, sa);
-                sb = fmaf(qr[t].x, xb.x, sb);
-                sb = fmaf(qr[t].y, xb.y, sb);
-                sb = fmaf(qr[t].z, xb.z, sb);
-                sb = fmaf(qr[t].w, xb.w, sb);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const float4 x = reinterpret_cast<const float4*>(Xc + (int64_t)(id[u] < 0 ? 0 : id[u]) * a.ldx)[ch];
+                    sc[u] = fmaf(qr[t].x, x.x, sc[u]);
+                    sc[u] = fmaf(qr[t].y, x.y, sc[u]);
+                    sc[u] = fmaf(qr[t].z, x.z, sc[u]);
+                    sc[u] = fmaf(qr[t].w, x.w, sc[u]);
+                }
             }
         }
-        sa += __shfl_xor(sa, 1, WAVE);
-        sb += __shfl_xor(sb, 1, WAVE);
-        sa += __shfl_xor(sa, 2, WAVE);
-        sb += __shfl_xor(sb, 2, WAVE);
-        sa += __shfl_xor(sa, 4, WAVE);
-        sb += __shfl_xor(sb, 4, WAVE);
-        if (g8 == 0) {
-            if (va) sq[ia] = sa;
-            if (vb) sq[ib] = sb;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            sc[u] += __shfl_xor(sc[u], 1, WAVE);
+            sc[u] += __shfl_xor(sc[u], 2, WAVE);
+            sc[u] += __shfl_xor(sc[u], 4, WAVE);
+        }
+        if (!wsel) {
+            if (g8 == 0) {
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (id[u] >= 0) sq[i0 + u * NG + grp] = sc[u];
+            }
+            continue;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t key = id[u] < 0 ? ~0ull : (((uint64_t)desc_key_any(sc[u]) << 32) | (uint32_t)id[u]);
+            // the wave's 8 groups, one candidate each, in group order
+            uint64_t better = __ballot(key < thr) & 0x0101010101010101ull;
+            while (better) {
+                const int src = (int)__ffsll((long long)better) - 1;
+                better &= better - 1;
+                const uint64_t nk = __shfl(key, src, WAVE);
+                if (nk >= thr) continue;
+                const int pos = (int)__popcll(__ballot(wk < nk));
+                const uint64_t up = __shfl_up(wk, 1, WAVE);
+                wk = lane > pos ? up : (lane == pos ? nk : wk);
+                thr = __shfl(wk, a.k - 1, WAVE);
+            }
         }
     }
     __syncthreads();
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
-    // top-k: radix select of the k-th smallest descending key, ties -> smaller id
-    const uint32_t kk = min((uint32_t)a.k, nc);
-    uint32_t prefix = 0, pmask = 0, k = kk, eqc = 0;
-    if (kk > 0) {
-        for (int shift = 24; shift >= 0; shift -= 8) {
-            for (int i = threadIdx.x; i < 256; i += TB) hist[i] = 0;
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < nc; i += TB) {
-                const uint32_t key = desc_key_any(sq[i]);
-                if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
-            }
-            __syncthreads();
-            int d;
-            uint32_t below;
-            block_find_digit<TB>(hist, k, sh, d, below, eqc, &s_digit, &s_below);
-            prefix |= (uint32_t)d << shift;
-            pmask |= 255u << shift;
-            k -= below;
-        }
-    }
-    const uint32_t T = prefix;
-    uint32_t written = 0, eq_taken = 0;
-    for (uint32_t c0 = 0; c0 < nc && kk > 0; c0 += TB) {
-        const uint32_t i = c0 + threadIdx.x;
-        const uint32_t key = i < nc ? desc_key_any(sq[i]) : 0xffffffffu;
-        const uint32_t lt = (i < nc && key < T) ? 1u : 0u;
-        const uint32_t eqk = (i < nc && key == T) ? 1u : 0u;
-        uint32_t tot_eq;
-        const uint32_t ex_eq = block_excl_scan_u32<TB>(eqk, sh, tot_eq);
-        const uint32_t take = lt | ((eqk && eq_taken + ex_eq < k) ? 1u : 0u);
-        uint32_t tot_take;
-        const uint32_t ex_take = block_excl_scan_u32<TB>(take, sh, tot_take);
-        if (take && written + ex_take < kk) {
-            tk_key[written + ex_take] = key;
-            tk_id[written + ex_take] = (uint32_t)cq[i];
-        }
-        written += tot_take;
-        eq_taken += min(tot_eq, k - min(k, eq_taken));
-    }
-    __syncthreads();
-    // order the kept entries by (key, id): rank by counting (kk <= 1024)
     int32_t* oq = out_ids + (size_t)q * a.k;
     float* osq = out_scores ? out_scores + (size_t)q * a.k : nullptr;
-    for (uint32_t i = threadIdx.x; i < kk; i += TB) {
-        const uint32_t ki = tk_key[i], ii = tk_id[i];
-        uint32_t r = 0;
-        for (uint32_t j = 0; j < kk; ++j) {
-            const uint32_t kj = tk_key[j], ij = tk_id[j];
-            r += (kj < ki || (kj == ki && ij < ii)) ? 1u : 0u;
+    const uint32_t kk = min((uint32_t)a.k, nc);
+    if (wsel) {
+        // merge the waves' lists: rank each entry among all of them
+        constexpr int NW = TB / WAVE;
+        uint64_t* wl = reinterpret_cast<uint64_t*>(tk_key);  // NW * k entries (<= 1024 x 8 B)
+        if (lane < a.k) wl[wv * a.k + lane] = wk;
+        __syncthreads();
+        const int tot = NW * a.k;
+        for (int e = threadIdx.x; e < tot; e += TB) {
+            const uint64_t ke = wl[e];
+            if (ke == ~0ull) continue;
+            uint32_t r = 0;
+            for (int o = 0; o < tot; ++o) r += wl[o] < ke ? 1u : 0u;
+            if (r < kk) {
+                oq[r] = (int32_t)(uint32_t)(ke & 0xffffffffu);
+                if (osq) osq[r] = desc_key_any_value((uint32_t)(ke >> 32));
+            }
         }
-        oq[r] = (int32_t)ii;
-        if (osq) osq[r] = desc_key_any_value(ki);
+    } else {
+        __syncthreads();
+        // radix select of the k-th smallest descending key, ties -> smaller id
+        uint32_t prefix = 0, pmask = 0, k = kk, eqc = 0;
+        if (kk > 0) {
+            for (int shift = 24; shift >= 0; shift -= 8) {
+                for (int i = threadIdx.x; i < 256; i += TB) hist[i] = 0;
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < nc; i += TB) {
+                    const uint32_t key = desc_key_any(sq[i]);
+                    if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+                }
+                __syncthreads();
+                int d;
+                uint32_t below;
+                block_find_digit<TB>(hist, k, sh, d, below, eqc, &s_digit, &s_below);
+                prefix |= (uint32_t)d << shift;
+                pmask |= 255u << shift;
+                k -= below;
+            }
+        }
+        const uint32_t T = prefix;
+        uint32_t written = 0, eq_taken = 0;
+        for (uint32_t c0 = 0; c0 < nc && kk > 0; c0 += TB) {
+            const uint32_t i = c0 + threadIdx.x;
+            const uint32_t key = i < nc ? desc_key_any(sq[i]) : 0xffffffffu;
+            const uint32_t lt = (i < nc && key < T) ? 1u : 0u;
+            const uint32_t eqk = (i < nc && key == T) ? 1u : 0u;
+            uint32_t tot_eq;
+            const uint32_t ex_eq = block_excl_scan_u32<TB>(eqk, sh, tot_eq);
+            const uint32_t take = lt | ((eqk && eq_taken + ex_eq < k) ? 1u : 0u);
+            uint32_t tot_take;
+            const uint32_t ex_take = block_excl_scan_u32<TB>(take, sh, tot_take);
+            if (take && written + ex_take < kk) {
+                tk_key[written + ex_take] = key;
+                tk_id[written + ex_take] = (uint32_t)cq[i];
+            }
+            written += tot_take;
+            eq_taken += min(tot_eq, k - min(k, eq_taken));
+        }
+        __syncthreads();
+        // order the kept entries by (key, id): rank by counting (kk <= 1024)
+        for (uint32_t i = threadIdx.x; i < kk; i += TB) {
+            const uint32_t ki = tk_key[i], ii = tk_id[i];
+            uint32_t r = 0;
+            for (uint32_t j = 0; j < kk; ++j) {
+                const uint32_t kj = tk_key[j], ij = tk_id[j];
+                r += (kj < ki || (kj == ki && ij < ii)) ? 1u : 0u;
+            }
+            oq[r] = (int32_t)ii;
+            if (osq) osq[r] = desc_key_any_value(ki);
+        }
     }
     for (uint32_t i = kk + threadIdx.x; i < (uint32_t)a.k; i += TB) {
         oq[i] = 0;
@@ -590,7 +669,7 @@ void dispatch_gather_cpl(const GatherArgs& a, int nq, const uint32_t* probes, co
                          hipStream_t st) {
     const int64_t nchunk = a.ldx / 4;
     const int cpl = (int)ceil_div(nchunk, 8);
-    const size_t lds = (size_t)(a.window_bits / 32 + TB + 256 + 2048) * 4;
+    const size_t lds = (size_t)(a.window_bits / 32 + 1 + TB + 256 + 2048) * 4;
 #define FPP_G(C) \
     k_gather_score_f32<TB, C><<<nq, TB, lds, st>>>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids, out_scores)
     if (cpl <= 1)
@@ -647,8 +726,9 @@ void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, 
     k_probe_exact_tables<<<(unsigned)ceil_div(nt, 64), 64, 0, st>>>(hg, g, Q, ldq, masks, qlist, nlist, qflag, qc, qv,
                                                                    w, work_per_query);
     FPP_HIP(hipGetLastError());
-    k_probe_exact_merge<<<(unsigned)ceil_div(nlist, 64), 64, 0, st>>>(g, qlist, nlist, bucket_off, probes, ctot, w,
-                                                                    work_per_query);
+    const size_t lds = merge_lds_bytes(g);
+    FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "probe lists (n_tables * perTable) too large for the exact merge");
+    k_probe_exact_merge<<<nlist, MG_T, lds, st>>>(g, qlist, bucket_off, probes, ctot, w, work_per_query);
     FPP_HIP(hipGetLastError());
 }
 

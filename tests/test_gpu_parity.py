@@ -36,6 +36,8 @@ CASES = {
     "mix_d128_p2": (lambda: synth.gaussian_mixture(12000, 128, 60, seed=13), 8, 128, 10, 0.2, 2, 160, 10, 50),
     "nproj16_d40": (lambda: synth.lognormal_sphere(5000, 40, 0.3, seed=14), 5, 16, 8, 0.05, 2, 60, 15, 40),
     "int_grid_ties": (lambda: _sym_int_grid(3000, 24, -2, 2, 15), 4, 32, 5, 0.3, 2, 40, 10, 40),
+    # N above the 2^20-id LDS bitset window: the dedup runs over two windows
+    "multiwindow_n1.2m": (lambda: synth.gaussian_mixture(1_200_000, 16, 300, seed=16), 2, 16, 30, 0.05, 1, 8, 20, 12),
 }
 
 _cache = {}
