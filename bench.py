@@ -46,6 +46,7 @@ def main():
     ap.add_argument("--qprobes", type=int, default=0, help="override qProbes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="skip the BF16/FP8 + refine runs")
     ap.add_argument("--cpu-queries", type=int, default=1000)
     ap.add_argument("--cpu-tables", type=int, default=16)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -135,16 +136,42 @@ def main():
                                                              "ph_topk_us")})
 
     res = {}
+    ans32 = out.cpu().numpy()
+    truth = None
     if not args.no_recall:
         t = time.perf_counter()
-        ans = out.cpu().numpy()
-        truth, _ = E.bruteforce_f64_device(dX.data_ptr(), d, n, d, dQ.data_ptr(), d, nq, k, ans)
-        res["recall_at_k"] = round(E.recall(truth, ans, k), 4)
+        truth, _ = E.bruteforce_f64_device(dX.data_ptr(), d, n, d, dQ.data_ptr(), d, nq, k, ans32)
+        res["recall_at_k"] = round(E.recall(truth, ans32, k), 4)
         log(f"recall@{k} = {res['recall_at_k']} (FP64 brute force {time.perf_counter() - t:.1f}s)")
+
+    variants = {}
+    if not args.no_variants:
+        for name, mode in (("bf16_refine", E.PREC_BF16), ("fp8_refine", E.PREC_FP8)):
+            refine = 4 * k
+            idx.set_precision(mode, refine)
+            for _ in range(max(1, args.warmup)):
+                idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
+            torch.cuda.synchronize()
+            vstage = []
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
+                vstage.append(idx.timings()["gather"])
+            torch.cuda.synchronize()
+            vdt = time.perf_counter() - t0
+            v = {"value": round(nq * args.steps / vdt, 1), "ms_per_step": round(vdt / args.steps * 1e3, 3),
+                 "refine": refine, "gather_ms": round(float(np.mean(vstage)), 3),
+                 "speedup_vs_fp32": round((nq * args.steps / vdt) / qps, 3)}
+            if truth is not None:
+                v["recall_at_k"] = round(E.recall(truth, out.cpu().numpy(), k), 4)
+                v["score_rel_err"] = approx_score_error(X, Q, idx.center(), truth, mode)
+            variants[name] = v
+            log(f"{name}: {v}")
+        idx.set_precision(E.PREC_FP32, 0)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args, X, Q, cfg, idx, out.cpu().numpy())
+        cpu = cpu_baseline(args, X, Q, cfg, idx, ans32)
 
     line = {
         "metric": METRIC,
@@ -173,9 +200,47 @@ def main():
         "stage_ms": stages,
         "candidates_per_query": {"c_tot": round(st["ctot"] / nq, 1), "c_uniq": round(st["cuniq"] / nq, 1)},
         "roofline": roofline,
+        "variants": variants,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def _bf16(a):
+    import numpy as np
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return (u & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+
+
+def _e4m3(a):
+    """Round to the nearest e4m3 value (3 mantissa bits, min normal 2^-6), ties to even."""
+    import numpy as np
+    a = np.asarray(a, np.float64)
+    e = np.floor(np.log2(np.maximum(np.abs(a), 2.0 ** -6)))
+    step = 2.0 ** (e - 3)
+    return np.round(a / step) * step
+
+
+def approx_score_error(X, Q, center, truth, mode, sample=200):
+    """Relative error of the reduced-precision scores of the true top-k rows
+    against exact FP64 scores q.(x - c), on a sample of queries (host
+    emulation of the engine's rounding)."""
+    import numpy as np
+    errs = []
+    for j in range(min(sample, len(Q))):
+        rows = (X[truth[j]] - center).astype(np.float32)
+        q = Q[j].astype(np.float32)
+        exact = rows.astype(np.float64) @ q.astype(np.float64)
+        if mode == 1:
+            approx = _bf16(rows).astype(np.float64) @ _bf16(q).astype(np.float64)
+        else:
+            sr = np.maximum(np.abs(rows).max(1), 1e-30) / 240.0
+            sq = max(float(np.abs(q).max()), 1e-30) / 240.0
+            approx = (_e4m3(rows / sr[:, None]) @ _e4m3(q / sq)) * sr * sq
+        errs.append(np.abs(approx - exact) / np.maximum(np.abs(exact), 1e-12))
+    e = np.concatenate(errs)
+    return {"mean": round(float(e.mean()), 6), "p99": round(float(np.quantile(e, 0.99)), 6)}
 
 
 def cpu_baseline(args, X, Q, cfg, idx, gpu_ans):

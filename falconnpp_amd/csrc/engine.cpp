@@ -92,6 +92,17 @@ struct Index {
 
     int64_t stat_hash_ties = 0, stat_bucket_ties = 0, stat_ctot = 0, stat_cuniq = 0, stat_qties = 0;
 
+    // reduced-precision scoring: 0 = FP32 (reference semantics), 1 = BF16,
+    // 2 = FP8-e4m3 + per-row scale; survivors rescored exactly in FP32
+    int precision = 0;
+    int refine = 0;  // 0: 4 * n_neighbors
+    int lp_ready = 0;  // format of the current low-precision copy
+    int64_t ldl = 0;
+    DevBuf<uint16_t> Xb;
+    DevBuf<uint8_t> X8;
+    DevBuf<float> s8;
+    int refine_count(int k) const { return refine > 0 ? refine : std::min(256, 4 * k); }
+
     // optional per-stage timing of query calls (HIP events on the call's stream)
     bool profiling = false;   // HIP events around the query stages
     bool phase_stamps = false;  // plus per-query phase stamps in the gather kernel
@@ -193,6 +204,7 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
     const int D = I.n_features, L = I.n_tables, p = I.iProbes, NB = I.NB();
     FPP_REQUIRE(ld >= D, FPP_EINVAL, "leading dimension of X must be >= n_features");
     I.built = false;
+    I.lp_ready = 0;
     make_sign_bits(I);
     // centre (deterministic FP64 chunk sums) and the centred copy
     I.ldx = round_up(D, 4);
@@ -343,6 +355,26 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
     I.built = true;
 }
 
+// Low-precision copy of the centred data for the selected format, made once
+// per build.
+void ensure_lowprec(Index& I, hipStream_t st) {
+    if (I.lp_ready == I.precision) return;
+    const int64_t N = I.n_points;
+    I.ldl = round_up(I.n_features, 32);
+    if (I.precision == 1) {
+        I.X8.reset();
+        I.s8.reset();
+        I.Xb.reserve((size_t)N * I.ldl);
+        launch_quant_bf16(I.Xc.get(), I.ldx, N, I.n_features, I.ldl, I.Xb.get(), st);
+    } else {
+        I.Xb.reset();
+        I.X8.reserve((size_t)N * I.ldl);
+        I.s8.reserve(N);
+        launch_quant_fp8(I.Xc.get(), I.ldx, N, I.n_features, I.ldl, I.X8.get(), I.s8.get(), st);
+    }
+    I.lp_ready = I.precision;
+}
+
 struct QueryPlan {
     int perTable, perLayer;
 };
@@ -361,6 +393,8 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     FPP_REQUIRE(nq >= 0, FPP_EINVAL, "number of queries must be >= 0");
     FPP_REQUIRE(k >= 1 && k <= 1024, FPP_EINVAL, "n_neighbors must be in [1, 1024]");
     FPP_REQUIRE(I.qProbes >= 1, FPP_EINVAL, "qProbes must be >= 1");
+    FPP_REQUIRE(I.precision == 0 || k <= I.refine_count(k), FPP_EINVAL,
+                "n_neighbors exceeds the refine count of the reduced-precision path");
     FPP_REQUIRE(ldq >= I.n_features, FPP_EINVAL, "leading dimension of Q must be >= n_features");
     const QueryPlan pl = plan_query(I);
     FPP_REQUIRE(pl.perLayer <= I.n_proj, FPP_EINVAL,
@@ -427,7 +461,10 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     ga.ldx = I.ldx;
     ga.qProbes = I.qProbes;
     ga.k = k;
-    ga.window_bits = std::min<int64_t>(round_up(I.n_points, 32), (int64_t)1 << 20);
+    const bool lowp = I.precision != 0;
+    ga.window_bits = gather_window_bits(I.n_points, I.qProbes,
+                                        lowp ? lowprec_extra_words(I.refine_count(k))
+                                             : (int64_t)I.gather_threads + 256 + 2048);
     if (I.order_queries && nq > 1) {
         const int bits = ilog2_exact(I.NB());
         const int64_t rh = radix_hist_elems(nq);
@@ -450,10 +487,24 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         I.phase_ts.reserve((size_t)nq * 5);
         ga.phase_ts = I.phase_ts.get();
     }
+    if (lowp) ensure_lowprec(I, st);
     I.mark(5, st);
-    launch_gather_score_f32(ga, nq, I.probes.get(), I.bucket_off.get(), I.ids.get(), I.Xc.get(), I.Qpad.get(),
-                            I.ldx, I.cand_off.get(), I.cand.get(), I.score.get(), I.ncand.get(), d_out, d_scores,
-                            I.gather_threads, st);
+    if (lowp) {
+        LowPrecArgs lp;
+        lp.format = I.precision;
+        lp.refine = I.refine_count(k);
+        lp.ldl = I.ldl;
+        lp.xb = I.Xb.get();
+        lp.x8 = I.X8.get();
+        lp.scale = I.s8.get();
+        launch_gather_score_lp(ga, lp, nq, I.probes.get(), I.bucket_off.get(), I.ids.get(), I.Xc.get(),
+                               I.Qpad.get(), I.ldx, I.cand_off.get(), I.cand.get(), I.ncand.get(), d_out, d_scores,
+                               st);
+    } else {
+        launch_gather_score_f32(ga, nq, I.probes.get(), I.bucket_off.get(), I.ids.get(), I.Xc.get(), I.Qpad.get(),
+                                I.ldx, I.cand_off.get(), I.cand.get(), I.score.get(), I.ncand.get(), d_out, d_scores,
+                                I.gather_threads, st);
+    }
     I.mark(6, st);
     std::vector<uint64_t> nc(nq);
     FPP_HIP(hipMemcpyAsync(nc.data(), I.ncand.get(), (size_t)nq * 8, hipMemcpyDeviceToHost, st));
@@ -562,6 +613,10 @@ int fpp_clear(fpp_index* h) {
         I.center.reset();
         I.bit1.clear();
         I.bit2.clear();
+        I.Xb.reset();
+        I.X8.reset();
+        I.s8.reset();
+        I.lp_ready = 0;
         I.n_ids = 0;
         I.built = false;
     });
@@ -735,6 +790,16 @@ int fpp_get_timings(fpp_index* h, double* ms) {
     return guard([&] {
         FPP_REQUIRE(h && ms, FPP_EINVAL, "null argument");
         for (int i = 0; i < 8; ++i) ms[i] = h->I.stage_ms[i];
+    });
+}
+
+int fpp_set_precision(fpp_index* h, int mode, int refine) {
+    return guard([&] {
+        FPP_REQUIRE(h, FPP_EINVAL, "null argument");
+        FPP_REQUIRE(mode >= 0 && mode <= 2, FPP_EINVAL, "precision mode must be 0 (FP32), 1 (BF16) or 2 (FP8)");
+        FPP_REQUIRE(refine >= 0 && refine <= 256, FPP_EINVAL, "refine count must be in [0, 256]");
+        h->I.precision = mode;
+        h->I.refine = refine;
     });
 }
 

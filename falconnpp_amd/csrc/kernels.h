@@ -100,12 +100,33 @@ struct GatherArgs {
 // Query order for cache reuse: sort key per query, then the XCD-aware map.
 void launch_order_keys(const int32_t* qc, int nq, int L, int PL, int two_np, uint32_t* key, hipStream_t st);
 void launch_xcd_order(const uint32_t* sorted, int nq, int nxcd, uint32_t* order, hipStream_t st);
+// Largest dedup window (ids) whose bitset, cursors and scratch fit in LDS.
+int64_t gather_window_bits(int64_t N, int qProbes, int64_t extra_words);
 // Dedup (LDS bitset windows), FP32 rerank and top-k of each query.
 // cand/score: per-query segments at cand_off[q]; ncand[q] = unique count.
 void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,
                              const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
                              const uint64_t* cand_off, int32_t* cand, float* score, uint64_t* ncand,
                              int32_t* out_ids, float* out_scores, int threads, hipStream_t st);
+
+// ---- reduced-precision scoring + FP32 refine (lowprec.hip) ----------------
+struct LowPrecArgs {
+    int format;              // 1 = BF16, 2 = FP8-e4m3 with per-row scale
+    int refine;              // survivors rescored in FP32 (R), 1..256
+    int64_t ldl;             // row stride of the low-precision copy (elements, multiple of 32)
+    const uint16_t* xb;      // BF16 copy (bit patterns)
+    const uint8_t* x8;       // FP8 copy
+    const float* scale;      // FP8 per-row scale
+};
+int lowprec_threads();
+int64_t lowprec_extra_words(int refine);
+void launch_quant_bf16(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint16_t* Xb, hipStream_t st);
+void launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint8_t* X8, float* scale,
+                      hipStream_t st);
+void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, int nq, const uint32_t* probes,
+                            const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q,
+                            int64_t ldq, const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids,
+                            float* out_scores, hipStream_t st);
 
 // ---- FP64 brute force (bf.hip), verification only -------------------------
 void launch_bf64(const float* X, int64_t ldx, int64_t N, int D, const float* Q, int64_t ldq, int nq, int k,

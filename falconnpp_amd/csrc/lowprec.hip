@@ -1,0 +1,338 @@
+// Reduced-precision scoring with an exact FP32 refine (no counterpart in the
+// reference; its FP32 rerank is src/FalconnPP.cpp:627-673).
+//
+// The centred data gets a second copy in BF16, or in FP8-e4m3 with one FP32
+// scale per row.  A query gathers and dedups its candidates exactly as the
+// FP32 path does.  It then scores them on MFMA (v_mfma_f32_16x16x32_bf16 /
+// _fp8_fp8, FP32 accumulation), keeps the best R by approximate score,
+// rescores those R exactly in FP32 from the master copy and returns the top-k.
+//
+// MFMA use: 16 candidate rows form the A operand (lane l holds row l%16,
+// elements 32t + 8*(l/16) .. +7); the query, converted once, is the B operand
+// broadcast to all 16 columns, so every column of the 16x16 result holds the
+// 16 row scores and lane l reads rows 4*(l/16) + i from its 4 accumulators.
+#include "common.h"
+#include "gather_common.h"
+#include "kernels.h"
+
+namespace fpp {
+namespace {
+
+using namespace gather;
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+// Round to nearest even, as a bit pattern.
+__device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// FP8 scale: the row's largest magnitude maps to 240, inside the finite range
+// of both e4m3 encodings.
+constexpr float FP8_TARGET = 240.0f;
+
+__global__ void k_quant_bf16(const float* __restrict__ Xc, int64_t ldx, int64_t N, int D, int64_t ldl,
+                             uint16_t* __restrict__ Xb) {
+    const int64_t total = N * ldl;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = i / ldl;
+        const int d = (int)(i - n * ldl);
+        Xb[i] = d < D ? f32_to_bf16_bits(Xc[n * ldx + d]) : (uint16_t)0;
+    }
+}
+
+// One wave per row: scale = amax / 240, codes = e4m3(x / scale), zero padded.
+__global__ __launch_bounds__(256) void k_quant_fp8(const float* __restrict__ Xc, int64_t ldx, int64_t N, int D,
+                                                   int64_t ldl, uint8_t* __restrict__ X8, float* __restrict__ scale) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t n = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (n >= N) return;
+    const float* x = Xc + n * ldx;
+    float amax = 0.0f;
+    for (int d = lane; d < D; d += WAVE) amax = fmaxf(amax, fabsf(x[d]));
+    for (int off = 32; off > 0; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off, WAVE));
+    const float s = amax > 0.0f ? amax / FP8_TARGET : 1.0f;
+    const float inv = 1.0f / s;
+    if (lane == 0) scale[n] = s;
+    // four codes per lane per step
+    for (int d0 = lane * 4; d0 < ldl; d0 += WAVE * 4) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = (d0 + u < D) ? x[d0 + u] * inv : 0.0f;
+        int w = 0;
+        w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], w, false);
+        w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);
+        *reinterpret_cast<int*>(X8 + n * ldl + d0) = w;
+    }
+}
+
+enum Fmt { BF16 = 1, FP8 = 2 };
+
+template <int TB, int FMT, int KT, int KPL>
+__global__ __launch_bounds__(TB) void k_gather_score_lp(GatherArgs a, LowPrecArgs lp,
+                                                        const uint32_t* __restrict__ probes,
+                                                        const uint64_t* __restrict__ bucket_off,
+                                                        const int32_t* __restrict__ ids, const float* __restrict__ Xc,
+                                                        const float* __restrict__ Q, int64_t ldq,
+                                                        const uint64_t* __restrict__ cand_off,
+                                                        int32_t* __restrict__ cand, uint64_t* __restrict__ ncand,
+                                                        int32_t* __restrict__ out_ids, float* __restrict__ out_scores) {
+    constexpr int NW = TB / WAVE;
+    extern __shared__ uint32_t lds[];
+    const int64_t W = a.window_bits;
+    const int Wwords = (int)(W / 32);
+    const bool multi = a.N > W;
+    const int R = lp.refine;
+    uint32_t* bits = lds;
+    uint32_t* cur = bits + ((Wwords + 1) & ~1);
+    uint32_t* sh = cur + (multi ? ((a.qProbes + 1) & ~1) : 0);
+    uint32_t* red = sh + TB;                                           // 64 words
+    uint64_t* wkeys = reinterpret_cast<uint64_t*>(red + 64);           // NW * R
+    uint64_t* rkeys = wkeys + (size_t)NW * R;                          // R exact keys
+    int32_t* rid = reinterpret_cast<int32_t*>(rkeys + R);              // R ids
+    const int q = a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
+    uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * 5 : nullptr;
+    if (ts && threadIdx.x == 0) ts[0] = wall_clock64();
+    const uint32_t* pr = probes + (size_t)q * a.qProbes;
+    const uint64_t cbase = cand_off[q];
+    const uint64_t ccap = cand_off[q + 1] - cbase;
+    int32_t* cq = cand + cbase;
+    const uint32_t nc = collect_candidates<TB>(a, pr, bucket_off, ids, bits, cur, sh, cq, ccap, ts);
+    if (threadIdx.x == 0) ncand[q] = nc;
+    if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = threadIdx.x / WAVE;
+    const int g16 = lane >> 4;  // k-group of this lane in an MFMA operand
+    const int r16 = lane & 15;  // row (A) / column (B) of this lane
+    const float* qf32 = Q + (int64_t)q * ldq;
+    const int kt = (int)(lp.ldl / 32);
+    // query in the scoring format, as the broadcast B operand
+    float qscale = 1.0f;
+    if constexpr (FMT == FP8) {
+        float m = 0.0f;
+        for (int d = threadIdx.x; d < a.D; d += TB) m = fmaxf(m, fabsf(qf32[d]));
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, WAVE));
+        if (lane == 0) red[wv] = __float_as_uint(m);
+        __syncthreads();
+        m = 0.0f;
+        for (int w = 0; w < NW; ++w) m = fmaxf(m, __uint_as_float(red[w]));
+        qscale = m > 0.0f ? m / FP8_TARGET : 1.0f;
+        __syncthreads();
+    }
+    bf16x8 qb[KT];
+    long q8[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int d = 32 * t + 8 * g16 + j;
+            v[j] = (t < kt && d < a.D) ? qf32[d] : 0.0f;
+        }
+        if constexpr (FMT == BF16) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) qb[t][j] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(v[j]));
+        } else {
+            const float inv = 1.0f / qscale;
+            int lo = 0, hi = 0;
+            lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, lo, false);
+            lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, lo, true);
+            hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4] * inv, v[5] * inv, hi, false);
+            hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6] * inv, v[7] * inv, hi, true);
+            q8[t] = (long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+        }
+    }
+    // approximate scores, 2 tiles of 16 candidates per wave step
+    WaveTopK<KPL> tk;
+    tk.init(R);
+    constexpr uint64_t LEADERS = 0x0001000100010001ull;  // lanes 0, 16, 32, 48
+    for (uint32_t t0 = (uint32_t)wv * 32; t0 < nc; t0 += (uint32_t)NW * 32) {
+        int32_t id[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t i = t0 + h * 16 + r16;
+            id[h] = i < nc ? cq[i] : -1;
+        }
+        f32x4 acc[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) acc[h] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            if (t < kt) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int64_t row = id[h] < 0 ? 0 : id[h];
+                    const int64_t off = row * lp.ldl + 32 * t + 8 * g16;
+                    if constexpr (FMT == BF16) {
+                        const bf16x8 av = *reinterpret_cast<const bf16x8*>(lp.xb + off);
+                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, qb[t], acc[h], 0, 0, 0);
+                    } else {
+                        const long av = *reinterpret_cast<const long*>(lp.x8 + off);
+                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av, q8[t], acc[h], 0, 0, 0);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = 4 * g16 + i;  // row of this accumulator
+                const int32_t idm = __shfl(id[h], m, WAVE);
+                float sc = acc[h][i];
+                if constexpr (FMT == FP8) sc *= (idm < 0 ? 0.0f : lp.scale[idm]) * qscale;
+                tk.offer(score_key(sc, idm), LEADERS);
+            }
+        }
+    }
+    // merge the waves' lists into the best R overall
+#pragma unroll
+    for (int s = 0; s < KPL; ++s) {
+        const int i = s * WAVE + lane;
+        if (i < R) wkeys[wv * R + i] = tk.w[s];
+    }
+    __syncthreads();
+    const uint32_t rr = min((uint32_t)R, nc);
+    for (int e = threadIdx.x; e < NW * R; e += TB) {
+        const uint64_t ke = wkeys[e];
+        if (ke == ~0ull) continue;
+        uint32_t r = 0;
+        for (int o = 0; o < NW * R; ++o) r += wkeys[o] < ke ? 1u : 0u;
+        if (r < rr) rid[r] = (int32_t)(uint32_t)(ke & 0xffffffffu);
+    }
+    __syncthreads();
+    if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
+    // exact FP32 rescoring of the survivors, 8 lanes per row
+    const int g8 = threadIdx.x & 7;
+    const int grp = threadIdx.x >> 3;
+    const int64_t nchunk = a.ldx / 4;
+    const float4* q4 = reinterpret_cast<const float4*>(qf32);
+    for (uint32_t i = (uint32_t)grp; i < rr; i += TB / 8) {
+        const bool v = i < rr;
+        const int32_t id = v ? rid[i] : 0;
+        const float4* row = reinterpret_cast<const float4*>(Xc + (int64_t)id * a.ldx);
+        float s = 0.f;
+        for (int64_t ch = g8; ch < nchunk; ch += 8) {
+            const float4 x = row[ch];
+            const float4 y = q4[ch];
+            s = fmaf(y.x, x.x, s);
+            s = fmaf(y.y, x.y, s);
+            s = fmaf(y.z, x.z, s);
+            s = fmaf(y.w, x.w, s);
+        }
+        s += __shfl_xor(s, 1, WAVE);
+        s += __shfl_xor(s, 2, WAVE);
+        s += __shfl_xor(s, 4, WAVE);
+        if (v && g8 == 0) rkeys[i] = score_key(s, id);
+    }
+    __syncthreads();
+    int32_t* oq = out_ids + (size_t)q * a.k;
+    float* osq = out_scores ? out_scores + (size_t)q * a.k : nullptr;
+    const uint32_t kk = min((uint32_t)a.k, rr);
+    for (uint32_t e = threadIdx.x; e < rr; e += TB) {
+        const uint64_t ke = rkeys[e];
+        uint32_t r = 0;
+        for (uint32_t o = 0; o < rr; ++o) r += rkeys[o] < ke ? 1u : 0u;
+        if (r < kk) {
+            oq[r] = (int32_t)(uint32_t)(ke & 0xffffffffu);
+            if (osq) osq[r] = desc_key_any_value((uint32_t)(ke >> 32));
+        }
+    }
+    for (uint32_t i = kk + threadIdx.x; i < (uint32_t)a.k; i += TB) {
+        oq[i] = 0;
+        if (osq) osq[i] = 0.0f;
+    }
+    if (ts && threadIdx.x == 0) ts[4] = wall_clock64();
+}
+
+constexpr int LP_TB = 512;
+
+template <int FMT, int KT>
+void dispatch_lp_kpl(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t lds, const uint32_t* probes,
+                     const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
+                     const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids, float* out_scores,
+                     hipStream_t st) {
+#define FPP_LP(KPLV)                                                                                          \
+    k_gather_score_lp<LP_TB, FMT, KT, KPLV><<<nq, LP_TB, lds, st>>>(a, lp, probes, bucket_off, ids, Xc, Q, ldq, \
+                                                                   cand_off, cand, ncand, out_ids, out_scores)
+    if (lp.refine <= WAVE)
+        FPP_LP(1);
+    else if (lp.refine <= 2 * WAVE)
+        FPP_LP(2);
+    else
+        FPP_LP(4);
+#undef FPP_LP
+}
+
+template <int FMT>
+void dispatch_lp_kt(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t lds, const uint32_t* probes,
+                    const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
+                    const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids, float* out_scores,
+                    hipStream_t st) {
+    const int64_t kt = lp.ldl / 32;
+    if (kt <= 4)
+        dispatch_lp_kpl<FMT, 4>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+                                out_scores, st);
+    else if (kt <= 8)
+        dispatch_lp_kpl<FMT, 8>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+                                out_scores, st);
+    else if (kt <= 16)
+        dispatch_lp_kpl<FMT, 16>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+                                 out_scores, st);
+    else
+        dispatch_lp_kpl<FMT, 32>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+                                 out_scores, st);
+}
+
+}  // namespace
+
+int lowprec_threads() { return LP_TB; }
+
+// LDS words beyond the bitset and cursors: scan scratch, reduction, wave
+// lists, survivors.
+int64_t lowprec_extra_words(int refine) {
+    const int NW = LP_TB / WAVE;
+    return LP_TB + 64 + 2 * ((int64_t)NW * refine + refine) + refine + 64;
+}
+
+void launch_quant_bf16(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint16_t* Xb, hipStream_t st) {
+    const int64_t total = N * ldl;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, 256), 1 << 20));
+    k_quant_bf16<<<grid, 256, 0, st>>>(Xc, ldx, N, D, ldl, Xb);
+    FPP_HIP(hipGetLastError());
+}
+
+void launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint8_t* X8, float* scale,
+                      hipStream_t st) {
+    FPP_REQUIRE(ldl % 4 == 0, FPP_EINTERNAL, "fp8 row stride must be a multiple of 4");
+    k_quant_fp8<<<(unsigned)ceil_div(N, 4), 256, 0, st>>>(Xc, ldx, N, D, ldl, X8, scale);
+    FPP_HIP(hipGetLastError());
+}
+
+void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, int nq, const uint32_t* probes,
+                            const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q,
+                            int64_t ldq, const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids,
+                            float* out_scores, hipStream_t st) {
+    FPP_REQUIRE(lp.format == BF16 || lp.format == FP8, FPP_EINVAL, "unknown scoring format");
+    FPP_REQUIRE(lp.refine >= 1 && lp.refine <= 4 * WAVE, FPP_EINVAL, "refine count must be in [1, 256]");
+    FPP_REQUIRE(a.k >= 1 && a.k <= lp.refine, FPP_EINVAL, "n_neighbors must not exceed the refine count");
+    FPP_REQUIRE(lp.ldl % 32 == 0 && lp.ldl / 32 <= 32, FPP_EINVAL, "n_features must be <= 1024");
+    if (nq == 0) return;
+    const int64_t wwords = (a.window_bits / 32 + 1) & ~1;
+    const int64_t curw = a.N > a.window_bits ? ((a.qProbes + 1) & ~1) : 0;
+    const size_t lds = (size_t)(wwords + curw + lowprec_extra_words(lp.refine)) * 4;
+    FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "LDS budget exceeded (refine count or qProbes too large)");
+    if (lp.format == BF16)
+        dispatch_lp_kt<BF16>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+                             out_scores, st);
+    else
+        dispatch_lp_kt<FP8>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+                            out_scores, st);
+    FPP_HIP(hipGetLastError());
+}
+
+}  // namespace fpp

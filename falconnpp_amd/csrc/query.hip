@@ -11,93 +11,14 @@
 //                   the top-k (:650-673), ties broken towards smaller ids.
 #include "common.h"
 #include "heap.h"
+#include "gather_common.h"
 #include "kernels.h"
 
 namespace fpp {
 namespace {
 
-// ---- block / wave helpers -------------------------------------------------
-template <int T>
-__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* sh, uint32_t& total) {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int off = 1; off < T; off <<= 1) {
-        const uint32_t a = t >= off ? sh[t - off] : 0u;
-        __syncthreads();
-        sh[t] += a;
-        __syncthreads();
-    }
-    total = sh[T - 1];
-    const uint32_t incl = sh[t];
-    __syncthreads();
-    return incl - v;
-}
+using namespace gather;
 
-// Digit holding the k-th smallest key (1-based) of a 256-bin histogram.
-template <int T>
-__device__ __forceinline__ void block_find_digit(const uint32_t* hist, uint32_t k, uint32_t* sh, int& digit,
-                                                 uint32_t& below, uint32_t& eq, int* s_digit, uint32_t* s_below) {
-    static_assert(T >= 256, "needs >= 256 threads");
-    if (threadIdx.x == 0) {
-        *s_digit = 255;
-        *s_below = 0;
-    }
-    const uint32_t h = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
-    uint32_t total;
-    const uint32_t ex = block_excl_scan_u32<T>(h, sh, total);
-    if (threadIdx.x < 256 && ex < k && k <= ex + h) {
-        *s_digit = threadIdx.x;
-        *s_below = ex;
-    }
-    __syncthreads();
-    digit = *s_digit;
-    below = *s_below;
-    eq = hist[digit];
-    __syncthreads();
-}
-
-// Wave variant: lane t owns bins 4t..4t+3.
-__device__ __forceinline__ void wave_find_digit(const uint32_t* hist, uint32_t k, int& digit, uint32_t& below,
-                                                uint32_t& eq) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    uint32_t b[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) b[i] = hist[lane * 4 + i];
-    const uint32_t s = b[0] + b[1] + b[2] + b[3];
-    uint32_t incl = s;
-#pragma unroll
-    for (int off = 1; off < WAVE; off <<= 1) {
-        const uint32_t a = __shfl_up(incl, off, WAVE);
-        if (lane >= off) incl += a;
-    }
-    const uint32_t ex = incl - s;
-    int d = -1;
-    uint32_t bl = 0, e = 0;
-    if (ex < k && k <= incl) {
-        uint32_t run = ex;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (d < 0 && run + b[i] >= k) {
-                d = lane * 4 + i;
-                bl = run;
-                e = b[i];
-            }
-            run += b[i];
-        }
-    }
-    const uint64_t who = __ballot(d >= 0);
-    const int src = who ? (int)__ffsll((long long)who) - 1 : WAVE - 1;
-    if (!who && lane == WAVE - 1) d = 255;
-    digit = __shfl(d, src, WAVE);
-    below = __shfl(bl, src, WAVE);
-    eq = __shfl(e, src, WAVE);
-}
-
-__device__ __forceinline__ uint64_t lanes_below_mask() {
-    const int lane = threadIdx.x & (WAVE - 1);
-    return lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
-}
 
 constexpr int PS_T = 256;
 constexpr int PS_WAVES = PS_T / WAVE;
@@ -416,8 +337,12 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
     extern __shared__ uint32_t lds[];
     const int64_t W = a.window_bits;
     const int Wwords = (int)(W / 32);
+    const bool multi = a.N > W;
     uint32_t* bits = lds;
-    uint32_t* sh = bits + ((Wwords + 1) & ~1);  // TB (8-byte aligned from here on)
+    // per-probe cursors into the (ascending) bucket id lists, kept across
+    // windows so every id is read once
+    uint32_t* cur = bits + ((Wwords + 1) & ~1);
+    uint32_t* sh = cur + (multi ? ((a.qProbes + 1) & ~1) : 0);  // TB (8-byte aligned from here on)
     uint32_t* hist = sh + TB;           // 256
     uint32_t* tk_key = hist + 256;      // kcap
     uint32_t* tk_id = tk_key + 1024;    // kcap
@@ -431,48 +356,7 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
     const uint64_t ccap = cand_off[q + 1] - cbase;
     int32_t* cq = cand + cbase;
     float* sq = score + cbase;
-    uint32_t nc = 0;
-    for (int64_t w0 = 0; w0 < a.N; w0 += W) {
-        for (int i = threadIdx.x; i < Wwords; i += TB) bits[i] = 0u;
-        __syncthreads();
-        for (int i = threadIdx.x; i < a.qProbes; i += TB) {
-            const uint32_t b = pr[i];
-            const uint64_t s = bucket_off[b], e = bucket_off[b + 1];
-            // four independent id loads in flight per step
-            for (uint64_t j = s; j < e; j += 4) {
-                int32_t v[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) v[u] = (j + u < e) ? ids[j + u] : -1;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int64_t rel = (int64_t)v[u] - w0;
-                    if (v[u] >= 0 && rel >= 0 && rel < W) atomicOr(&bits[rel >> 5], 1u << (rel & 31));
-                }
-            }
-        }
-        __syncthreads();
-        if (ts && threadIdx.x == 0 && w0 == 0) ts[1] = wall_clock64();
-        // ordered compaction: thread t scans a contiguous run of words
-        const int per = (Wwords + TB - 1) / TB;
-        const int wb = threadIdx.x * per;
-        uint32_t c = 0;
-        for (int i = 0; i < per; ++i)
-            if (wb + i < Wwords) c += __popc(bits[wb + i]);
-        uint32_t tot;
-        uint32_t pos = nc + block_excl_scan_u32<TB>(c, sh, tot);
-        for (int i = 0; i < per; ++i) {
-            if (wb + i >= Wwords) break;
-            uint32_t word = bits[wb + i];
-            while (word) {
-                const int bit = __ffs(word) - 1;
-                word &= word - 1;
-                if (pos < ccap) cq[pos] = (int32_t)(w0 + (int64_t)(wb + i) * 32 + bit);
-                pos++;
-            }
-        }
-        nc += tot;
-        __syncthreads();
-    }
+    const uint32_t nc = collect_candidates<TB>(a, pr, bucket_off, ids, bits, cur, sh, cq, ccap, ts);
     if (threadIdx.x == 0) ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
     // FP32 scores: groups of 8 lanes per candidate row, CPL float4 chunks per
@@ -495,8 +379,8 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
     // wave-level running top-k (k <= 64): lane j holds the wave's j-th best
     // key (desc_key(score) << 32 | id), ascending; misses cost one compare
     const bool wsel = a.k <= WAVE;
-    uint64_t wk = ~0ull;
-    uint64_t thr = ~0ull;
+    WaveTopK<1> tk;
+    tk.init(a.k);
     int idn[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -542,22 +426,9 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
             }
             continue;
         }
+        // the wave's 8 groups, one candidate each, in group order
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint64_t key = id[u] < 0 ? ~0ull : (((uint64_t)desc_key_any(sc[u]) << 32) | (uint32_t)id[u]);
-            // the wave's 8 groups, one candidate each, in group order
-            uint64_t better = __ballot(key < thr) & 0x0101010101010101ull;
-            while (better) {
-                const int src = (int)__ffsll((long long)better) - 1;
-                better &= better - 1;
-                const uint64_t nk = __shfl(key, src, WAVE);
-                if (nk >= thr) continue;
-                const int pos = (int)__popcll(__ballot(wk < nk));
-                const uint64_t up = __shfl_up(wk, 1, WAVE);
-                wk = lane > pos ? up : (lane == pos ? nk : wk);
-                thr = __shfl(wk, a.k - 1, WAVE);
-            }
-        }
+        for (int u = 0; u < U; ++u) tk.offer(score_key(sc[u], id[u]), 0x0101010101010101ull);
     }
     __syncthreads();
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
@@ -568,7 +439,7 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
         // merge the waves' lists: rank each entry among all of them
         constexpr int NW = TB / WAVE;
         uint64_t* wl = reinterpret_cast<uint64_t*>(tk_key);  // NW * k entries (<= 1024 x 8 B)
-        if (lane < a.k) wl[wv * a.k + lane] = wk;
+        if (lane < a.k) wl[wv * a.k + lane] = tk.w[0];
         __syncthreads();
         const int tot = NW * a.k;
         for (int e = threadIdx.x; e < tot; e += TB) {
@@ -662,6 +533,12 @@ __global__ void k_xcd_order(const uint32_t* __restrict__ sorted, int nq, int nxc
     order[b] = sorted[start + j];
 }
 
+size_t gather_lds_bytes(const GatherArgs& a, int TB) {
+    const int64_t wwords = (a.window_bits / 32 + 1) & ~1;
+    const int64_t curw = a.N > a.window_bits ? ((a.qProbes + 1) & ~1) : 0;
+    return (size_t)(wwords + curw + TB + 256 + 2048) * 4;
+}
+
 template <int TB>
 void dispatch_gather_cpl(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,
                          const int32_t* ids, const float* Xc, const float* Q, int64_t ldq, const uint64_t* cand_off,
@@ -669,7 +546,7 @@ void dispatch_gather_cpl(const GatherArgs& a, int nq, const uint32_t* probes, co
                          hipStream_t st) {
     const int64_t nchunk = a.ldx / 4;
     const int cpl = (int)ceil_div(nchunk, 8);
-    const size_t lds = (size_t)(a.window_bits / 32 + 1 + TB + 256 + 2048) * 4;
+    const size_t lds = gather_lds_bytes(a, TB);
 #define FPP_G(C) \
     k_gather_score_f32<TB, C><<<nq, TB, lds, st>>>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids, out_scores)
     if (cpl <= 1)
@@ -730,6 +607,17 @@ void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, 
     FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "probe lists (n_tables * perTable) too large for the exact merge");
     k_probe_exact_merge<<<nlist, MG_T, lds, st>>>(g, qlist, bucket_off, probes, ctot, w, work_per_query);
     FPP_HIP(hipGetLastError());
+}
+
+// extra_words: LDS words the kernel needs beyond the bitset and cursors.
+int64_t gather_window_bits(int64_t N, int qProbes, int64_t extra_words) {
+    const int64_t fixed = (extra_words + 64) * 4;  // + static LDS margin
+    const int64_t budget = 160 * 1024 - fixed;
+    const int64_t single = (N + 31) / 32 * 32;
+    if (single / 8 + 8 <= budget) return single;
+    const int64_t w = ((budget - 4 * ((int64_t)qProbes + 2)) * 8) / 64 * 64;
+    FPP_REQUIRE(w >= (1 << 16), FPP_EINVAL, "qProbes too large for the dedup window in LDS");
+    return w;
 }
 
 void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,

@@ -21,8 +21,9 @@ EXPORTS = (
     "fpp_clear", "fpp_build", "fpp_build_device", "fpp_query", "fpp_query_device", "fpp_get_shape",
     "fpp_get_center", "fpp_get_sign_bits", "fpp_get_buckets", "fpp_hash_points", "fpp_get_stats",
     "fpp_get_last_probes", "fpp_get_last_candidates", "fpp_bruteforce_f64", "fpp_set_profiling",
-    "fpp_get_timings", "fpp_set_tuning",
+    "fpp_get_timings", "fpp_set_tuning", "fpp_set_precision",
 )
+PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
 TUNE_QUERY_ORDER, TUNE_GATHER_THREADS = 1, 2
 
 FPP_EINVAL, FPP_ESTATE, FPP_EHIP, FPP_ENOMEM, FPP_EINTERNAL = 1, 2, 3, 4, 5
@@ -68,6 +69,7 @@ def load():
     L.fpp_set_profiling.argtypes = [vp, i32]
     L.fpp_get_timings.argtypes = [vp, vp]
     L.fpp_set_tuning.argtypes = [vp, i32, i32]
+    L.fpp_set_precision.argtypes = [vp, i32, i32]
     L.fpp_get_last_candidates.restype = i64
     L.fpp_get_last_candidates.argtypes = [vp, i32, vp, i64]
     L.fpp_bruteforce_f64.argtypes = [vp, i64, i64, i32, vp, i64, i32, i32, vp, vp, vp]
@@ -196,6 +198,10 @@ class Index:
         _check(load().fpp_get_timings(self.h, _ptr(t)))
         return dict(hash=t[0], probe=t[1], replay=t[2], gather=t[3], ph_gather_us=t[4], ph_compact_us=t[5],
                     ph_score_us=t[6], ph_topk_us=t[7])
+
+    def set_precision(self, mode: int, refine: int = 0):
+        """0 FP32 (reference), 1 BF16 + FP32 refine, 2 FP8 + FP32 refine."""
+        _check(load().fpp_set_precision(self.h, mode, refine))
 
     def set_tuning(self, which: int, value: int):
         _check(load().fpp_set_tuning(self.h, which, value))

@@ -100,6 +100,14 @@ int fpp_get_stats(fpp_index* h, int64_t* out);
  * compaction, [6]=scoring, [7]=top-k.  ms must hold 8 doubles. */
 int fpp_set_profiling(fpp_index* h, int on);
 int fpp_get_timings(fpp_index* h, double* ms);
+/* Scoring precision (no reference counterpart; the reference scores in FP32):
+ * mode 0 = FP32 (default, reference results), 1 = BF16 copy scored on MFMA,
+ * 2 = FP8-e4m3 copy with a per-row FP32 scale scored on MFMA.  Modes 1-2 keep
+ * the `refine` best approximate candidates (0: 4 * n_neighbors, at most 256)
+ * and rescore them exactly in FP32, so returned scores are FP32 scores; the
+ * ids can differ from mode 0 when a true top-k row falls outside the refine
+ * set.  The low-precision copy is made on the first query after a build. */
+int fpp_set_precision(fpp_index* h, int mode, int refine);
 /* launch-shape knobs (no reference counterpart): FPP_TUNE_QUERY_ORDER 0/1
  * (process queries grouped by their best table-0 bucket, XCD-aware),
  * FPP_TUNE_GATHER_THREADS 256/512/1024 (workgroup of the gather kernel) */

@@ -36,8 +36,8 @@ CASES = {
     "mix_d128_p2": (lambda: synth.gaussian_mixture(12000, 128, 60, seed=13), 8, 128, 10, 0.2, 2, 160, 10, 50),
     "nproj16_d40": (lambda: synth.lognormal_sphere(5000, 40, 0.3, seed=14), 5, 16, 8, 0.05, 2, 60, 15, 40),
     "int_grid_ties": (lambda: _sym_int_grid(3000, 24, -2, 2, 15), 4, 32, 5, 0.3, 2, 40, 10, 40),
-    # N above the 2^20-id LDS bitset window: the dedup runs over two windows
-    "multiwindow_n1.2m": (lambda: synth.gaussian_mixture(1_200_000, 16, 300, seed=16), 2, 16, 30, 0.05, 1, 8, 20, 12),
+    # N above the LDS bitset window (~1.2M ids): the dedup runs over two windows
+    "multiwindow_n1.3m": (lambda: synth.gaussian_mixture(1_300_000, 16, 300, seed=16), 2, 16, 30, 0.05, 1, 8, 20, 12),
 }
 
 _cache = {}
@@ -253,3 +253,59 @@ def test_query_order_and_launch_shape_do_not_change_results():
     t = g.timings()
     assert t["gather"] > 0 and t["ph_score_us"] > 0
     g.set_profiling(0)
+
+
+@pytest.mark.parametrize("mode", [E.PREC_BF16, E.PREC_FP8])
+def test_low_precision_refine_equals_fp32_when_every_candidate_is_refined(mode):
+    # few probes: every query has <= 256 unique candidates, so the FP32 refine
+    # rescores all of them and the result must equal the FP32 path exactly
+    X, Q, g, o = _built("mix_d128_p2")
+    g.set_qProbes(6)
+    ref = g.query(Q, 10)
+    nc = [len(g.last_candidates(j)) for j in range(len(Q))]
+    assert max(nc) <= 256
+    try:
+        g.set_precision(mode, 256)
+        got = g.query(Q, 10)
+    finally:
+        g.set_precision(E.PREC_FP32, 0)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("case,mode,min_recall", [("mix_d128_p2", E.PREC_BF16, 0.98), ("mix_d128_p2", E.PREC_FP8, 0.9),
+                                                  ("mix_d200_p3", E.PREC_BF16, 0.98), ("mix_d200_p3", E.PREC_FP8, 0.9)])
+def test_low_precision_recall_and_exact_scores(case, mode, min_recall):
+    import torch
+    X, Q, g, o = _built(case)
+    qp, k = CASES[case][6], CASES[case][7]
+    g.set_qProbes(qp)
+    ref = g.query(Q, k)
+    d = X.shape[1]
+    dQ = torch.from_numpy(np.ascontiguousarray(Q)).cuda()
+    out = torch.zeros((len(Q), k), dtype=torch.int32, device="cuda")
+    sc = torch.zeros((len(Q), k), dtype=torch.float32, device="cuda")
+    try:
+        g.set_precision(mode, 4 * k)
+        g.query_device(dQ.data_ptr(), d, len(Q), k, out.data_ptr(), sc.data_ptr())
+        torch.cuda.synchronize()
+    finally:
+        g.set_precision(E.PREC_FP32, 0)
+    got, scores = out.cpu().numpy(), sc.cpu().numpy()
+    assert E.recall(ref, got, k) >= min_recall
+    # returned scores are the exact FP32 scores of the returned ids
+    Xc = (X - g.center()).astype(np.float64)
+    for j in range(len(Q)):
+        exact = Xc[got[j]] @ Q[j].astype(np.float64)
+        np.testing.assert_allclose(scores[j], exact, rtol=1e-5, atol=1e-5)
+        assert np.all(np.diff(scores[j]) <= 0)
+
+
+def test_low_precision_rejects_k_above_refine():
+    X, Q, g, o = _built("mix_d128_p2")
+    g.set_precision(E.PREC_BF16, 8)
+    try:
+        with pytest.raises(E.FppError) as e:
+            g.query(Q[:2], 10)
+        assert e.value.code == E.FPP_EINVAL
+    finally:
+        g.set_precision(E.PREC_FP32, 0)
