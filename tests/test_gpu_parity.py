@@ -295,9 +295,11 @@ def test_low_precision_recall_and_exact_scores(case, mode, min_recall):
     # returned scores are the exact FP32 scores of the returned ids
     Xc = (X - g.center()).astype(np.float64)
     for j in range(len(Q)):
-        exact = Xc[got[j]] @ Q[j].astype(np.float64)
-        np.testing.assert_allclose(scores[j], exact, rtol=1e-5, atol=1e-5)
-        assert np.all(np.diff(scores[j]) <= 0)
+        m = min(k, len(g.last_candidates(j)))
+        exact = Xc[got[j, :m]] @ Q[j].astype(np.float64)
+        np.testing.assert_allclose(scores[j, :m], exact, rtol=1e-5, atol=1e-5)
+        assert np.all(np.diff(scores[j, :m]) <= 0)
+        assert np.all(got[j, m:] == 0) and np.all(scores[j, m:] == 0)
 
 
 def test_low_precision_rejects_k_above_refine():
