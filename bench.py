@@ -162,6 +162,14 @@ def main():
             v = {"value": round(nq * args.steps / vdt, 1), "ms_per_step": round(vdt / args.steps * 1e3, 3),
                  "refine": refine, "gather_ms": round(float(np.mean(vstage)), 3),
                  "speedup_vs_fp32": round((nq * args.steps / vdt) / qps, 3)}
+            # per-query phases (approximate scoring under ph_score_us, exact
+            # refine + ranking under ph_topk_us), outside the timed region
+            idx.set_profiling(2)
+            idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
+            vph = idx.timings()
+            idx.set_profiling(1)
+            v["phases_us"] = {key: round(float(vph[key]), 2)
+                              for key in ("ph_gather_us", "ph_compact_us", "ph_score_us", "ph_topk_us")}
             if truth is not None:
                 v["recall_at_k"] = round(E.recall(truth, out.cpu().numpy(), k), 4)
                 v["score_rel_err"] = approx_score_error(X, Q, idx.center(), truth, mode)

@@ -67,6 +67,6 @@ inline int ilog2_exact(int64_t x) {
     return ((int64_t(1) << l) == x) ? l : -1;
 }
 
-inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace fpp

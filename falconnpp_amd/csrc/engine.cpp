@@ -461,10 +461,10 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     ga.ldx = I.ldx;
     ga.qProbes = I.qProbes;
     ga.k = k;
+    ga.part = reinterpret_cast<int32_t*>(I.score.get());
     const bool lowp = I.precision != 0;
-    ga.window_bits = gather_window_bits(I.n_points, I.qProbes,
-                                        lowp ? lowprec_extra_words(I.refine_count(k))
-                                             : (int64_t)I.gather_threads + 256 + 2048);
+    ga.window_bits = gather_window_bits(I.n_points, lowp ? lowprec_extra_words(I.refine_count(k))
+                                                         : (int64_t)I.gather_threads + 256 + 2048);
     if (I.order_queries && nq > 1) {
         const int bits = ilog2_exact(I.NB());
         const int64_t rh = radix_hist_elems(nq);
@@ -840,8 +840,14 @@ int64_t fpp_get_last_candidates(fpp_index* h, int j, int32_t* out, int64_t cap) 
         FPP_HIP(hipMemcpy(&nc, I.ncand.get() + j, 8, hipMemcpyDeviceToHost));
         n = (int64_t)nc;
         const int64_t c = std::min<int64_t>(cap, n);
-        if (c > 0 && out)
-            FPP_HIP(hipMemcpy(out, I.cand.get() + I.last_cand_off[j], (size_t)c * 4, hipMemcpyDeviceToHost));
+        if (c > 0 && out) {
+            // above one dedup window the device list is unordered inside each
+            // window; report it ascending
+            std::vector<int32_t> all((size_t)n);
+            FPP_HIP(hipMemcpy(all.data(), I.cand.get() + I.last_cand_off[j], (size_t)n * 4, hipMemcpyDeviceToHost));
+            std::sort(all.begin(), all.end());
+            std::copy(all.begin(), all.begin() + c, out);
+        }
     });
     return rc == FPP_OK ? n : -(int64_t)rc;
 }

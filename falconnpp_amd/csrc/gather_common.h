@@ -93,80 +93,323 @@ __device__ __forceinline__ uint64_t lanes_below_mask() {
     return lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
 }
 
-// Collect the unique candidates of one query, ascending, into cq (capacity
-// ccap); returns their count.  Bucket ids are ORed into an LDS bitset window
-// of W ids (the reference's boost::dynamic_bitset, src/FalconnPP.cpp:552-611);
-// when N exceeds the window, per-probe cursors into the ascending bucket
-// lists carry over between windows so every id is read once.
+// Wave-level inclusive prefix sum.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, WAVE);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Calls f(id, true) for every id of the probed buckets pr[0 .. np), with all
+// lanes of the wave present (lanes without an id pass false).  A wave takes
+// 64 probes at a time and walks the concatenation of their bucket lists with
+// one lane per id, so neighbouring lanes read neighbouring ids and a load
+// instruction touches a few cache lines rather than one per lane.  A binary
+// search over the wave's prefix sums of bucket lengths maps a position to its
+// probe.  The next 64 probes' buckets are fetched while the current ones are
+// walked.
+template <int TB, class F>
+__device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ pr,
+                                                   const uint64_t* __restrict__ bucket_off,
+                                                   const int32_t* __restrict__ ids, int np, F&& f) {
+    constexpr int NW = TB / WAVE;
+    constexpr int UNR = 4;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = threadIdx.x / WAVE;
+    int c0 = wv * WAVE;
+    uint64_t s = 0;
+    uint32_t len = 0;
+    if (c0 + lane < np) {
+        const uint32_t b = pr[c0 + lane];
+        s = bucket_off[b];
+        len = (uint32_t)(bucket_off[b + 1] - s);
+    }
+    for (; c0 < np; c0 += NW * WAVE) {
+        const int cn = c0 + NW * WAVE;
+        const uint32_t nb = cn + lane < np ? pr[cn + lane] : 0u;
+        const uint32_t incl = wave_incl_scan(len);
+        const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+        const uint64_t sx = s - (uint64_t)(incl - len);  // ids index of flat position f is sx + f
+        for (uint32_t f0 = 0; f0 < total; f0 += WAVE * UNR) {
+            int32_t v[UNR];
+#pragma unroll
+            for (int r = 0; r < UNR; ++r) {
+                const uint32_t fp = f0 + r * WAVE + lane;
+                // probe of position fp: number of lanes whose inclusive sum is <= fp
+                int lo = 0;
+#pragma unroll
+                for (int st = WAVE / 2; st > 0; st >>= 1)
+                    if (__shfl(incl, lo + st - 1, WAVE) <= fp) lo += st;
+                const uint64_t base = __shfl(sx, lo, WAVE);
+                v[r] = fp < total ? ids[base + fp] : 0;
+            }
+#pragma unroll
+            for (int r = 0; r < UNR; ++r) f(v[r], f0 + r * WAVE + lane < total);
+        }
+        s = 0;
+        len = 0;
+        if (cn + lane < np) {
+            s = bucket_off[nb];
+            len = (uint32_t)(bucket_off[nb + 1] - s);
+        }
+    }
+}
+
+// Rank of this lane among the set lanes of m.
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Exclusive wave prefix sum of p (0 <= p <= 32) and its total, from ballots of
+// the bits of p: no cross-lane data movement, so no LDS round trips.
+__device__ __forceinline__ uint32_t wave_excl_prefix_small(uint32_t p, uint32_t& total) {
+    if (__ballot(p > 1u) == 0ull) {
+        const uint64_t m = __ballot(p != 0u);
+        total = (uint32_t)__popcll(m);
+        return lane_rank(m);
+    }
+    uint32_t ex = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const uint64_t m = __ballot((p >> b) & 1u);
+        ex += lane_rank(m) << b;
+        tot += (uint32_t)__popcll(m) << b;
+    }
+    total = tot;
+    return ex;
+}
+
+// Ordered compaction of one window's bitset (words words, ids from w0) into
+// cq[nc ...]: wave v owns a contiguous run of words and its lanes read
+// consecutive words.  red holds TB / 64 words.  Returns the new count.
+template <int TB>
+__device__ __forceinline__ uint32_t compact_window(const uint32_t* bits, int words, int64_t w0, uint32_t nc,
+                                                   uint32_t* red, int32_t* __restrict__ cq, uint64_t ccap) {
+    constexpr int NW = TB / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = threadIdx.x / WAVE;
+    const int per = (int)ceil_div(ceil_div(words, NW), WAVE) * WAVE;
+    const int wb = wv * per;
+    const int we = min(words, wb + per);
+    uint32_t c = 0;
+    for (int i = wb + lane; i < we; i += WAVE) c += __popc(bits[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, WAVE);
+    if (lane == 0) red[wv] = c;
+    __syncthreads();
+    uint32_t base = nc, tot = 0;
+#pragma unroll
+    for (int v = 0; v < NW; ++v) {
+        const uint32_t x = red[v];
+        base += v < wv ? x : 0u;
+        tot += x;
+    }
+    for (int i0 = wb; i0 < we; i0 += WAVE) {
+        const int i = i0 + lane;
+        uint32_t word = i < we ? bits[i] : 0u;
+        uint32_t slab;
+        uint32_t pos = base + wave_excl_prefix_small(__popc(word), slab);
+        while (word) {
+            const int bit = __ffs(word) - 1;
+            word &= word - 1;
+            if (pos < ccap) cq[pos] = (int32_t)(w0 + (int64_t)i * 32 + bit);
+            pos++;
+        }
+        base += slab;
+    }
+    __syncthreads();
+    return nc + tot;
+}
+
+// Appends v of the lanes with take set to cq at *count (LDS), one atomic per
+// wave.  Must be reached by the whole wave.
+__device__ __forceinline__ void wave_append(bool take, int32_t v, uint32_t* count, int32_t* __restrict__ cq,
+                                            uint64_t ccap) {
+    const uint64_t m = __ballot(take);
+    if (m == 0ull) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int leader = (int)__ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, WAVE);
+    const uint32_t r = lane_rank(m);
+    if (take && base + r < ccap) cq[base + r] = v;
+}
+
+// Test-and-set of id rel in the LDS bitset: true when the bit was clear.
+__device__ __forceinline__ bool bitset_insert(uint32_t* bits, int64_t rel) {
+    const uint32_t m = 1u << (rel & 31);
+    return (atomicOr(&bits[rel >> 5], m) & m) == 0u;
+}
+
+// Collect the unique candidates of one query into cq (capacity ccap, the
+// query's total bucket size); returns their count.  Bucket ids go through an
+// LDS bitset window of W ids (the reference's boost::dynamic_bitset,
+// src/FalconnPP.cpp:552-611).
+//  * N fits one window: ids are ORed in and the set bits compacted in id
+//    order, so cq is ascending.
+//  * Otherwise (W is then a power of two): two passes over the bucket ids
+//    count and then scatter them by window into part (same capacity); each
+//    window's segment goes through a test-and-set and first occurrences are
+//    appended, so cq is ordered by window and unordered inside one.
+// Callers rank by (score, id), which does not depend on the order of cq.
+// wc holds window_counter_words(N, W) words, sh at least TB words.
 template <int TB>
 __device__ uint32_t collect_candidates(const GatherArgs& a, const uint32_t* __restrict__ pr,
                                        const uint64_t* __restrict__ bucket_off, const int32_t* __restrict__ ids,
-                                       uint32_t* bits, uint32_t* cur, uint32_t* sh, int32_t* __restrict__ cq,
-                                       uint64_t ccap, uint64_t* ts) {
+                                       uint32_t* bits, uint32_t* wc, uint32_t* sh, int32_t* __restrict__ cq,
+                                       int32_t* __restrict__ part, uint64_t ccap, uint64_t* ts) {
     const int64_t W = a.window_bits;
-    const int Wwords = (int)(W / 32);
-    const bool multi = a.N > W;
-    uint32_t nc = 0;
-    constexpr uint32_t DONE = 0xffffffffu;
-    if (multi)
-        for (int i = threadIdx.x; i < a.qProbes; i += TB) cur[i] = 0u;
-    for (int64_t w0 = 0; w0 < a.N; w0 += W) {
-        for (int i = threadIdx.x; i < Wwords; i += TB) bits[i] = 0u;
+    const int np = a.qProbes;
+    if (a.N <= W) {
+        const int words = (int)(W / 32);
+        for (int i = threadIdx.x; i < words; i += TB) bits[i] = 0u;
         __syncthreads();
-        const int64_t wend = w0 + W;
-        for (int i = threadIdx.x; i < a.qProbes; i += TB) {
-            uint32_t cu = 0;
-            if (multi) {
-                cu = cur[i];
-                if (cu == DONE) continue;
-            }
-            const uint32_t b = pr[i];
-            const uint64_t s = bucket_off[b], e = bucket_off[b + 1];
-            uint64_t j = s + cu;
-            // four independent id loads in flight per step; a bucket's ids are
-            // ascending, so the first id past the window ends this probe's turn
-            while (j < e) {
-                int32_t v[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) v[u] = (j + u < e) ? ids[j + u] : INT32_MAX;
-                int taken = 0;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (taken == u && (int64_t)v[u] < wend) {
-                        const int64_t rel = (int64_t)v[u] - w0;
-                        atomicOr(&bits[rel >> 5], 1u << (rel & 31));
-                        taken = u + 1;
-                    }
-                }
-                j += taken;
-                if (taken < 4) break;
-            }
-            if (multi) cur[i] = (j >= e) ? DONE : (uint32_t)(j - s);
-        }
+        for_each_bucket_id<TB>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+            if (ok) atomicOr(&bits[v >> 5], 1u << (v & 31));
+        });
         __syncthreads();
-        if (ts && threadIdx.x == 0 && w0 == 0) ts[1] = wall_clock64();
-        // ordered compaction: thread t scans a contiguous run of words
-        const int per = (Wwords + TB - 1) / TB;
-        const int wb = threadIdx.x * per;
+        if (ts && threadIdx.x == 0) ts[1] = wall_clock64();
+        return compact_window<TB>(bits, words, 0, 0u, sh, cq, ccap);
+    }
+    const int nwin = (int)window_count(a.N, W);
+    const int shift = __ffsll((long long)W) - 1;
+    uint32_t* wcnt = wc;
+    uint32_t* woff = wc + nwin;
+    uint32_t* wfill = woff + nwin;
+    for (int i = threadIdx.x; i < nwin; i += TB) {
+        wcnt[i] = 0u;
+        wfill[i] = 0u;
+    }
+    __syncthreads();
+    for_each_bucket_id<TB>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+        if (ok) atomicAdd(&wcnt[v >> shift], 1u);
+    });
+    __syncthreads();
+    {
+        // exclusive scan of the window counts: thread t sums a contiguous run
+        const int per = (nwin + TB - 1) / TB;
+        const int b0 = threadIdx.x * per;
         uint32_t c = 0;
         for (int i = 0; i < per; ++i)
-            if (wb + i < Wwords) c += __popc(bits[wb + i]);
+            if (b0 + i < nwin) c += wcnt[b0 + i];
         uint32_t tot;
-        uint32_t pos = nc + block_excl_scan_u32<TB>(c, sh, tot);
-        for (int i = 0; i < per; ++i) {
-            if (wb + i >= Wwords) break;
-            uint32_t word = bits[wb + i];
-            while (word) {
-                const int bit = __ffs(word) - 1;
-                word &= word - 1;
-                if (pos < ccap) cq[pos] = (int32_t)(w0 + (int64_t)(wb + i) * 32 + bit);
-                pos++;
+        uint32_t run = block_excl_scan_u32<TB>(c, sh, tot);
+        for (int i = 0; i < per; ++i)
+            if (b0 + i < nwin) {
+                woff[b0 + i] = run;
+                run += wcnt[b0 + i];
             }
+    }
+    __syncthreads();
+    for_each_bucket_id<TB>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+        if (ok) {
+            const int w = v >> shift;
+            const uint32_t pos = woff[w] + atomicAdd(&wfill[w], 1u);
+            if (pos < ccap) part[pos] = v;
         }
-        nc += tot;
+    });
+    uint32_t* count = sh;  // appended ids
+    if (threadIdx.x == 0) *count = 0u;
+    __syncthreads();
+    if (ts && threadIdx.x == 0) ts[1] = wall_clock64();
+    for (int w = 0; w < nwin; ++w) {
+        const uint32_t cnt = wcnt[w];
+        if (cnt == 0) continue;
+        const int64_t w0 = (int64_t)w << shift;
+        const int words = (int)((min(W, a.N - w0) + 31) / 32);
+        for (int i = threadIdx.x; i < words; i += TB) bits[i] = 0u;
+        __syncthreads();
+        const uint32_t base = woff[w];
+        constexpr int UNR = 4;
+        for (uint32_t i0 = 0; i0 < cnt; i0 += TB * UNR) {
+            int32_t v[UNR];
+#pragma unroll
+            for (int r = 0; r < UNR; ++r) {
+                const uint32_t i = i0 + r * TB + threadIdx.x;
+                v[r] = (i < cnt && base + i < ccap) ? part[base + i] : -1;
+            }
+#pragma unroll
+            for (int r = 0; r < UNR; ++r)
+                wave_append(v[r] >= 0 && bitset_insert(bits, (int64_t)v[r] - w0), v[r], count, cq, ccap);
+        }
         __syncthreads();
     }
+    const uint32_t nc = *count;
+    __syncthreads();
     return nc;
+}
+
+// Selects the want smallest (key, id) pairs among keys[0 .. n), where the id
+// of position i is cq[i] (ids unique), and writes their ids to out_id in
+// unspecified order; returns min(want, n).  Radix select over the 32-bit key;
+// when more entries share the boundary key than are needed, a second radix
+// select over their ids keeps the smallest.  hist: 256 words, sh: TB words,
+// count: one word.
+template <int TB>
+__device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restrict__ cq, uint32_t n, uint32_t want,
+                                    uint32_t* hist, uint32_t* sh, int* s_digit, uint32_t* s_below, uint32_t* count,
+                                    int32_t* out_id) {
+    const uint32_t kk = min(want, n);
+    if (kk == 0) return 0;
+    uint32_t prefix = 0, pmask = 0, k = kk, eqc = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int i = threadIdx.x; i < 256; i += TB) hist[i] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += TB) {
+            const uint32_t key = keys[i];
+            if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        int d;
+        uint32_t below;
+        block_find_digit<TB>(hist, k, sh, d, below, eqc, s_digit, s_below);
+        prefix |= (uint32_t)d << shift;
+        pmask |= 255u << shift;
+        k -= below;
+    }
+    const uint32_t T = prefix;
+    uint32_t IT = 0xffffffffu;
+    if (eqc > k) {
+        uint32_t ip = 0, im = 0, kr = k;
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            for (int i = threadIdx.x; i < 256; i += TB) hist[i] = 0;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += TB) {
+                const uint32_t id = (uint32_t)cq[i];
+                if (keys[i] == T && (id & im) == ip) atomicAdd(&hist[(id >> shift) & 255u], 1u);
+            }
+            __syncthreads();
+            int d;
+            uint32_t below, eqd;
+            block_find_digit<TB>(hist, kr, sh, d, below, eqd, s_digit, s_below);
+            ip |= (uint32_t)d << shift;
+            im |= 255u << shift;
+            kr -= below;
+        }
+        IT = ip;
+    }
+    if (threadIdx.x == 0) *count = 0u;
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < n; c0 += TB) {
+        const uint32_t i = c0 + threadIdx.x;
+        bool take = false;
+        int32_t id = 0;
+        if (i < n) {
+            const uint32_t key = keys[i];
+            if (key <= T) {
+                id = cq[i];
+                take = key < T || (uint32_t)id <= IT;
+            }
+        }
+        wave_append(take, id, count, out_id, kk);
+    }
+    __syncthreads();
+    return kk;
 }
 
 // Wave-level running top-R (R <= 64*KPL) over unique 64-bit keys, smaller is

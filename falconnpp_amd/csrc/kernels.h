@@ -96,12 +96,21 @@ struct GatherArgs {
     int64_t window_bits;  // LDS bitset window
     const uint32_t* order = nullptr;  // block -> query (null: identity)
     uint64_t* phase_ts = nullptr;     // per query 5 wall-clock stamps (null: off)
+    int32_t* part = nullptr;          // dedup-window scratch, per-query segments at cand_off
 };
 // Query order for cache reuse: sort key per query, then the XCD-aware map.
 void launch_order_keys(const int32_t* qc, int nq, int L, int PL, int two_np, uint32_t* key, hipStream_t st);
 void launch_xcd_order(const uint32_t* sorted, int nq, int nxcd, uint32_t* order, hipStream_t st);
-// Largest dedup window (ids) whose bitset, cursors and scratch fit in LDS.
-int64_t gather_window_bits(int64_t N, int qProbes, int64_t extra_words);
+// Dedup windows over the ids [0, N) and the LDS words of their counters
+// (count, segment offset, fill per window) when there is more than one.
+__host__ __device__ inline int64_t window_count(int64_t N, int64_t W) { return (N + W - 1) / W; }
+__host__ __device__ inline int64_t window_counter_words(int64_t N, int64_t W) {
+    return N > W ? (3 * window_count(N, W) + 1) & ~int64_t(1) : 0;
+}
+// Dedup window (ids): all of N when its bitset fits in LDS beside
+// extra_words of scratch, else the largest power of two that fits together
+// with the window counters.
+int64_t gather_window_bits(int64_t N, int64_t extra_words);
 // Dedup (LDS bitset windows), FP32 rerank and top-k of each query.
 // cand/score: per-query segments at cand_off[q]; ncand[q] = unique count.
 void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,

@@ -71,8 +71,16 @@ __global__ __launch_bounds__(256) void k_quant_fp8(const float* __restrict__ Xc,
 
 enum Fmt { BF16 = 1, FP8 = 2 };
 
-template <int TB, int FMT, int KT, int KPL>
-__global__ __launch_bounds__(TB) void k_gather_score_lp(GatherArgs a, LowPrecArgs lp,
+// Tiles of 16 rows per wave step: about 16 KB of row data in flight per wave,
+// at least one tile.
+constexpr int lp_tiles(int fmt, int kt) {
+    const int row_bytes = kt * 32 * (fmt == BF16 ? 2 : 1);
+    const int t = 16384 / (16 * row_bytes);
+    return t < 1 ? 1 : (t > 8 ? 8 : t);
+}
+
+template <int TB, int FMT, int KT>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB / 256))) void k_gather_score_lp(GatherArgs a, LowPrecArgs lp,
                                                         const uint32_t* __restrict__ probes,
                                                         const uint64_t* __restrict__ bucket_off,
                                                         const int32_t* __restrict__ ids, const float* __restrict__ Xc,
@@ -84,15 +92,17 @@ __global__ __launch_bounds__(TB) void k_gather_score_lp(GatherArgs a, LowPrecArg
     extern __shared__ uint32_t lds[];
     const int64_t W = a.window_bits;
     const int Wwords = (int)(W / 32);
-    const bool multi = a.N > W;
     const int R = lp.refine;
     uint32_t* bits = lds;
-    uint32_t* cur = bits + ((Wwords + 1) & ~1);
-    uint32_t* sh = cur + (multi ? ((a.qProbes + 1) & ~1) : 0);
-    uint32_t* red = sh + TB;                                           // 64 words
-    uint64_t* wkeys = reinterpret_cast<uint64_t*>(red + 64);           // NW * R
-    uint64_t* rkeys = wkeys + (size_t)NW * R;                          // R exact keys
-    int32_t* rid = reinterpret_cast<int32_t*>(rkeys + R);              // R ids
+    uint32_t* wc = bits + ((Wwords + 1) & ~1);
+    uint32_t* sh = wc + window_counter_words(a.N, W);
+    uint32_t* red = sh + TB;                                  // 64 words
+    uint32_t* hist = red + 64;                                // 256
+    uint64_t* rkeys = reinterpret_cast<uint64_t*>(hist + 256);  // R exact keys
+    int32_t* rid = reinterpret_cast<int32_t*>(rkeys + R);     // R ids
+    uint32_t* count = reinterpret_cast<uint32_t*>(rid + R);   // 1
+    __shared__ int s_digit;
+    __shared__ uint32_t s_below;
     const int q = a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
     uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * 5 : nullptr;
     if (ts && threadIdx.x == 0) ts[0] = wall_clock64();
@@ -100,7 +110,7 @@ __global__ __launch_bounds__(TB) void k_gather_score_lp(GatherArgs a, LowPrecArg
     const uint64_t cbase = cand_off[q];
     const uint64_t ccap = cand_off[q + 1] - cbase;
     int32_t* cq = cand + cbase;
-    const uint32_t nc = collect_candidates<TB>(a, pr, bucket_off, ids, bits, cur, sh, cq, ccap, ts);
+    const uint32_t nc = collect_candidates<TB>(a, pr, bucket_off, ids, bits, wc, sh, cq, a.part + cbase, ccap, ts);
     if (threadIdx.x == 0) ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
 
@@ -146,65 +156,89 @@ __global__ __launch_bounds__(TB) void k_gather_score_lp(GatherArgs a, LowPrecArg
             q8[t] = (long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
         }
     }
-    // approximate scores, 2 tiles of 16 candidates per wave step
-    WaveTopK<KPL> tk;
-    tk.init(R);
-    constexpr uint64_t LEADERS = 0x0001000100010001ull;  // lanes 0, 16, 32, 48
-    for (uint32_t t0 = (uint32_t)wv * 32; t0 < nc; t0 += (uint32_t)NW * 32) {
-        int32_t id[2];
+    // approximate scores, NT tiles of 16 candidates per wave step (the ids of
+    // the next step fetched one step ahead), as sort keys by candidate
+    // position: in LDS over the dedup bitset when they fit, else in the
+    // query's scratch segment
+    const uint32_t lds_cap = (uint32_t)Wwords;
+    uint32_t* keys = nc <= lds_cap ? bits : reinterpret_cast<uint32_t*>(a.part + cbase);
+    constexpr int NT = lp_tiles(FMT, KT);
+    constexpr uint32_t SPAN = 16 * NT;
+    const uint32_t stride = (uint32_t)NW * SPAN;
+    int32_t idn[NT];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const uint32_t i = t0 + h * 16 + r16;
-            id[h] = i < nc ? cq[i] : -1;
+    for (int h = 0; h < NT; ++h) {
+        const uint32_t i = (uint32_t)wv * SPAN + h * 16 + r16;
+        idn[h] = i < nc ? cq[i] : -1;
+    }
+    for (uint32_t t0 = (uint32_t)wv * SPAN; t0 < nc; t0 += stride) {
+        int32_t id[NT];
+        float rs[NT];
+#pragma unroll
+        for (int h = 0; h < NT; ++h) {
+            id[h] = idn[h];
+            const uint32_t i = t0 + stride + h * 16 + r16;
+            idn[h] = i < nc ? cq[i] : -1;
+            if constexpr (FMT == FP8) rs[h] = id[h] < 0 ? 0.0f : lp.scale[id[h]];
         }
-        f32x4 acc[2];
+        f32x4 acc[NT];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) acc[h] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int h = 0; h < NT; ++h) acc[h] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // k-steps past the row end re-read its last step against a zero
+        // query fragment (adding exactly 0): no branches, so all NT * KT
+        // loads of a step are in flight before the first MFMA waits
+        if constexpr (FMT == BF16) {
+            bf16x8 av[KT][NT];
 #pragma unroll
-        for (int t = 0; t < KT; ++t) {
-            if (t < kt) {
+            for (int t = 0; t < KT; ++t) {
+                const int tt = t < kt ? t : kt - 1;
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < NT; ++h) {
                     const int64_t row = id[h] < 0 ? 0 : id[h];
-                    const int64_t off = row * lp.ldl + 32 * t + 8 * g16;
-                    if constexpr (FMT == BF16) {
-                        const bf16x8 av = *reinterpret_cast<const bf16x8*>(lp.xb + off);
-                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, qb[t], acc[h], 0, 0, 0);
-                    } else {
-                        const long av = *reinterpret_cast<const long*>(lp.x8 + off);
-                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av, q8[t], acc[h], 0, 0, 0);
-                    }
+                    av[t][h] = *reinterpret_cast<const bf16x8*>(lp.xb + row * lp.ldl + 32 * tt + 8 * g16);
                 }
             }
-        }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+            for (int t = 0; t < KT; ++t)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int m = 4 * g16 + i;  // row of this accumulator
-                const int32_t idm = __shfl(id[h], m, WAVE);
-                float sc = acc[h][i];
-                if constexpr (FMT == FP8) sc *= (idm < 0 ? 0.0f : lp.scale[idm]) * qscale;
-                tk.offer(score_key(sc, idm), LEADERS);
+                for (int h = 0; h < NT; ++h)
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t][h], qb[t], acc[h], 0, 0, 0);
+        } else {
+            long av[KT][NT];
+#pragma unroll
+            for (int t = 0; t < KT; ++t) {
+                const int tt = t < kt ? t : kt - 1;
+#pragma unroll
+                for (int h = 0; h < NT; ++h) {
+                    const int64_t row = id[h] < 0 ? 0 : id[h];
+                    av[t][h] = *reinterpret_cast<const long*>(lp.x8 + row * lp.ldl + 32 * tt + 8 * g16);
+                }
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < KT; ++t)
+#pragma unroll
+                for (int h = 0; h < NT; ++h)
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av[t][h], q8[t], acc[h], 0, 0, 0);
+        }
+        // accumulator i of lane (g16, r16) holds row 4*g16 + i of the tile
+        // (every column alike); lane (g16, r16 < 4) stores row 4*g16 + r16
+        const int m = 4 * g16 + (r16 & 3);
+#pragma unroll
+        for (int h = 0; h < NT; ++h) {
+            float sc = r16 == 0 ? acc[h][0] : (r16 == 1 ? acc[h][1] : (r16 == 2 ? acc[h][2] : acc[h][3]));
+            if constexpr (FMT == FP8) sc *= __shfl(rs[h], m, WAVE) * qscale;
+            const uint32_t i = t0 + h * 16 + m;
+            if (r16 < 4 && i < nc) keys[i] = desc_key_any(sc);
         }
     }
-    // merge the waves' lists into the best R overall
-#pragma unroll
-    for (int s = 0; s < KPL; ++s) {
-        const int i = s * WAVE + lane;
-        if (i < R) wkeys[wv * R + i] = tk.w[s];
-    }
     __syncthreads();
-    const uint32_t rr = min((uint32_t)R, nc);
-    for (int e = threadIdx.x; e < NW * R; e += TB) {
-        const uint64_t ke = wkeys[e];
-        if (ke == ~0ull) continue;
-        uint32_t r = 0;
-        for (int o = 0; o < NW * R; ++o) r += wkeys[o] < ke ? 1u : 0u;
-        if (r < rr) rid[r] = (int32_t)(uint32_t)(ke & 0xffffffffu);
-    }
-    __syncthreads();
+    // the R best approximate scores (ties: smaller id)
+    const uint32_t rr = nc <= lds_cap
+                            ? select_smallest<TB>(bits, cq, nc, (uint32_t)R, hist, sh, &s_digit, &s_below, count, rid)
+                            : select_smallest<TB>(reinterpret_cast<const uint32_t*>(a.part + cbase), cq, nc,
+                                                  (uint32_t)R, hist, sh, &s_digit, &s_below, count, rid);
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
     // exact FP32 rescoring of the survivors, 8 lanes per row
     const int g8 = threadIdx.x & 7;
@@ -252,20 +286,12 @@ __global__ __launch_bounds__(TB) void k_gather_score_lp(GatherArgs a, LowPrecArg
 constexpr int LP_TB = 512;
 
 template <int FMT, int KT>
-void dispatch_lp_kpl(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t lds, const uint32_t* probes,
-                     const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
-                     const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids, float* out_scores,
-                     hipStream_t st) {
-#define FPP_LP(KPLV)                                                                                          \
-    k_gather_score_lp<LP_TB, FMT, KT, KPLV><<<nq, LP_TB, lds, st>>>(a, lp, probes, bucket_off, ids, Xc, Q, ldq, \
-                                                                   cand_off, cand, ncand, out_ids, out_scores)
-    if (lp.refine <= WAVE)
-        FPP_LP(1);
-    else if (lp.refine <= 2 * WAVE)
-        FPP_LP(2);
-    else
-        FPP_LP(4);
-#undef FPP_LP
+void dispatch_lp_launch(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t lds, const uint32_t* probes,
+                        const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
+                        const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids,
+                        float* out_scores, hipStream_t st) {
+    k_gather_score_lp<LP_TB, FMT, KT><<<nq, LP_TB, lds, st>>>(a, lp, probes, bucket_off, ids, Xc, Q, ldq, cand_off,
+                                                             cand, ncand, out_ids, out_scores);
 }
 
 template <int FMT>
@@ -275,16 +301,16 @@ void dispatch_lp_kt(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t l
                     hipStream_t st) {
     const int64_t kt = lp.ldl / 32;
     if (kt <= 4)
-        dispatch_lp_kpl<FMT, 4>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+        dispatch_lp_launch<FMT, 4>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
                                 out_scores, st);
     else if (kt <= 8)
-        dispatch_lp_kpl<FMT, 8>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+        dispatch_lp_launch<FMT, 8>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
                                 out_scores, st);
     else if (kt <= 16)
-        dispatch_lp_kpl<FMT, 16>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+        dispatch_lp_launch<FMT, 16>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
                                  out_scores, st);
     else
-        dispatch_lp_kpl<FMT, 32>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+        dispatch_lp_launch<FMT, 32>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
                                  out_scores, st);
 }
 
@@ -292,12 +318,9 @@ void dispatch_lp_kt(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t l
 
 int lowprec_threads() { return LP_TB; }
 
-// LDS words beyond the bitset and cursors: scan scratch, reduction, wave
-// lists, survivors.
-int64_t lowprec_extra_words(int refine) {
-    const int NW = LP_TB / WAVE;
-    return LP_TB + 64 + 2 * ((int64_t)NW * refine + refine) + refine + 64;
-}
+// LDS words beyond the bitset and window counters: scan scratch, reduction,
+// histogram, survivors (exact keys and ids), append counter.
+int64_t lowprec_extra_words(int refine) { return LP_TB + 64 + 256 + 3 * (int64_t)refine + 2; }
 
 void launch_quant_bf16(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint16_t* Xb, hipStream_t st) {
     const int64_t total = N * ldl;
@@ -323,8 +346,8 @@ void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, int nq, 
     FPP_REQUIRE(lp.ldl % 32 == 0 && lp.ldl / 32 <= 32, FPP_EINVAL, "n_features must be <= 1024");
     if (nq == 0) return;
     const int64_t wwords = (a.window_bits / 32 + 1) & ~1;
-    const int64_t curw = a.N > a.window_bits ? ((a.qProbes + 1) & ~1) : 0;
-    const size_t lds = (size_t)(wwords + curw + lowprec_extra_words(lp.refine)) * 4;
+    const size_t lds =
+        (size_t)(wwords + window_counter_words(a.N, a.window_bits) + lowprec_extra_words(lp.refine)) * 4;
     FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "LDS budget exceeded (refine count or qProbes too large)");
     if (lp.format == BF16)
         dispatch_lp_kt<BF16>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,

@@ -326,7 +326,7 @@ size_t merge_lds_bytes(const ProbeGeom& g) {
 
 // ---- gather + dedup + FP32 rerank + top-k ---------------------------------
 template <int TB, int CPL>
-__global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uint32_t* __restrict__ probes,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB / 256))) void k_gather_score_f32(GatherArgs a, const uint32_t* __restrict__ probes,
                                                           const uint64_t* __restrict__ bucket_off,
                                                           const int32_t* __restrict__ ids,
                                                           const float* __restrict__ Xc, const float* __restrict__ Q,
@@ -337,12 +337,10 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
     extern __shared__ uint32_t lds[];
     const int64_t W = a.window_bits;
     const int Wwords = (int)(W / 32);
-    const bool multi = a.N > W;
     uint32_t* bits = lds;
-    // per-probe cursors into the (ascending) bucket id lists, kept across
-    // windows so every id is read once
-    uint32_t* cur = bits + ((Wwords + 1) & ~1);
-    uint32_t* sh = cur + (multi ? ((a.qProbes + 1) & ~1) : 0);  // TB (8-byte aligned from here on)
+    // window counters when N exceeds one window
+    uint32_t* wc = bits + ((Wwords + 1) & ~1);
+    uint32_t* sh = wc + window_counter_words(a.N, W);  // TB (8-byte aligned from here on)
     uint32_t* hist = sh + TB;           // 256
     uint32_t* tk_key = hist + 256;      // kcap
     uint32_t* tk_id = tk_key + 1024;    // kcap
@@ -356,12 +354,15 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
     const uint64_t ccap = cand_off[q + 1] - cbase;
     int32_t* cq = cand + cbase;
     float* sq = score + cbase;
-    const uint32_t nc = collect_candidates<TB>(a, pr, bucket_off, ids, bits, cur, sh, cq, ccap, ts);
+    const uint32_t nc = collect_candidates<TB>(a, pr, bucket_off, ids, bits, wc, sh, cq, a.part + cbase, ccap, ts);
     if (threadIdx.x == 0) ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
     // FP32 scores: groups of 8 lanes per candidate row, CPL float4 chunks per
     // lane, U rows per group in flight, candidate ids fetched one step ahead.
-    constexpr int U = 4;
+    // Chunks past the row end re-read its last chunk against a zero query
+    // chunk (adding exactly 0), so the loads carry no branches and a step's
+    // CPL * U loads are all in flight before the first use.
+    constexpr int U = CPL >= 16 ? 32 / CPL : 4;
     const int lane = threadIdx.x & (WAVE - 1);
     const int g8 = threadIdx.x & 7;
     const int grp = threadIdx.x >> 3;
@@ -398,18 +399,23 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
         float sc[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) sc[u] = 0.f;
+        float4 x[CPL][U];
 #pragma unroll
         for (int t = 0; t < CPL; ++t) {
-            const int64_t ch = g8 + 8 * t;
-            if (ch < nchunk) {
+            const int64_t ch = min<int64_t>(g8 + 8 * t, nchunk - 1);
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const float4 x = reinterpret_cast<const float4*>(Xc + (int64_t)(id[u] < 0 ? 0 : id[u]) * a.ldx)[ch];
-                    sc[u] = fmaf(qr[t].x, x.x, sc[u]);
-                    sc[u] = fmaf(qr[t].y, x.y, sc[u]);
-                    sc[u] = fmaf(qr[t].z, x.z, sc[u]);
-                    sc[u] = fmaf(qr[t].w, x.w, sc[u]);
-                }
+            for (int u = 0; u < U; ++u)
+                x[t][u] = reinterpret_cast<const float4*>(Xc + (int64_t)(id[u] < 0 ? 0 : id[u]) * a.ldx)[ch];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                sc[u] = fmaf(qr[t].x, x[t][u].x, sc[u]);
+                sc[u] = fmaf(qr[t].y, x[t][u].y, sc[u]);
+                sc[u] = fmaf(qr[t].z, x[t][u].z, sc[u]);
+                sc[u] = fmaf(qr[t].w, x[t][u].w, sc[u]);
             }
         }
 #pragma unroll
@@ -454,7 +460,7 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
         }
     } else {
         __syncthreads();
-        // radix select of the k-th smallest descending key, ties -> smaller id
+        // radix select of the k-th smallest descending key
         uint32_t prefix = 0, pmask = 0, k = kk, eqc = 0;
         if (kk > 0) {
             for (int shift = 24; shift >= 0; shift -= 8) {
@@ -474,23 +480,41 @@ __global__ __launch_bounds__(TB) void k_gather_score_f32(GatherArgs a, const uin
             }
         }
         const uint32_t T = prefix;
-        uint32_t written = 0, eq_taken = 0;
+        // when more candidates share the k-th key than are needed, a second
+        // radix select over their (unique) ids keeps the k smallest
+        uint32_t IT = 0xffffffffu;
+        if (kk > 0 && eqc > k) {
+            uint32_t ip = 0, im = 0, kr = k;
+            for (int shift = 24; shift >= 0; shift -= 8) {
+                for (int i = threadIdx.x; i < 256; i += TB) hist[i] = 0;
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < nc; i += TB) {
+                    const uint32_t id = (uint32_t)cq[i];
+                    if (desc_key_any(sq[i]) == T && (id & im) == ip) atomicAdd(&hist[(id >> shift) & 255u], 1u);
+                }
+                __syncthreads();
+                int d;
+                uint32_t below, eqd;
+                block_find_digit<TB>(hist, kr, sh, d, below, eqd, &s_digit, &s_below);
+                ip |= (uint32_t)d << shift;
+                im |= 255u << shift;
+                kr -= below;
+            }
+            IT = ip;
+        }
+        uint32_t written = 0;
         for (uint32_t c0 = 0; c0 < nc && kk > 0; c0 += TB) {
             const uint32_t i = c0 + threadIdx.x;
             const uint32_t key = i < nc ? desc_key_any(sq[i]) : 0xffffffffu;
-            const uint32_t lt = (i < nc && key < T) ? 1u : 0u;
-            const uint32_t eqk = (i < nc && key == T) ? 1u : 0u;
-            uint32_t tot_eq;
-            const uint32_t ex_eq = block_excl_scan_u32<TB>(eqk, sh, tot_eq);
-            const uint32_t take = lt | ((eqk && eq_taken + ex_eq < k) ? 1u : 0u);
+            const uint32_t id = i < nc ? (uint32_t)cq[i] : 0xffffffffu;
+            const uint32_t take = (i < nc && (key < T || (key == T && id <= IT))) ? 1u : 0u;
             uint32_t tot_take;
             const uint32_t ex_take = block_excl_scan_u32<TB>(take, sh, tot_take);
             if (take && written + ex_take < kk) {
                 tk_key[written + ex_take] = key;
-                tk_id[written + ex_take] = (uint32_t)cq[i];
+                tk_id[written + ex_take] = id;
             }
             written += tot_take;
-            eq_taken += min(tot_eq, k - min(k, eq_taken));
         }
         __syncthreads();
         // order the kept entries by (key, id): rank by counting (kk <= 1024)
@@ -535,8 +559,7 @@ __global__ void k_xcd_order(const uint32_t* __restrict__ sorted, int nq, int nxc
 
 size_t gather_lds_bytes(const GatherArgs& a, int TB) {
     const int64_t wwords = (a.window_bits / 32 + 1) & ~1;
-    const int64_t curw = a.N > a.window_bits ? ((a.qProbes + 1) & ~1) : 0;
-    return (size_t)(wwords + curw + TB + 256 + 2048) * 4;
+    return (size_t)(wwords + window_counter_words(a.N, a.window_bits) + TB + 256 + 2048) * 4;
 }
 
 template <int TB>
@@ -609,15 +632,16 @@ void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, 
     FPP_HIP(hipGetLastError());
 }
 
-// extra_words: LDS words the kernel needs beyond the bitset and cursors.
-int64_t gather_window_bits(int64_t N, int qProbes, int64_t extra_words) {
+// extra_words: LDS words the kernel needs beyond the bitset and window counters.
+int64_t gather_window_bits(int64_t N, int64_t extra_words) {
     const int64_t fixed = (extra_words + 64) * 4;  // + static LDS margin
     const int64_t budget = 160 * 1024 - fixed;
     const int64_t single = (N + 31) / 32 * 32;
     if (single / 8 + 8 <= budget) return single;
-    const int64_t w = ((budget - 4 * ((int64_t)qProbes + 2)) * 8) / 64 * 64;
-    FPP_REQUIRE(w >= (1 << 16), FPP_EINVAL, "qProbes too large for the dedup window in LDS");
-    return w;
+    for (int64_t w = int64_t(1) << 20; w >= (int64_t(1) << 16); w >>= 1)
+        if (w / 8 + 8 + 4 * window_counter_words(N, w) <= budget) return w;
+    FPP_REQUIRE(false, FPP_EINVAL, "n_points too large for the dedup window in LDS");
+    return 0;
 }
 
 void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,

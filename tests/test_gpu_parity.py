@@ -151,6 +151,25 @@ def test_query_probes_candidates_topk(name):
     assert st["ctot"] == ctot
 
 
+@pytest.mark.parametrize("k", [10, 64, 65, 200])
+def test_topk_ties_break_to_smaller_id(k):
+    """Integer data and integer queries: every score is an exact integer, so
+    equal scores are common; both top-k paths (wave lists for k <= 64, radix
+    select beyond) must keep the smaller ids among equal scores."""
+    X, _, g, _ = _built("int_grid_ties")
+    assert np.all(g.center() == 0)
+    Q = X[:16].copy()
+    g.set_qProbes(40)
+    out = g.query(Q, k)
+    for j, q in enumerate(Q):
+        c = g.last_candidates(j)
+        s = X[c].astype(np.float64) @ q.astype(np.float64)
+        order = np.lexsort((c, -s))
+        m = min(k, len(c))
+        assert np.array_equal(out[j, :m], c[order[:m]]), j
+        assert np.all(out[j, m:] == 0)
+
+
 def test_tie_replay_paths_exercised():
     X, Q, g, o = _built("int_grid_ties")
     st = g.stats()
