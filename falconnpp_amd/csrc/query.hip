@@ -104,8 +104,9 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     }
     if (tie) atomicOr(&s_tie, 1u);
     __syncthreads();
-    // global top-qProbes of the pool
+    // global top-qProbes of the pool: radix select, the digit found by wave 0
     uint32_t prefix = 0, pmask = 0, k = (uint32_t)g.qProbes, eqc = 0;
+    __shared__ uint32_t s_eqc, s_cnt;
     for (int shift = 24; shift >= 0; shift -= 8) {
         bhist[threadIdx.x] = 0;
         __syncthreads();
@@ -114,38 +115,46 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
             if ((key & pmask) == prefix) atomicAdd(&bhist[(key >> shift) & 255u], 1u);
         }
         __syncthreads();
-        int d;
-        uint32_t below;
-        block_find_digit<PS_T>(bhist, k, sh, d, below, eqc, &s_digit, &s_below);
-        prefix |= (uint32_t)d << shift;
+        if (w == 0) {
+            int d;
+            uint32_t below, e;
+            wave_find_digit(bhist, k, d, below, e);
+            if (lane == 0) {
+                s_digit = d;
+                s_below = below;
+                s_eqc = e;
+            }
+        }
+        __syncthreads();
+        prefix |= (uint32_t)s_digit << shift;
         pmask |= 255u << shift;
-        k -= below;
+        k -= s_below;
+        eqc = s_eqc;
     }
     if (eqc != k && threadIdx.x == 0) atomicOr(&s_tie, 2u);
+    // every key above the cut and every key on it: exactly qProbes when the
+    // cut is unambiguous; otherwise the first qProbes appended (the query is
+    // flagged and its probes replayed exactly)
     const uint32_t T = prefix;
-    uint32_t written = 0, eq_taken = 0;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
     unsigned long long my_ctot = 0;
     uint32_t* out = probes + (size_t)q * g.qProbes;
     for (int c0 = 0; c0 < pool_n; c0 += PS_T) {
         const int i = c0 + threadIdx.x;
-        const uint32_t key = i < pool_n ? pkey[i] : 0xffffffffu;
-        const uint32_t lt = (i < pool_n && key < T) ? 1u : 0u;
-        const uint32_t eqk = (i < pool_n && key == T) ? 1u : 0u;
-        uint32_t tot_eq;
-        const uint32_t ex_eq = block_excl_scan_u32<PS_T>(eqk, sh, tot_eq);
-        const uint32_t take = lt | ((eqk && eq_taken + ex_eq < k) ? 1u : 0u);
-        uint32_t tot_take;
-        const uint32_t ex_take = block_excl_scan_u32<PS_T>(take, sh, tot_take);
-        if (take) {
-            const uint32_t pos = written + ex_take;
-            if (pos < (uint32_t)g.qProbes) {
-                const uint32_t b = pbkt[i];
-                out[pos] = b;
-                my_ctot += bucket_off[b + 1] - bucket_off[b];
-            }
+        const bool take = i < pool_n && pkey[i] <= T;
+        const uint64_t m = __ballot(take);
+        if (m == 0ull) continue;
+        const int leader = (int)__ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&s_cnt, (uint32_t)__popcll(m));
+        base = __shfl(base, leader, WAVE);
+        const uint32_t pos = base + lane_rank(m);
+        if (take && pos < (uint32_t)g.qProbes) {
+            const uint32_t b = pbkt[i];
+            out[pos] = b;
+            my_ctot += bucket_off[b + 1] - bucket_off[b];
         }
-        written += tot_take;
-        eq_taken += min(tot_eq, k - min(k, eq_taken));
     }
     atomicAdd(&s_ctot, my_ctot);
     __syncthreads();
@@ -156,27 +165,35 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
 }
 
 // ---- exact replay of a flagged query's probing sequence -------------------
-__device__ void fwht_serial(float* a, int F) {
-    for (int h = 1; h < F; h <<= 1)
-        for (int i = 0; i < F; i += 2 * h)
-            for (int j = i; j < i + h; ++j) {
-                const float u = a[j], v = a[j + h];
-                a[j] = u + v;
-                a[j + h] = u - v;
-            }
-}
-
-__device__ void rotate_serial(const float* src, int D, int F, const uint32_t* masks_layer, float* x) {
+// One wave per (flagged query, table).  Rotation of the query for one layer
+// into x (F floats of LDS): three rounds of sign flips and the unnormalised
+// FWHT with the stages in ascending stride (src/FalconnPP.cpp:392-411 via
+// FFHT), butterflies of a stage spread over the lanes (bit-identical: the
+// butterflies of a stage are independent).
+__device__ void rotate_wave(const float* src, int D, int F, const uint32_t* masks_layer, float* x) {
+    const int lane = threadIdx.x & (WAVE - 1);
     const int MW = (F + 31) / 32;
-    for (int d = 0; d < F; ++d) x[d] = d < D ? src[d] : 0.0f;
+    for (int d = lane; d < F; d += WAVE) x[d] = d < D ? src[d] : 0.0f;
     for (int r = 0; r < 3; ++r) {
         const uint32_t* mb = masks_layer + r * MW;
-        for (int d = 0; d < F; ++d)
+        __syncthreads();
+        for (int d = lane; d < F; d += WAVE)
             x[d] = __uint_as_float(__float_as_uint(x[d]) ^ (((mb[d >> 5] >> (d & 31)) & 1u) << 31));
-        fwht_serial(x, F);
+        for (int h = 1; h < F; h <<= 1) {
+            __syncthreads();
+            for (int b = lane; b < F / 2; b += WAVE) {
+                const int j = (b / h) * 2 * h + (b % h);
+                const float u = x[j], v = x[j + h];
+                x[j] = u + v;
+                x[j + h] = u - v;
+            }
+        }
     }
+    __syncthreads();
 }
 
+// The per-layer heap of src/FalconnPP.cpp:413-440 (min-heap of the p largest
+// |projections|, then popped smallest first into vec[p-1 .. 0]).
 __device__ void layer_heap(const float* x, int n_proj, int log2_np, int p, IFPair* hs, IFPair* vec) {
     MinHeap q;
     q.init(hs);
@@ -198,57 +215,57 @@ __device__ void layer_heap(const float* x, int n_proj, int log2_np, int p, IFPai
     }
 }
 
-// Work area of one flagged query: the per-table probe lists, the merge heap
-// and table cursors, then one scratch slot per table.
-__host__ __device__ inline int64_t exact_table_bytes(int F, int PL, int PT) {
-    return ((int64_t)2 * F * 4 + (int64_t)(3 * PL + PT) * 8 + 15) / 16 * 16;
+// LDS of one (query, table) wave: two rotated layers, two layer heaps and
+// their lists, the table's pair heap.
+__host__ __device__ inline int64_t exact_table_lds_bytes(int F, int PL, int PT) {
+    return (int64_t)2 * F * 4 + (int64_t)(4 * PL + PT) * 8;
 }
-__host__ __device__ inline int64_t exact_head_bytes(int L, int PT) {
-    return ((int64_t)L * PT * 8 + (int64_t)L * 8 + (int64_t)L * 4 + 15) / 16 * 16;
-}
-__host__ __device__ inline int64_t exact_work_bytes(int F, int L, int PL, int PT) {
-    return exact_head_bytes(L, PT) + (int64_t)L * exact_table_bytes(F, PL, PT);
-}
+// Work area of one flagged query in global memory: the per-table probe lists.
+__host__ __device__ inline int64_t exact_work_bytes(int L, int PT) { return ((int64_t)L * PT * 8 + 15) / 16 * 16; }
 
-// Per (flagged query, table): the table's probe list, src/FalconnPP.cpp:358-531
-// replayed with the emulated heaps.  When no per-layer tie was seen (qflag bit
-// 0 clear), the per-layer top lists of the fast kernel are exact (distinct
-// values, largest first, as the reference pops them) and are reused;
-// otherwise the table is rotated again and its layer heaps replayed.
-__global__ void k_probe_exact_tables(HashGeom hg, ProbeGeom g, const float* __restrict__ Q, int64_t ldq,
-                                     const uint32_t* __restrict__ masks, const uint32_t* __restrict__ qlist,
-                                     uint32_t nlist, const uint32_t* __restrict__ qflag,
-                                     const int32_t* __restrict__ qc, const float* __restrict__ qvals,
-                                     unsigned char* __restrict__ work, int64_t wpq) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nlist * g.L) return;
-    const uint32_t f = (uint32_t)(t / g.L);
-    const int l = (int)(t % g.L);
+// Per (flagged query, table), one wave: the table's probe list,
+// src/FalconnPP.cpp:358-531 replayed with the emulated heaps.  When no
+// per-layer tie was seen (qflag bit 0 clear), the per-layer top lists of the
+// fast kernel are exact (distinct values, largest first, as the reference
+// pops them) and are reused; otherwise the table is rotated again and its
+// layer heaps replayed.
+__global__ __launch_bounds__(WAVE) void k_probe_exact_tables(HashGeom hg, ProbeGeom g, const float* __restrict__ Q,
+                                                             int64_t ldq, const uint32_t* __restrict__ masks,
+                                                             const uint32_t* __restrict__ qlist,
+                                                             const uint32_t* __restrict__ qflag,
+                                                             const int32_t* __restrict__ qc,
+                                                             const float* __restrict__ qvals,
+                                                             unsigned char* __restrict__ work, int64_t wpq) {
+    extern __shared__ float xs[];
+    const uint32_t f = blockIdx.x / g.L;
+    const int l = (int)(blockIdx.x % g.L);
     const uint32_t q = qlist[f];
     const int F = hg.F, PL = g.PL, PT = g.perTable;
-    unsigned char* wp = work + (int64_t)f * wpq;
-    IFPair* list = reinterpret_cast<IFPair*>(wp);
-    unsigned char* ts = wp + exact_head_bytes(g.L, PT) + (int64_t)l * exact_table_bytes(F, PL, PT);
-    float* x1 = reinterpret_cast<float*>(ts);
+    const int lane = threadIdx.x;
+    float* x1 = xs;
     float* x2 = x1 + F;
-    IFPair* hs = reinterpret_cast<IFPair*>(x2 + F);
-    IFPair* v1 = hs + PL;
+    IFPair* hs1 = reinterpret_cast<IFPair*>(x2 + F);
+    IFPair* hs2 = hs1 + PL;
+    IFPair* v1 = hs2 + PL;
     IFPair* v2 = v1 + PL;
     IFPair* tq = v2 + PL;
     if (qflag[q] & 1u) {
         const int MW = (F + 31) / 32;
         const float* qv = Q + (int64_t)q * ldq;
-        rotate_serial(qv, hg.D, F, masks + (size_t)(l * 2 + 0) * 3 * MW, x1);
-        rotate_serial(qv, hg.D, F, masks + (size_t)(l * 2 + 1) * 3 * MW, x2);
-        layer_heap(x1, hg.n_proj, hg.log2_nproj, PL, hs, v1);
-        layer_heap(x2, hg.n_proj, hg.log2_nproj, PL, hs, v2);
+        rotate_wave(qv, hg.D, F, masks + (size_t)(l * 2 + 0) * 3 * MW, x1);
+        rotate_wave(qv, hg.D, F, masks + (size_t)(l * 2 + 1) * 3 * MW, x2);
+        if (lane == 0) layer_heap(x1, hg.n_proj, hg.log2_nproj, PL, hs1, v1);
+        if (lane == 1) layer_heap(x2, hg.n_proj, hg.log2_nproj, PL, hs2, v2);
     } else {
         const size_t base = ((size_t)q * g.L + l) * 2 * PL;
-        for (int i = 0; i < PL; ++i) {
+        for (int i = lane; i < PL; i += WAVE) {
             v1[i] = IFPair{qc[base + i], qvals[base + i]};
             v2[i] = IFPair{qc[base + PL + i], qvals[base + PL + i]};
         }
     }
+    __syncthreads();
+    if (lane != 0) return;
+    IFPair* list = reinterpret_cast<IFPair*>(work + (int64_t)f * wpq);
     MinHeap th;
     th.init(tq);
     for (int i = 0; i < PL; ++i)
@@ -273,13 +290,76 @@ __global__ void k_probe_exact_tables(HashGeom hg, ProbeGeom g, const float* __re
 }
 
 // Per flagged query: the global merge of src/FalconnPP.cpp:540-582.  The
-// per-table lists are staged in LDS by the workgroup; one lane then replays
-// the max-heap merge out of LDS (heap and cursors in LDS too).
+// per-table lists are staged in LDS by the workgroup; the first wave then
+// replays the max-heap merge with the heap and the table cursors in its
+// registers (one table per lane and slot).
 constexpr int MG_T = 256;
+template <int S>
 __global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const uint32_t* __restrict__ qlist,
                                                             const uint64_t* __restrict__ bucket_off,
                                                             uint32_t* __restrict__ probes, uint64_t* __restrict__ ctot,
                                                             const unsigned char* __restrict__ work, int64_t wpq) {
+    extern __shared__ IFPair mlds[];
+    const uint32_t f = blockIdx.x;
+    const uint32_t q = qlist[f];
+    const int L = g.L, PT = g.perTable;
+    const IFPair* list_g = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
+    IFPair* list = mlds;
+    for (int64_t i = threadIdx.x; i < (int64_t)L * PT; i += MG_T) list[i] = list_g[i];
+    __syncthreads();
+    uint32_t* out = probes + (size_t)q * g.qProbes;
+    if (threadIdx.x < WAVE) {
+        const int lane = threadIdx.x;
+        WaveMaxHeap<S> mh;
+        // table cursors: table l's in lane l % 64 of register l / 64
+        int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+        for (int l = 0; l < L; ++l) {
+            const IFPair e = list[(int64_t)l * PT];
+            mh.push(heap_key(e.val), e.idx);
+        }
+        for (int c = 0; c < g.qProbes; ++c) {
+            const int32_t top = mh.id_at(0);
+            mh.pop();
+            if (lane == 0) out[c] = (uint32_t)top;
+            const int tb = top / g.NB;
+            const int tl = tb & 63, ts = tb >> 6;
+            int32_t tk = __builtin_amdgcn_readlane(c0, tl);
+            if (S > 1 && ts == 1) tk = __builtin_amdgcn_readlane(c1, tl);
+            if (S > 2 && ts == 2) tk = __builtin_amdgcn_readlane(c2, tl);
+            if (S > 3 && ts == 3) tk = __builtin_amdgcn_readlane(c3, tl);
+            ++tk;
+            const bool mine = lane == tl;
+            if (mine && ts == 0) c0 = tk;
+            if (S > 1 && mine && ts == 1) c1 = tk;
+            if (S > 2 && mine && ts == 2) c2 = tk;
+            if (S > 3 && mine && ts == 3) c3 = tk;
+            if (tk < PT) {
+                const IFPair e = list[(int64_t)tb * PT + tk];
+                mh.push(heap_key(e.val), e.idx);
+            }
+        }
+    }
+    __syncthreads();
+    // C_tot of the probed set
+    __shared__ unsigned long long s_ct;
+    if (threadIdx.x == 0) s_ct = 0;
+    __syncthreads();
+    unsigned long long ct = 0;
+    for (int c = threadIdx.x; c < g.qProbes; c += MG_T) {
+        const uint32_t b = out[c];
+        ct += bucket_off[b + 1] - bucket_off[b];
+    }
+    atomicAdd(&s_ct, ct);
+    __syncthreads();
+    if (threadIdx.x == 0) ctot[q] = s_ct;
+}
+
+// The same merge for more than 256 tables: one lane, heap and cursors in LDS.
+__global__ __launch_bounds__(MG_T) void k_probe_exact_merge_lds(ProbeGeom g, const uint32_t* __restrict__ qlist,
+                                                                const uint64_t* __restrict__ bucket_off,
+                                                                uint32_t* __restrict__ probes,
+                                                                uint64_t* __restrict__ ctot,
+                                                                const unsigned char* __restrict__ work, int64_t wpq) {
     extern __shared__ IFPair mlds[];
     const uint32_t f = blockIdx.x;
     const uint32_t q = qlist[f];
@@ -306,7 +386,6 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const u
         }
     }
     __syncthreads();
-    // C_tot of the probed set
     __shared__ unsigned long long s_ct;
     if (threadIdx.x == 0) s_ct = 0;
     __syncthreads();
@@ -321,7 +400,8 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const u
 }
 
 size_t merge_lds_bytes(const ProbeGeom& g) {
-    return (size_t)g.L * g.perTable * sizeof(IFPair) + (size_t)g.L * sizeof(IFPair) + (size_t)g.L * sizeof(int);
+    const size_t lists = (size_t)g.L * g.perTable * sizeof(IFPair);
+    return g.L <= 4 * WAVE ? lists : lists + (size_t)g.L * (sizeof(IFPair) + sizeof(int));
 }
 
 // ---- gather + dedup + FP32 rerank + top-k ---------------------------------
@@ -613,7 +693,7 @@ void launch_xcd_order(const uint32_t* sorted, int nq, int nxcd, uint32_t* order,
 }
 
 int64_t probe_exact_work_bytes(const HashGeom& hg, const ProbeGeom& g) {
-    return (exact_work_bytes(hg.F, g.L, g.PL, g.perTable) + 255) / 256 * 256;
+    return (exact_work_bytes(g.L, g.perTable) + 255) / 256 * 256;
 }
 
 void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, int64_t ldq,
@@ -622,13 +702,26 @@ void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, 
                         uint64_t* ctot, void* work, int64_t work_per_query, hipStream_t st) {
     if (nlist == 0) return;
     unsigned char* w = reinterpret_cast<unsigned char*>(work);
-    const int64_t nt = (int64_t)nlist * g.L;
-    k_probe_exact_tables<<<(unsigned)ceil_div(nt, 64), 64, 0, st>>>(hg, g, Q, ldq, masks, qlist, nlist, qflag, qc, qv,
-                                                                   w, work_per_query);
+    const size_t tlds = (size_t)exact_table_lds_bytes(hg.F, g.PL, g.perTable);
+    FPP_REQUIRE(tlds <= 160 * 1024, FPP_EINVAL, "perTable too large for the exact probe replay");
+    k_probe_exact_tables<<<(unsigned)((int64_t)nlist * g.L), WAVE, tlds, st>>>(hg, g, Q, ldq, masks, qlist, qflag,
+                                                                            qc, qv, w, work_per_query);
     FPP_HIP(hipGetLastError());
     const size_t lds = merge_lds_bytes(g);
     FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "probe lists (n_tables * perTable) too large for the exact merge");
-    k_probe_exact_merge<<<nlist, MG_T, lds, st>>>(g, qlist, bucket_off, probes, ctot, w, work_per_query);
+#define FPP_MG(SV) \
+    k_probe_exact_merge<SV><<<nlist, MG_T, lds, st>>>(g, qlist, bucket_off, probes, ctot, w, work_per_query)
+    if (g.L <= WAVE)
+        FPP_MG(1);
+    else if (g.L <= 2 * WAVE)
+        FPP_MG(2);
+    else if (g.L <= 3 * WAVE)
+        FPP_MG(3);
+    else if (g.L <= 4 * WAVE)
+        FPP_MG(4);
+    else
+        k_probe_exact_merge_lds<<<nlist, MG_T, lds, st>>>(g, qlist, bucket_off, probes, ctot, w, work_per_query);
+#undef FPP_MG
     FPP_HIP(hipGetLastError());
 }
 

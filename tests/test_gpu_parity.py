@@ -36,6 +36,8 @@ CASES = {
     "mix_d128_p2": (lambda: synth.gaussian_mixture(12000, 128, 60, seed=13), 8, 128, 10, 0.2, 2, 160, 10, 50),
     "nproj16_d40": (lambda: synth.lognormal_sphere(5000, 40, 0.3, seed=14), 5, 16, 8, 0.05, 2, 60, 15, 40),
     "int_grid_ties": (lambda: _sym_int_grid(3000, 24, -2, 2, 15), 4, 32, 5, 0.3, 2, 40, 10, 40),
+    # 200 tables: the exact probe merge keeps its heap in 4 register slots per lane
+    "int_ties_l200": (lambda: _sym_int_grid(2000, 16, -2, 2, 17), 200, 16, 5, 0.3, 2, 300, 10, 20),
     # N above the LDS bitset window (~1.2M ids): the dedup runs over two windows
     "multiwindow_n1.3m": (lambda: synth.gaussian_mixture(1_300_000, 16, 300, seed=16), 2, 16, 30, 0.05, 1, 8, 20, 12),
 }
