@@ -401,7 +401,27 @@ void launch_bucket_limit(const uint32_t* cnt, int64_t nbk, float alpha, int p, i
     FPP_HIP(hipGetLastError());
 }
 
+// Largest bucket (ids) of the built index.
+__global__ void k_max_bucket(const uint64_t* __restrict__ off, int64_t nbk, uint32_t* __restrict__ out) {
+    uint32_t m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbk; i += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, (uint32_t)(off[i + 1] - off[i]));
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, WAVE));
+    if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(out, m);
+}
+
 int64_t scan_tmp_elems(int64_t n) { return ceil_div(std::max<int64_t>(n, 1), SCAN_TILE) + 1; }
+
+uint32_t max_bucket_size(const uint64_t* off, int64_t nbk, uint32_t* scratch, hipStream_t st) {
+    FPP_HIP(hipMemsetAsync(scratch, 0, 4, st));
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nbk, 256), 4096));
+    k_max_bucket<<<grid, 256, 0, st>>>(off, nbk, scratch);
+    FPP_HIP(hipGetLastError());
+    uint32_t m = 0;
+    FPP_HIP(hipMemcpyAsync(&m, scratch, 4, hipMemcpyDeviceToHost, st));
+    FPP_HIP(hipStreamSynchronize(st));
+    return m;
+}
 
 void scan_u32_to_u64(const uint32_t* in, int64_t n, uint64_t* out, uint64_t* tmp, hipStream_t st) {
     const int64_t ntiles = ceil_div(std::max<int64_t>(n, 1), SCAN_TILE);

@@ -86,7 +86,14 @@ struct Index {
     int gather_threads = 512;
     bool order_queries = true;
     int wall_khz = 0;
-    DevBuf<uint32_t> okey, oval, okey2, oval2, ohist, order;
+    DevBuf<uint32_t> okey, oval, okey2, oval2, ohist, order, order_u;
+    // second stream: exact replay + gather of the flagged queries, beside the
+    // main gather of the others
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    std::vector<uint32_t> flagged;
+    uint32_t max_bucket = 0;  // largest bucket of the built index
+    DevBuf<uint32_t> scratch_word;
     DevBuf<uint64_t> ohist_off, oscan;
     DevBuf<uint64_t> phase_ts;
 
@@ -123,6 +130,9 @@ struct Index {
     ~Index() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -351,6 +361,8 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
         }
     }
     FPP_HIP(hipStreamSynchronize(st));
+    I.scratch_word.reserve(1);
+    I.max_bucket = max_bucket_size(I.bucket_off.get(), (int64_t)L * NB, I.scratch_word.get(), st);
     I.stat_bucket_ties = bucket_ties;
     I.built = true;
 }
@@ -423,33 +435,26 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     launch_probe_select(pg, nq, I.qc.get(), I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(),
                         I.qflag.get(), st);
     I.mark(2, st);
+    // flags and candidate totals of the fast path
     std::vector<uint32_t> flag(nq);
-    FPP_HIP(hipMemcpyAsync(flag.data(), I.qflag.get(), (size_t)nq * 4, hipMemcpyDeviceToHost, st));
-    FPP_HIP(hipStreamSynchronize(st));
-    std::vector<uint32_t> lst;
-    for (int q = 0; q < nq; ++q)
-        if (flag[q]) lst.push_back((uint32_t)q);
-    I.stat_qties = (int64_t)lst.size();
-    I.mark(3, st);
-    if (!lst.empty()) {
-        const int64_t wpq = probe_exact_work_bytes(hg, pg);
-        I.qlist.reserve(lst.size());
-        I.qwork.reserve((size_t)wpq * lst.size());
-        FPP_HIP(hipMemcpyAsync(I.qlist.get(), lst.data(), lst.size() * 4, hipMemcpyHostToDevice, st));
-        launch_probe_exact(hg, pg, I.Qpad.get(), I.ldx, I.masks.get(), I.qlist.get(), (uint32_t)lst.size(),
-                           I.qflag.get(), I.qc.get(), I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(),
-                           I.qwork.get(), wpq, st);
-    }
-    I.mark(4, st);
     std::vector<uint64_t> ct(nq);
+    FPP_HIP(hipMemcpyAsync(flag.data(), I.qflag.get(), (size_t)nq * 4, hipMemcpyDeviceToHost, st));
     FPP_HIP(hipMemcpyAsync(ct.data(), I.ctot.get(), (size_t)nq * 8, hipMemcpyDeviceToHost, st));
     FPP_HIP(hipStreamSynchronize(st));
+    std::vector<uint32_t>& lst = I.flagged;
+    lst.clear();
+    for (int q = 0; q < nq; ++q)
+        if (flag[q]) lst.push_back((uint32_t)q);
+    const uint32_t nl = (uint32_t)lst.size();
+    const int nu = nq - (int)nl;
+    I.stat_qties = (int64_t)nl;
+    // candidate segments: C_tot for the others; for a flagged query (its
+    // probes are replayed later) the qProbes largest buckets bound C_tot
+    const uint64_t flag_cap = std::min<uint64_t>((uint64_t)I.qProbes * I.max_bucket, (uint64_t)I.n_points);
     std::vector<uint64_t> off(nq + 1);
     off[0] = 0;
-    for (int q = 0; q < nq; ++q) {
-        off[q + 1] = off[q] + std::min<uint64_t>(ct[q], (uint64_t)I.n_points);
-        I.stat_ctot += (int64_t)ct[q];
-    }
+    for (int q = 0; q < nq; ++q)
+        off[q + 1] = off[q] + (flag[q] ? flag_cap : std::min<uint64_t>(ct[q], (uint64_t)I.n_points));
     I.cand_off.reserve(nq + 1);
     I.cand.reserve(off[nq]);
     I.score.reserve(off[nq]);
@@ -465,6 +470,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     const bool lowp = I.precision != 0;
     ga.window_bits = gather_window_bits(I.n_points, lowp ? lowprec_extra_words(I.refine_count(k))
                                                          : (int64_t)I.gather_threads + 256 + 2048);
+    const uint32_t* order = nullptr;
     if (I.order_queries && nq > 1) {
         const int bits = ilog2_exact(I.NB());
         const int64_t rh = radix_hist_elems(nq);
@@ -481,34 +487,71 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         radix_sort_pairs(I.okey.get(), I.oval.get(), I.okey2.get(), I.oval2.get(), nq, bits, true, I.ohist.get(),
                          I.ohist_off.get(), I.oscan.get(), &sk, &sv, st);
         launch_xcd_order(sv, nq, 8, I.order.get(), st);
-        ga.order = I.order.get();
+        order = I.order.get();
+    }
+    if (nl > 0) {
+        I.order_u.reserve(nq);
+        launch_split_order(order, nq, I.qflag.get(), I.order_u.get(), st);
+        order = I.order_u.get();
     }
     if (I.phase_stamps) {
         I.phase_ts.reserve((size_t)nq * 5);
         ga.phase_ts = I.phase_ts.get();
     }
     if (lowp) ensure_lowprec(I, st);
-    I.mark(5, st);
+    LowPrecArgs lp;
     if (lowp) {
-        LowPrecArgs lp;
         lp.format = I.precision;
         lp.refine = I.refine_count(k);
         lp.ldl = I.ldl;
         lp.xb = I.Xb.get();
         lp.x8 = I.X8.get();
         lp.scale = I.s8.get();
-        launch_gather_score_lp(ga, lp, nq, I.probes.get(), I.bucket_off.get(), I.ids.get(), I.Xc.get(),
-                               I.Qpad.get(), I.ldx, I.cand_off.get(), I.cand.get(), I.ncand.get(), d_out, d_scores,
-                               st);
-    } else {
-        launch_gather_score_f32(ga, nq, I.probes.get(), I.bucket_off.get(), I.ids.get(), I.Xc.get(), I.Qpad.get(),
-                                I.ldx, I.cand_off.get(), I.cand.get(), I.score.get(), I.ncand.get(), d_out, d_scores,
-                                I.gather_threads, st);
     }
+    auto gather = [&](const GatherArgs& a, int n, hipStream_t s2) {
+        if (lowp)
+            launch_gather_score_lp(a, lp, n, I.probes.get(), I.bucket_off.get(), I.ids.get(), I.Xc.get(),
+                                   I.Qpad.get(), I.ldx, I.cand_off.get(), I.cand.get(), I.ncand.get(), d_out,
+                                   d_scores, s2);
+        else
+            launch_gather_score_f32(a, n, I.probes.get(), I.bucket_off.get(), I.ids.get(), I.Xc.get(), I.Qpad.get(),
+                                    I.ldx, I.cand_off.get(), I.cand.get(), I.score.get(), I.ncand.get(), d_out,
+                                    d_scores, I.gather_threads, s2);
+    };
+    if (nl > 0) {
+        // flagged queries on the second stream: exact replay of their probes,
+        // then their gather; overlapped with the main gather below
+        if (!I.side) FPP_HIP(hipStreamCreateWithFlags(&I.side, hipStreamNonBlocking));
+        if (!I.ev_fork) FPP_HIP(hipEventCreateWithFlags(&I.ev_fork, hipEventDisableTiming));
+        if (!I.ev_join) FPP_HIP(hipEventCreateWithFlags(&I.ev_join, hipEventDisableTiming));
+        FPP_HIP(hipEventRecord(I.ev_fork, st));
+        FPP_HIP(hipStreamWaitEvent(I.side, I.ev_fork, 0));
+        const int64_t wpq = probe_exact_work_bytes(hg, pg);
+        I.qlist.reserve(nl);
+        I.qwork.reserve((size_t)wpq * nl);
+        FPP_HIP(hipMemcpyAsync(I.qlist.get(), lst.data(), (size_t)nl * 4, hipMemcpyHostToDevice, I.side));
+        I.mark(3, I.side);
+        launch_probe_exact(hg, pg, I.Qpad.get(), I.ldx, I.masks.get(), I.qlist.get(), nl, I.qflag.get(), I.qc.get(),
+                           I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(), I.qwork.get(), wpq, I.side);
+        I.mark(4, I.side);
+        GatherArgs gf = ga;
+        gf.order = I.qlist.get();
+        gather(gf, (int)nl, I.side);
+        FPP_HIP(hipEventRecord(I.ev_join, I.side));
+    } else {
+        I.mark(3, st);
+        I.mark(4, st);
+    }
+    I.mark(5, st);
+    ga.order = order;
+    gather(ga, nu, st);
     I.mark(6, st);
+    if (nl > 0) FPP_HIP(hipStreamWaitEvent(st, I.ev_join, 0));
     std::vector<uint64_t> nc(nq);
     FPP_HIP(hipMemcpyAsync(nc.data(), I.ncand.get(), (size_t)nq * 8, hipMemcpyDeviceToHost, st));
+    if (nl > 0) FPP_HIP(hipMemcpyAsync(ct.data(), I.ctot.get(), (size_t)nq * 8, hipMemcpyDeviceToHost, st));
     FPP_HIP(hipStreamSynchronize(st));
+    for (int q = 0; q < nq; ++q) I.stat_ctot += (int64_t)ct[q];
     if (I.profiling) {
         float a = 0, b = 0, c = 0, d = 0;
         FPP_HIP(hipEventElapsedTime(&a, I.ev[0], I.ev[1]));

@@ -110,80 +110,6 @@ __device__ __forceinline__ uint32_t heap_key(float v) {
     if (u == 0x80000000u) u = 0u;
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
-
-// MaxHeap (HeapQ<CmpLess>) with the same push / pop steps, its entries held
-// in the registers of one wave: entry i in lane i % 64, slot i / 64, S slots
-// (up to 64 * S entries).  All indices are wave-uniform, so an entry is read
-// with readlane and written by the lane that holds it; every lane runs the same control
-// flow.  Compares the integer keys of heap_key().
-template <int S>
-struct WaveMaxHeap {
-    static_assert(S >= 1 && S <= 4, "up to 256 entries");
-    // slots as named registers (no arrays, so nothing is placed in scratch)
-    uint32_t k0 = 0, k1 = 0, k2 = 0, k3 = 0;
-    int32_t i0 = 0, i1 = 0, i2 = 0, i3 = 0;
-    int n = 0;
-
-    __device__ __forceinline__ uint32_t k_at(int i) const {
-        const int l = i & 63, sl = i >> 6;
-        uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)k0, l);
-        if (S > 1 && sl == 1) r = (uint32_t)__builtin_amdgcn_readlane((int)k1, l);
-        if (S > 2 && sl == 2) r = (uint32_t)__builtin_amdgcn_readlane((int)k2, l);
-        if (S > 3 && sl == 3) r = (uint32_t)__builtin_amdgcn_readlane((int)k3, l);
-        return r;
-    }
-    __device__ __forceinline__ int32_t id_at(int i) const {
-        const int l = i & 63, sl = i >> 6;
-        int32_t r = __builtin_amdgcn_readlane(i0, l);
-        if (S > 1 && sl == 1) r = __builtin_amdgcn_readlane(i1, l);
-        if (S > 2 && sl == 2) r = __builtin_amdgcn_readlane(i2, l);
-        if (S > 3 && sl == 3) r = __builtin_amdgcn_readlane(i3, l);
-        return r;
-    }
-    __device__ __forceinline__ void set(int i, uint32_t k, int32_t v) {
-        const bool mine = (int)(threadIdx.x & 63) == (i & 63);
-        const int sl = i >> 6;
-        if (mine && sl == 0) { k0 = k; i0 = v; }
-        if (S > 1 && mine && sl == 1) { k1 = k; i1 = v; }
-        if (S > 2 && mine && sl == 2) { k2 = k; i2 = v; }
-        if (S > 3 && mine && sl == 3) { k3 = k; i3 = v; }
-    }
-    __device__ __forceinline__ void sift_up(int hole, int top, uint32_t k, int32_t v) {
-        while (hole > top) {
-            const int parent = (hole - 1) / 2;
-            const uint32_t pk = k_at(parent);
-            if (!(pk < k)) break;
-            set(hole, pk, id_at(parent));
-            hole = parent;
-        }
-        set(hole, k, v);
-    }
-    __device__ __forceinline__ void push(uint32_t k, int32_t v) {
-        n++;
-        sift_up(n - 1, 0, k, v);
-    }
-    __device__ __forceinline__ void pop() {
-        if (n > 1) {
-            const int len = n - 1;
-            const uint32_t vk = k_at(len);
-            const int32_t vv = id_at(len);
-            int hole = 0, second = 0;
-            while (second < (len - 1) / 2) {
-                second = 2 * (second + 1);
-                if (k_at(second) < k_at(second - 1)) second--;
-                set(hole, k_at(second), id_at(second));
-                hole = second;
-            }
-            if ((len & 1) == 0 && second == (len - 2) / 2) {
-                second = 2 * (second + 1);
-                set(hole, k_at(second - 1), id_at(second - 1));
-                hole = second - 1;
-            }
-            sift_up(hole, 0, vk, vv);
-        }
-        n--;
-    }
-};
 #endif
 
 }  // namespace fpp

@@ -80,9 +80,10 @@ void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const fl
                          const uint64_t* bucket_off, uint32_t* probes, uint64_t* ctot, uint32_t* qflag,
                          hipStream_t st);
 // Exact replay of the probing sequence for flagged queries: per (query,
-// table) lists in parallel, then the global heap merge per query.
+// table) lists in parallel, the global cut by radix select, and the global
+// heap merge only for queries whose cut has equal values on both sides.
 void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, int64_t ldq,
-                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, const uint32_t* qflag,
+                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, uint32_t* qflag,
                         const int32_t* qc, const float* qv, const uint64_t* bucket_off, uint32_t* probes,
                         uint64_t* ctot, void* work, int64_t work_per_query, hipStream_t st);
 int64_t probe_exact_work_bytes(const HashGeom& hg, const ProbeGeom& g);
@@ -101,6 +102,11 @@ struct GatherArgs {
 // Query order for cache reuse: sort key per query, then the XCD-aware map.
 void launch_order_keys(const int32_t* qc, int nq, int L, int PL, int two_np, uint32_t* key, hipStream_t st);
 void launch_xcd_order(const uint32_t* sorted, int nq, int nxcd, uint32_t* order, hipStream_t st);
+// The queries of order (identity when null) whose qflag is clear, in order,
+// into order_u.
+void launch_split_order(const uint32_t* order, int nq, const uint32_t* qflag, uint32_t* order_u, hipStream_t st);
+// Largest bucket of a built index (synchronises st); scratch: one word.
+uint32_t max_bucket_size(const uint64_t* off, int64_t nbk, uint32_t* scratch, hipStream_t st);
 // Dedup windows over the ids [0, N) and the LDS words of their counters
 // (count, segment offset, fill per window) when there is more than one.
 __host__ __device__ inline int64_t window_count(int64_t N, int64_t W) { return (N + W - 1) / W; }

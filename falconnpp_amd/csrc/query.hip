@@ -24,7 +24,7 @@ constexpr int PS_T = 256;
 constexpr int PS_WAVES = PS_T / WAVE;
 
 // LDS (dynamic): pool keys + buckets (L*perTable each), per-wave histograms,
-// block histogram, scan scratch.
+// block histogram.
 __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_t* __restrict__ qc,
                                                        const float* __restrict__ qv,
                                                        const uint64_t* __restrict__ bucket_off,
@@ -36,7 +36,6 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     uint32_t* pbkt = lds + pool_n;
     uint32_t* whist = pbkt + pool_n;           // PS_WAVES * 256
     uint32_t* bhist = whist + PS_WAVES * 256;  // 256
-    uint32_t* sh = bhist + 256;                // PS_T
     __shared__ int s_digit;
     __shared__ uint32_t s_below;
     __shared__ uint32_t s_tie;
@@ -289,80 +288,105 @@ __global__ __launch_bounds__(WAVE) void k_probe_exact_tables(HashGeom hg, ProbeG
     }
 }
 
-// Per flagged query: the global merge of src/FalconnPP.cpp:540-582.  The
-// per-table lists are staged in LDS by the workgroup; the first wave then
-// replays the max-heap merge with the heap and the table cursors in its
-// registers (one table per lane and slot).
+// Per flagged query, after the exact per-table lists: the reference's global
+// merge (src/FalconnPP.cpp:540-582) pops values in non-increasing order, so
+// its probe set is the top-qProbes of the pool by value, and libstdc++'s
+// heap order matters only when equal values straddle the cut.  A radix
+// select over the pool finds the cut; when it is unambiguous the set is
+// written here and the query marked resolved (qflag bit 2), else the merge is
+// replayed by k_probe_exact_merge.
 constexpr int MG_T = 256;
-template <int S>
+constexpr uint32_t RESOLVED = 4u;
+__global__ __launch_bounds__(MG_T) void k_probe_exact_select(ProbeGeom g, const uint32_t* __restrict__ qlist,
+                                                             const uint64_t* __restrict__ bucket_off,
+                                                             uint32_t* __restrict__ probes,
+                                                             uint64_t* __restrict__ ctot, uint32_t* __restrict__ qflag,
+                                                             const unsigned char* __restrict__ work, int64_t wpq) {
+    __shared__ uint32_t hist[256];
+    __shared__ int s_digit;
+    __shared__ uint32_t s_below, s_eqc, s_cnt;
+    __shared__ unsigned long long s_ct;
+    const uint32_t f = blockIdx.x;
+    const uint32_t q = qlist[f];
+    const int n = g.L * g.perTable;
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    const IFPair* pool = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
+    // descending order: smaller key = larger value
+    uint32_t prefix = 0, pmask = 0, k = (uint32_t)g.qProbes, eqc = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += MG_T) {
+            const uint32_t key = ~heap_key(pool[i].val);
+            if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (w == 0) {
+            int d;
+            uint32_t below, e;
+            wave_find_digit(hist, k, d, below, e);
+            if (lane == 0) {
+                s_digit = d;
+                s_below = below;
+                s_eqc = e;
+            }
+        }
+        __syncthreads();
+        prefix |= (uint32_t)s_digit << shift;
+        pmask |= 255u << shift;
+        k -= s_below;
+        eqc = s_eqc;
+    }
+    if (eqc != k) return;  // equal values straddle the cut: replay the merge
+    const uint32_t T = prefix;
+    if (threadIdx.x == 0) {
+        s_cnt = 0;
+        s_ct = 0;
+    }
+    __syncthreads();
+    uint32_t* out = probes + (size_t)q * g.qProbes;
+    unsigned long long ct = 0;
+    for (int c0 = 0; c0 < n; c0 += MG_T) {
+        const int i = c0 + threadIdx.x;
+        IFPair e{0, 0.0f};
+        bool take = false;
+        if (i < n) {
+            e = pool[i];
+            take = ~heap_key(e.val) <= T;
+        }
+        const uint64_t m = __ballot(take);
+        if (m == 0ull) continue;
+        const int leader = (int)__ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&s_cnt, (uint32_t)__popcll(m));
+        base = __shfl(base, leader, WAVE);
+        const uint32_t pos = base + lane_rank(m);
+        if (take && pos < (uint32_t)g.qProbes) {
+            const uint32_t b = (uint32_t)e.idx;
+            out[pos] = b;
+            ct += bucket_off[b + 1] - bucket_off[b];
+        }
+    }
+    atomicAdd(&s_ct, ct);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ctot[q] = s_ct;
+        qflag[q] |= RESOLVED;
+    }
+}
+
+// Per flagged query whose cut is ambiguous: the global merge replayed.  The
+// per-table lists are staged in LDS by the workgroup; one lane replays the
+// max-heap merge out of LDS (heap and cursors in LDS too).
 __global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const uint32_t* __restrict__ qlist,
                                                             const uint64_t* __restrict__ bucket_off,
                                                             uint32_t* __restrict__ probes, uint64_t* __restrict__ ctot,
+                                                            const uint32_t* __restrict__ qflag,
                                                             const unsigned char* __restrict__ work, int64_t wpq) {
     extern __shared__ IFPair mlds[];
     const uint32_t f = blockIdx.x;
     const uint32_t q = qlist[f];
-    const int L = g.L, PT = g.perTable;
-    const IFPair* list_g = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
-    IFPair* list = mlds;
-    for (int64_t i = threadIdx.x; i < (int64_t)L * PT; i += MG_T) list[i] = list_g[i];
-    __syncthreads();
-    uint32_t* out = probes + (size_t)q * g.qProbes;
-    if (threadIdx.x < WAVE) {
-        const int lane = threadIdx.x;
-        WaveMaxHeap<S> mh;
-        // table cursors: table l's in lane l % 64 of register l / 64
-        int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-        for (int l = 0; l < L; ++l) {
-            const IFPair e = list[(int64_t)l * PT];
-            mh.push(heap_key(e.val), e.idx);
-        }
-        for (int c = 0; c < g.qProbes; ++c) {
-            const int32_t top = mh.id_at(0);
-            mh.pop();
-            if (lane == 0) out[c] = (uint32_t)top;
-            const int tb = top / g.NB;
-            const int tl = tb & 63, ts = tb >> 6;
-            int32_t tk = __builtin_amdgcn_readlane(c0, tl);
-            if (S > 1 && ts == 1) tk = __builtin_amdgcn_readlane(c1, tl);
-            if (S > 2 && ts == 2) tk = __builtin_amdgcn_readlane(c2, tl);
-            if (S > 3 && ts == 3) tk = __builtin_amdgcn_readlane(c3, tl);
-            ++tk;
-            const bool mine = lane == tl;
-            if (mine && ts == 0) c0 = tk;
-            if (S > 1 && mine && ts == 1) c1 = tk;
-            if (S > 2 && mine && ts == 2) c2 = tk;
-            if (S > 3 && mine && ts == 3) c3 = tk;
-            if (tk < PT) {
-                const IFPair e = list[(int64_t)tb * PT + tk];
-                mh.push(heap_key(e.val), e.idx);
-            }
-        }
-    }
-    __syncthreads();
-    // C_tot of the probed set
-    __shared__ unsigned long long s_ct;
-    if (threadIdx.x == 0) s_ct = 0;
-    __syncthreads();
-    unsigned long long ct = 0;
-    for (int c = threadIdx.x; c < g.qProbes; c += MG_T) {
-        const uint32_t b = out[c];
-        ct += bucket_off[b + 1] - bucket_off[b];
-    }
-    atomicAdd(&s_ct, ct);
-    __syncthreads();
-    if (threadIdx.x == 0) ctot[q] = s_ct;
-}
-
-// The same merge for more than 256 tables: one lane, heap and cursors in LDS.
-__global__ __launch_bounds__(MG_T) void k_probe_exact_merge_lds(ProbeGeom g, const uint32_t* __restrict__ qlist,
-                                                                const uint64_t* __restrict__ bucket_off,
-                                                                uint32_t* __restrict__ probes,
-                                                                uint64_t* __restrict__ ctot,
-                                                                const unsigned char* __restrict__ work, int64_t wpq) {
-    extern __shared__ IFPair mlds[];
-    const uint32_t f = blockIdx.x;
-    const uint32_t q = qlist[f];
+    if (qflag[q] & RESOLVED) return;
     const int L = g.L, PT = g.perTable;
     const IFPair* list_g = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
     IFPair* list = mlds;
@@ -386,6 +410,7 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_merge_lds(ProbeGeom g, con
         }
     }
     __syncthreads();
+    // C_tot of the probed set
     __shared__ unsigned long long s_ct;
     if (threadIdx.x == 0) s_ct = 0;
     __syncthreads();
@@ -400,8 +425,7 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_merge_lds(ProbeGeom g, con
 }
 
 size_t merge_lds_bytes(const ProbeGeom& g) {
-    const size_t lists = (size_t)g.L * g.perTable * sizeof(IFPair);
-    return g.L <= 4 * WAVE ? lists : lists + (size_t)g.L * (sizeof(IFPair) + sizeof(int));
+    return (size_t)g.L * g.perTable * sizeof(IFPair) + (size_t)g.L * (sizeof(IFPair) + sizeof(int));
 }
 
 // ---- gather + dedup + FP32 rerank + top-k ---------------------------------
@@ -637,6 +661,40 @@ __global__ void k_xcd_order(const uint32_t* __restrict__ sorted, int nq, int nxc
     order[b] = sorted[start + j];
 }
 
+// Stable compaction of the unflagged queries of order (identity when null),
+// one workgroup.
+constexpr int SO_T = 1024;
+__global__ __launch_bounds__(SO_T) void k_split_order(const uint32_t* __restrict__ order, int nq,
+                                                      const uint32_t* __restrict__ qflag,
+                                                      uint32_t* __restrict__ order_u) {
+    __shared__ uint32_t wsum[SO_T / WAVE];
+    __shared__ uint32_t s_base;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    if (threadIdx.x == 0) s_base = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < nq; c0 += SO_T) {
+        const int i = c0 + threadIdx.x;
+        uint32_t q = 0;
+        bool keep = false;
+        if (i < nq) {
+            q = order ? order[i] : (uint32_t)i;
+            keep = qflag[q] == 0u;
+        }
+        const uint64_t m = __ballot(keep);
+        if (lane == 0) wsum[wv] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = s_base, tot = 0;
+        for (int v = 0; v < SO_T / WAVE; ++v) {
+            before += v < wv ? wsum[v] : 0u;
+            tot += wsum[v];
+        }
+        if (keep) order_u[before + lane_rank(m)] = q;
+        __syncthreads();
+        if (threadIdx.x == 0) s_base += tot;
+        __syncthreads();
+    }
+}
+
 size_t gather_lds_bytes(const GatherArgs& a, int TB) {
     const int64_t wwords = (a.window_bits / 32 + 1) & ~1;
     return (size_t)(wwords + window_counter_words(a.N, a.window_bits) + TB + 256 + 2048) * 4;
@@ -673,7 +731,7 @@ void dispatch_gather_cpl(const GatherArgs& a, int nq, const uint32_t* probes, co
 void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const float* qv,
                          const uint64_t* bucket_off, uint32_t* probes, uint64_t* ctot, uint32_t* qflag,
                          hipStream_t st) {
-    const size_t lds = ((size_t)2 * g.L * g.perTable + PS_WAVES * 256 + 256 + PS_T) * 4;
+    const size_t lds = ((size_t)2 * g.L * g.perTable + PS_WAVES * 256 + 256) * 4;
     FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "probe pool (n_tables * perTable) too large for one workgroup");
     if (nq == 0) return;
     k_probe_select<<<nq, PS_T, lds, st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
@@ -683,6 +741,12 @@ void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const fl
 void launch_order_keys(const int32_t* qc, int nq, int L, int PL, int two_np, uint32_t* key, hipStream_t st) {
     if (nq == 0) return;
     k_order_keys<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(qc, nq, L, PL, two_np, key);
+    FPP_HIP(hipGetLastError());
+}
+
+void launch_split_order(const uint32_t* order, int nq, const uint32_t* qflag, uint32_t* order_u, hipStream_t st) {
+    if (nq == 0) return;
+    k_split_order<<<1, SO_T, 0, st>>>(order, nq, qflag, order_u);
     FPP_HIP(hipGetLastError());
 }
 
@@ -697,7 +761,7 @@ int64_t probe_exact_work_bytes(const HashGeom& hg, const ProbeGeom& g) {
 }
 
 void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, int64_t ldq,
-                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, const uint32_t* qflag,
+                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, uint32_t* qflag,
                         const int32_t* qc, const float* qv, const uint64_t* bucket_off, uint32_t* probes,
                         uint64_t* ctot, void* work, int64_t work_per_query, hipStream_t st) {
     if (nlist == 0) return;
@@ -707,21 +771,12 @@ void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, 
     k_probe_exact_tables<<<(unsigned)((int64_t)nlist * g.L), WAVE, tlds, st>>>(hg, g, Q, ldq, masks, qlist, qflag,
                                                                             qc, qv, w, work_per_query);
     FPP_HIP(hipGetLastError());
+    k_probe_exact_select<<<nlist, MG_T, 0, st>>>(g, qlist, bucket_off, probes, ctot, qflag, w,
+                                                   work_per_query);
+    FPP_HIP(hipGetLastError());
     const size_t lds = merge_lds_bytes(g);
     FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "probe lists (n_tables * perTable) too large for the exact merge");
-#define FPP_MG(SV) \
-    k_probe_exact_merge<SV><<<nlist, MG_T, lds, st>>>(g, qlist, bucket_off, probes, ctot, w, work_per_query)
-    if (g.L <= WAVE)
-        FPP_MG(1);
-    else if (g.L <= 2 * WAVE)
-        FPP_MG(2);
-    else if (g.L <= 3 * WAVE)
-        FPP_MG(3);
-    else if (g.L <= 4 * WAVE)
-        FPP_MG(4);
-    else
-        k_probe_exact_merge_lds<<<nlist, MG_T, lds, st>>>(g, qlist, bucket_off, probes, ctot, w, work_per_query);
-#undef FPP_MG
+    k_probe_exact_merge<<<nlist, MG_T, lds, st>>>(g, qlist, bucket_off, probes, ctot, qflag, w, work_per_query);
     FPP_HIP(hipGetLastError());
 }
 
