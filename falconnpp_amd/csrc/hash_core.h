@@ -24,6 +24,8 @@
 // coordinates >= n_proj with per-lane bit words.
 #pragma once
 
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -175,7 +177,7 @@ __device__ __forceinline__ void out_of_range_bits(int base, int n_proj, uint32_t
 template <int F, int PM, bool FULL>
 __device__ __forceinline__ void hash_layer(const float* __restrict__ src, const uint32_t* __restrict__ masks, int part,
                                            int n_proj, const uint32_t* ob_in, float (&m)[PM], int (&c)[PM], int nchk,
-                                           bool& tie) {
+                                           int ncode, bool& tie) {
     constexpr int E = F < 128 ? F : 128;
     constexpr int LPH = F / E;
     constexpr int MW = (F + 31) / 32;
@@ -221,30 +223,29 @@ __device__ __forceinline__ void hash_layer(const float* __restrict__ src, const 
         }
     }
     merge_lists<PM, LPH>(m);
-    // code of the coordinate holding m[j]: index and signed value of the match
-    float sv[PM];
+    // coordinates past n_proj become NaN, which equals no value
+    if constexpr (!FULL) {
 #pragma unroll
-    for (int i = 0; i < PM; ++i) {
-        c[i] = -1;
-        sv[i] = 0.0f;
+        for (int e = 0; e < E; ++e)
+            x[e] = ((ob[e / 32] >> (e % 32)) & 1u) ? __int_as_float(0x7fc00000) : x[e];
     }
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        float a = fabsf(x[e]);
-        if constexpr (!FULL) a = ((ob[e / 32] >> (e % 32)) & 1u) ? -2.0f : a;
-#pragma unroll
-        for (int j = 0; j < PM - 1; ++j) {
-            const bool hit = a == m[j];
-            c[j] = hit ? e : c[j];
-            sv[j] = hit ? x[e] : sv[j];
-        }
-    }
+    // code of the coordinate holding m[j] (index and sign of the match), for
+    // the first ncode (wave-uniform) entries only
 #pragma unroll
     for (int j = 0; j < PM - 1; ++j) {
-        int v = c[j];
+        c[j] = 0;
+        if (j >= ncode) continue;
+        int cj = -1;
+        float sj = 0.0f;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const bool hit = fabsf(x[e]) == m[j];
+            cj = hit ? e : cj;
+            sj = hit ? x[e] : sj;
+        }
         // sgn(x) < 0 exactly when x < 0 (src/Utilities.h:26-31); -0.0 is positive
-        if (v >= 0) v = base + v + ((sv[j] < 0.0f) ? n_proj : 0);
-        c[j] = max(0, lanes_max<LPH>(v));
+        if (cj >= 0) cj = base + cj + ((sj < 0.0f) ? n_proj : 0);
+        c[j] = max(0, lanes_max<LPH>(cj));
     }
 #pragma unroll
     for (int j = 0; j + 1 < PM; ++j)
@@ -285,8 +286,8 @@ __global__ __launch_bounds__(HB_THREADS) void k_hash_build(HashGeom g, const flo
         float m1[PM], m2[PM];
         int c1[PM], c2[PM];
         bool tie = false;
-        hash_layer<F, PM, FULL>(src, masks + (size_t)(l * 2 + 0) * 3 * MW, part, g.n_proj, ob, m1, c1, nchk, tie);
-        hash_layer<F, PM, FULL>(src, masks + (size_t)(l * 2 + 1) * 3 * MW, part, g.n_proj, ob, m2, c2, nchk, tie);
+        hash_layer<F, PM, FULL>(src, masks + (size_t)(l * 2 + 0) * 3 * MW, part, g.n_proj, ob, m1, c1, nchk, p, tie);
+        hash_layer<F, PM, FULL>(src, masks + (size_t)(l * 2 + 1) * 3 * MW, part, g.n_proj, ob, m2, c2, nchk, p, tie);
         // top-(p+1) pair sums v2 + v1 (src/FalconnPP.cpp:159-189)
         float sm[PM];
 #pragma unroll
@@ -358,13 +359,16 @@ __global__ __launch_bounds__(HB_THREADS) void k_hash_query(HashGeom g, const flo
     uint32_t ob[(E + 31) / 32];
     out_of_range_bits<E>(part * E, g.n_proj, ob);
     bool tie = false;
-    for (int l = wave; l < g.L; l += HB_THREADS / WAVE) {
+    // workgroup (x, y): queries of block x, tables y*4 + wave, stepping by all
+    // workgroups of the column
+    constexpr int NWV = HB_THREADS / WAVE;
+    for (int l = blockIdx.y * NWV + wave; l < g.L; l += gridDim.y * NWV) {
 #pragma unroll
         for (int layer = 0; layer < 2; ++layer) {
             float m[PM];
             int c[PM];
             hash_layer<F, PM, FULL>(src, masks + (size_t)(l * 2 + layer) * 3 * MW, part, g.n_proj, ob, m, c, nchk,
-                                    tie);
+                                    PL, tie);
             if (valid && part == 0) {
                 const size_t o = (((size_t)q * g.L + l) * 2 + layer) * PL;
 #pragma unroll
@@ -400,7 +404,13 @@ void launch_query_variant(const HashGeom& g, const float* Q, int64_t ldq, int nq
                           int32_t* qc, float* qv, uint32_t* qflag, hipStream_t st) {
     constexpr int E = F < 128 ? F : 128;
     constexpr int HPW = WAVE / (F / E);
-    const dim3 grid((unsigned)ceil_div(nq, HPW));
+    constexpr int NWV = HB_THREADS / WAVE;
+    // split the tables over workgroups until about four waves per SIMD of
+    // the 256 CUs are in flight (every wave keeps at least one table)
+    const int64_t bx = ceil_div(nq, HPW);
+    const int64_t want = ceil_div(4 * 4 * 256, bx * NWV);
+    const int gy = (int)std::max<int64_t>(1, std::min<int64_t>(want, ceil_div(g.L, NWV)));
+    const dim3 grid((unsigned)bx, (unsigned)gy);
     k_hash_query<F, PM, FULL><<<grid, HB_THREADS, hash_lds_bytes<F>(), st>>>(g, Q, ldq, nq, PL, masks, qc, qv,
                                                                             qflag);
 }
@@ -441,6 +451,8 @@ void dispatch_query_f(const HashGeom& g, const float* Q, int64_t ldq, int nq, in
         FPP_V(4)
     } else if (PL + 1 <= 8) {
         FPP_V(8)
+    } else if (PL + 1 <= 12) {
+        FPP_V(12)
     } else if (PL + 1 <= 16) {
         FPP_V(16)
     } else {
