@@ -25,49 +25,91 @@ constexpr int PS_WAVES = PS_T / WAVE;
 
 // LDS (dynamic): pool keys + buckets (L*perTable each), per-wave histograms,
 // block histogram.
+template <bool FULLPOOL>
 __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_t* __restrict__ qc,
                                                        const float* __restrict__ qv,
                                                        const uint64_t* __restrict__ bucket_off,
                                                        uint32_t* __restrict__ probes, uint64_t* __restrict__ ctot,
                                                        uint32_t* __restrict__ qflag) {
     extern __shared__ uint32_t lds[];
-    const int pool_n = g.L * g.perTable;
+    const int PL = g.PL, np = PL * PL;
+    // FULLPOOL: every pair of every table enters the pool (per-table counts
+    // checked below); else the top-perTable pairs of each table
+    const int per = FULLPOOL ? np : g.perTable;
+    const int pool_n = g.L * per;
     uint32_t* pkey = lds;
     uint32_t* pbkt = lds + pool_n;
     uint32_t* whist = pbkt + pool_n;           // PS_WAVES * 256
     uint32_t* bhist = whist + PS_WAVES * 256;  // 256
+    uint32_t* tcnt = bhist + 256;              // L (FULLPOOL)
     __shared__ int s_digit;
     __shared__ uint32_t s_below;
     __shared__ uint32_t s_tie;
     __shared__ unsigned long long s_ctot;
     const int q = blockIdx.x;
     const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-    const int PL = g.PL, np = PL * PL;
     const uint64_t bmask = lanes_below_mask();
     if (threadIdx.x == 0) {
         s_tie = 0;
         s_ctot = 0;
     }
+    if constexpr (FULLPOOL)
+        for (int l = threadIdx.x; l < g.L; l += PS_T) tcnt[l] = 0;
     __syncthreads();
     uint32_t* hw = whist + w * 256;
     bool tie = false;
+    // a table's two top lists sit in lanes: v1/c1 in lanes [0, PL), v2/c2 in
+    // [PL, 2 PL); the wave's next table is fetched while this one is selected
+    auto fetch = [&](int l, float& v, int32_t& c) {
+        v = 0.0f;
+        c = 0;
+        if (l < g.L && lane < 2 * PL) {
+            const size_t base = ((size_t)q * g.L + l) * 2 * PL;
+            v = qv[base + lane];
+            c = qc[base + lane];
+        }
+    };
+    constexpr int KR = 16;  // pairs per lane: PL <= 31, so PL * PL <= 16 * 64
+    const int nr = (np + WAVE - 1) / WAVE;
+    float vn;
+    int32_t cn;
+    fetch(w, vn, cn);
     for (int l = w; l < g.L; l += PS_WAVES) {
-        const size_t base = ((size_t)q * g.L + l) * 2 * PL;
-        const float* v1 = qv + base;
-        const float* v2 = qv + base + PL;
-        const int32_t* c1 = qc + base;
-        const int32_t* c2 = qc + base + PL;
-        // perTable-th largest pair sum (src/FalconnPP.cpp:457-488: fAbsHashValue1 + fAbsHashValue2)
+        const float vv = vn;
+        const int32_t cc = cn;
+        fetch(l + PS_WAVES, vn, cn);
+        // key of pair t = r * 64 + lane: fAbsHashValue1 + fAbsHashValue2
+        // (src/FalconnPP.cpp:457-488), in the same order of addition
+        uint32_t key[KR];
+#pragma unroll
+        for (int r = 0; r < KR; ++r) {
+            key[r] = 0xffffffffu;
+            if (r < nr) {
+                const int t = r * WAVE + lane;
+                const int i = t < np ? t / PL : 0, j = t < np ? t - i * PL : 0;
+                const float a = __shfl(vv, i, WAVE), b = __shfl(vv, PL + j, WAVE);
+                if (t < np) key[r] = desc_key(a + b);
+                if constexpr (FULLPOOL) {
+                    const int32_t ci = __shfl(cc, i, WAVE), cj = __shfl(cc, PL + j, WAVE);
+                    if (t < np) {
+                        pkey[l * np + t] = key[r];
+                        pbkt[l * np + t] =
+                            (uint32_t)l * (uint32_t)g.NB + (uint32_t)ci * (uint32_t)g.two_np + (uint32_t)cj;
+                    }
+                }
+            }
+        }
+        if constexpr (FULLPOOL) continue;
+        // perTable-th largest pair sum
         uint32_t prefix = 0, pmask = 0, k = (uint32_t)g.perTable, eqc = 0;
         for (int shift = 24; shift >= 0; shift -= 8) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) hw[lane * 4 + i] = 0;
             __builtin_amdgcn_wave_barrier();
-            for (int t = lane; t < np; t += WAVE) {
-                const int i = t / PL, j = t - i * PL;
-                const uint32_t key = desc_key(v1[i] + v2[j]);
-                if ((key & pmask) == prefix) atomicAdd(&hw[(key >> shift) & 255u], 1u);
-            }
+#pragma unroll
+            for (int r = 0; r < KR; ++r)
+                if (r < nr && r * WAVE + lane < np && (key[r] & pmask) == prefix)
+                    atomicAdd(&hw[(key[r] >> shift) & 255u], 1u);
             __builtin_amdgcn_wave_barrier();
             int d;
             uint32_t below;
@@ -80,13 +122,15 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
         if (np > g.perTable && eqc != k) tie = true;
         const uint32_t T = prefix;
         uint32_t written = 0, eq_seen = 0;
-        for (int t0 = 0; t0 < np; t0 += WAVE) {
-            const int t = t0 + lane;
+#pragma unroll
+        for (int r = 0; r < KR; ++r) {
+            if (r >= nr) continue;
+            const int t = r * WAVE + lane;
             const bool valid = t < np;
             const int i = valid ? t / PL : 0, j = valid ? t - i * PL : 0;
-            const uint32_t key = valid ? desc_key(v1[i] + v2[j]) : 0xffffffffu;
-            const bool lt = valid && key < T;
-            const bool eqk = valid && key == T;
+            const int32_t ci = __shfl(cc, i, WAVE), cj = __shfl(cc, PL + j, WAVE);
+            const bool lt = valid && key[r] < T;
+            const bool eqk = valid && key[r] == T;
             const uint64_t meq = __ballot(eqk);
             const bool take_eq = eqk && (eq_seen + (uint32_t)__popcll(meq & bmask)) < k;
             const bool take = lt || take_eq;
@@ -94,8 +138,8 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
             const uint32_t slot = written + (uint32_t)__popcll(mtake & bmask);
             if (take && slot < (uint32_t)g.perTable) {
                 const int idx = l * g.perTable + (int)slot;
-                pkey[idx] = key;
-                pbkt[idx] = (uint32_t)l * (uint32_t)g.NB + (uint32_t)c1[i] * (uint32_t)g.two_np + (uint32_t)c2[j];
+                pkey[idx] = key[r];
+                pbkt[idx] = (uint32_t)l * (uint32_t)g.NB + (uint32_t)ci * (uint32_t)g.two_np + (uint32_t)cj;
             }
             written += (uint32_t)__popcll(mtake);
             eq_seen += (uint32_t)__popcll(meq);
@@ -154,6 +198,17 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
             out[pos] = b;
             my_ctot += bucket_off[b + 1] - bucket_off[b];
         }
+        if constexpr (FULLPOOL)
+            if (take) atomicAdd(&tcnt[i / np], 1u);
+    }
+    if constexpr (FULLPOOL) {
+        // the reference keeps the top-perTable pairs of each table; that cut
+        // is irrelevant when no table holds more than perTable pairs at or
+        // above the global cut, else the query is replayed exactly
+        __syncthreads();
+        bool over = false;
+        for (int l = threadIdx.x; l < g.L; l += PS_T) over |= tcnt[l] > (uint32_t)g.perTable;
+        if (over) atomicOr(&s_tie, 1u);
     }
     atomicAdd(&s_ctot, my_ctot);
     __syncthreads();
@@ -731,10 +786,15 @@ void dispatch_gather_cpl(const GatherArgs& a, int nq, const uint32_t* probes, co
 void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const float* qv,
                          const uint64_t* bucket_off, uint32_t* probes, uint64_t* ctot, uint32_t* qflag,
                          hipStream_t st) {
-    const size_t lds = ((size_t)2 * g.L * g.perTable + PS_WAVES * 256 + 256) * 4;
+    const size_t fixed = (size_t)(PS_WAVES * 256 + 256 + g.L) * 4;
+    const size_t lds_full = (size_t)2 * g.L * g.PL * g.PL * 4 + fixed;
+    const size_t lds = (size_t)2 * g.L * g.perTable * 4 + fixed;
     FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "probe pool (n_tables * perTable) too large for one workgroup");
     if (nq == 0) return;
-    k_probe_select<<<nq, PS_T, lds, st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
+    if (lds_full <= 80 * 1024)
+        k_probe_select<true><<<nq, PS_T, lds_full, st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
+    else
+        k_probe_select<false><<<nq, PS_T, lds, st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
     FPP_HIP(hipGetLastError());
 }
 
