@@ -36,6 +36,22 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def profiled_traffic(args, cfg, default_cfg):
+    """HBM bytes per launch of the FP32 gather kernel (FETCH_SIZE + WRITE_SIZE, each
+    scaled by a plain copy's reading) as recorded under profiles/ for this
+    workload; None for a workload that was not profiled."""
+    if args.scale != 1.0 or cfg["qProbes"] != default_cfg["qProbes"]:
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"hbm_traffic_gather_{args.config}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch k_gather_score_f32"):
+            return int(line.rsplit(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,7 +139,8 @@ def main():
     alg_bytes = (8 * cfg["qProbes"] * nq + 4 * st["ctot"] + 4 * D * st["cuniq"] + nq * (4 * D + 4 * k))
     achieved = alg_bytes / (gather_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic(args, cfg, synth.CONFIGS[args.config]),
                 "kernel": "k_gather_score_f32", "kernel_ms": round(gather_ms, 3),
                 "alg_bytes_per_launch": int(alg_bytes)}
     stages = {key: round(float(np.mean([s[key] for s in stage])), 3) for key in ("hash", "probe", "replay", "gather")}
