@@ -94,6 +94,30 @@ struct Index {
     std::vector<uint32_t> flagged;
     uint32_t max_bucket = 0;  // largest bucket of the built index
     DevBuf<uint32_t> scratch_word;
+    // per-query probe counts and k (batched serving)
+    std::vector<QueryParams> hqp;
+    DevBuf<QueryParams> qparams;
+    // persistent gather: work-queue counters (main, second stream) and the
+    // longest-first order of the queries
+    bool persistent = false;
+    DevBuf<uint32_t> queue;
+    std::vector<uint32_t> h_order;
+    // hashing + probe selection of a batch captured in a hipGraph, replayed
+    // while the batch shape and buffers stay the same
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;
+    uint64_t build_gen = 0;
+    uint64_t last_nprobes = 0;
+    std::vector<uint64_t> last_ts;  // per-query phase stamps of the last stamped call
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_key.clear();
+    }
     DevBuf<uint64_t> ohist_off, oscan;
     DevBuf<uint64_t> phase_ts;
 
@@ -130,6 +154,7 @@ struct Index {
     ~Index() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        drop_graph();
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (side) (void)hipStreamDestroy(side);
@@ -364,6 +389,8 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
     I.scratch_word.reserve(1);
     I.max_bucket = max_bucket_size(I.bucket_off.get(), (int64_t)L * NB, I.scratch_word.get(), st);
     I.stat_bucket_ties = bucket_ties;
+    I.build_gen++;
+    I.drop_graph();
     I.built = true;
 }
 
@@ -392,48 +419,119 @@ struct QueryPlan {
 };
 
 // src/FalconnPP.cpp:336-337 in double arithmetic
-QueryPlan plan_query(const Index& I) {
+QueryPlan plan_for(const Index& I, int qProbes) {
     QueryPlan pl;
-    pl.perTable = (int)std::ceil(2.0 * I.qProbes / I.n_tables);
+    pl.perTable = (int)std::ceil(2.0 * qProbes / I.n_tables);
     pl.perLayer = (int)std::ceil(std::sqrt(1.0 * pl.perTable));
     return pl;
 }
 
-void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_out, float* d_scores,
-              hipStream_t st) {
-    FPP_REQUIRE(I.built, FPP_ESTATE, "query called before build (or after clear)");
-    FPP_REQUIRE(nq >= 0, FPP_EINVAL, "number of queries must be >= 0");
-    FPP_REQUIRE(k >= 1 && k <= 1024, FPP_EINVAL, "n_neighbors must be in [1, 1024]");
-    FPP_REQUIRE(I.qProbes >= 1, FPP_EINVAL, "qProbes must be >= 1");
-    FPP_REQUIRE(I.precision == 0 || k <= I.refine_count(k), FPP_EINVAL,
-                "n_neighbors exceeds the refine count of the reduced-precision path");
-    FPP_REQUIRE(ldq >= I.n_features, FPP_EINVAL, "leading dimension of Q must be >= n_features");
-    const QueryPlan pl = plan_query(I);
+void check_plan(const Index& I, const QueryPlan& pl) {
     FPP_REQUIRE(pl.perLayer <= I.n_proj, FPP_EINVAL,
                 "qProbes too large: ceil(sqrt(ceil(2*qProbes/n_tables))) exceeds n_proj (the reference pops an empty heap)");
     FPP_REQUIRE(pl.perLayer <= 31, FPP_EINVAL, "qProbes too large: more than 31 probes per layer");
+}
+
+// One batch of queries.  k is the number of neighbours, or with per-query
+// counts (h_qprobes, h_k: host arrays of nq) the largest k and the row stride
+// of the outputs.
+void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_out, float* d_scores,
+              hipStream_t st, const int32_t* h_qprobes = nullptr, const int32_t* h_k = nullptr) {
+    FPP_REQUIRE(I.built, FPP_ESTATE, "query called before build (or after clear)");
+    FPP_REQUIRE(nq >= 0, FPP_EINVAL, "number of queries must be >= 0");
+    FPP_REQUIRE(k >= 1 && k <= 1024, FPP_EINVAL, "n_neighbors must be in [1, 1024]");
+    FPP_REQUIRE(I.precision == 0 || k <= I.refine_count(k), FPP_EINVAL,
+                "n_neighbors exceeds the refine count of the reduced-precision path");
+    FPP_REQUIRE(ldq >= I.n_features, FPP_EINVAL, "leading dimension of Q must be >= n_features");
+    FPP_REQUIRE((h_qprobes == nullptr) == (h_k == nullptr), FPP_EINVAL, "per-query qProbes and k go together");
+    const bool batched = h_qprobes != nullptr;
+    QueryPlan pl;
+    int qp_max = I.qProbes;
+    uint64_t nprobes = 0;
+    if (batched) {
+        // per query: its qProbes, perTable, PL and k; sizes from the largest
+        I.hqp.resize(nq);
+        pl = QueryPlan{0, 0};
+        qp_max = 0;
+        for (int q = 0; q < nq; ++q) {
+            FPP_REQUIRE(h_qprobes[q] >= 1, FPP_EINVAL, "qProbes must be >= 1");
+            FPP_REQUIRE(h_k[q] >= 1 && h_k[q] <= k, FPP_EINVAL, "per-query n_neighbors must be in [1, kmax]");
+            const QueryPlan p = plan_for(I, h_qprobes[q]);
+            check_plan(I, p);
+            I.hqp[q] = QueryParams{h_qprobes[q], p.perTable, p.perLayer, h_k[q], nprobes};
+            nprobes += (uint64_t)h_qprobes[q];
+            pl.perTable = std::max(pl.perTable, p.perTable);
+            pl.perLayer = std::max(pl.perLayer, p.perLayer);
+            qp_max = std::max(qp_max, h_qprobes[q]);
+        }
+    } else {
+        FPP_REQUIRE(I.qProbes >= 1, FPP_EINVAL, "qProbes must be >= 1");
+        pl = plan_for(I, I.qProbes);
+        check_plan(I, pl);
+        nprobes = (uint64_t)nq * I.qProbes;
+    }
     I.stat_ctot = I.stat_cuniq = I.stat_qties = 0;
     I.last_nq = nq;
     if (nq == 0) return;
     const HashGeom hg = I.hgeom();
-    ProbeGeom pg{I.n_tables, I.NB(), pl.perLayer, pl.perTable, I.qProbes, 2 * I.n_proj};
+    ProbeGeom pg{I.n_tables, I.NB(), pl.perLayer, pl.perTable, qp_max, 2 * I.n_proj};
+    if (batched) {
+        I.qparams.reserve(nq);
+        FPP_HIP(hipMemcpyAsync(I.qparams.get(), I.hqp.data(), (size_t)nq * sizeof(QueryParams),
+                               hipMemcpyHostToDevice, st));
+        pg.qp = I.qparams.get();
+    }
     const int L = I.n_tables, PL = pl.perLayer;
-    // queries padded to the scored row stride
     I.Qpad.reserve((size_t)nq * I.ldx);
-    FPP_HIP(hipMemsetAsync(I.Qpad.get(), 0, (size_t)nq * I.ldx * 4, st));
-    FPP_HIP(hipMemcpy2DAsync(I.Qpad.get(), I.ldx * 4, dQ, ldq * 4, (size_t)I.n_features * 4, nq,
-                             hipMemcpyDeviceToDevice, st));
     I.qc.reserve((size_t)nq * L * 2 * PL);
     I.qv.reserve((size_t)nq * L * 2 * PL);
     I.qflag.reserve(nq);
-    I.probes.reserve((size_t)nq * I.qProbes);
+    I.probes.reserve(nprobes);
     I.ctot.reserve(nq);
-    FPP_HIP(hipMemsetAsync(I.qflag.get(), 0, (size_t)nq * 4, st));
-    I.mark(0, st);
-    launch_hash_query(hg, I.Qpad.get(), I.ldx, nq, PL, I.masks.get(), I.qc.get(), I.qv.get(), I.qflag.get(), st);
-    I.mark(1, st);
-    launch_probe_select(pg, nq, I.qc.get(), I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(),
-                        I.qflag.get(), st);
+    I.last_nprobes = nprobes;
+    // hashing and probe selection: queries padded to the scored row stride,
+    // flags cleared by fill kernels (no memset nodes in a captured graph)
+    auto phase1 = [&](bool marks) {
+        launch_fill_u32(reinterpret_cast<uint32_t*>(I.Qpad.get()), 0u, (int)((size_t)nq * I.ldx), st);
+        FPP_HIP(hipMemcpy2DAsync(I.Qpad.get(), I.ldx * 4, dQ, ldq * 4, (size_t)I.n_features * 4, nq,
+                                 hipMemcpyDeviceToDevice, st));
+        launch_fill_u32(I.qflag.get(), 0u, nq, st);
+        if (marks) I.mark(0, st);
+        launch_hash_query(hg, I.Qpad.get(), I.ldx, nq, PL, I.masks.get(), I.qc.get(), I.qv.get(), I.qflag.get(),
+                          st);
+        if (marks) I.mark(1, st);
+        launch_probe_select(pg, nq, I.qc.get(), I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(),
+                            I.qflag.get(), st);
+    };
+    if (I.use_graph) {
+        const std::vector<uint64_t> key = {
+            (uint64_t)nq, (uint64_t)ldq, (uint64_t)(uintptr_t)dQ, (uint64_t)(uintptr_t)st, I.build_gen,
+            (uint64_t)(uintptr_t)I.Qpad.get(), (uint64_t)(uintptr_t)I.qc.get(), (uint64_t)(uintptr_t)I.qv.get(),
+            (uint64_t)(uintptr_t)I.qflag.get(), (uint64_t)(uintptr_t)I.probes.get(),
+            (uint64_t)(uintptr_t)I.ctot.get(), (uint64_t)(uintptr_t)pg.qp, (uint64_t)pg.PL,
+            (uint64_t)pg.perTable, (uint64_t)pg.qProbes};
+        if (!I.graph_exec || key != I.graph_key) {
+            I.drop_graph();
+            FPP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                phase1(false);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            FPP_HIP(hipStreamEndCapture(st, &I.graph));
+            FPP_HIP(hipGraphInstantiate(&I.graph_exec, I.graph, nullptr, nullptr, 0));
+            I.graph_key = key;
+        }
+        // hashing is timed together with probe selection here
+        I.mark(0, st);
+        I.mark(1, st);
+        FPP_HIP(hipGraphLaunch(I.graph_exec, st));
+    } else {
+        phase1(true);
+    }
     I.mark(2, st);
     // flags and candidate totals of the fast path
     std::vector<uint32_t> flag(nq);
@@ -450,11 +548,13 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     I.stat_qties = (int64_t)nl;
     // candidate segments: C_tot for the others; for a flagged query (its
     // probes are replayed later) the qProbes largest buckets bound C_tot
-    const uint64_t flag_cap = std::min<uint64_t>((uint64_t)I.qProbes * I.max_bucket, (uint64_t)I.n_points);
     std::vector<uint64_t> off(nq + 1);
     off[0] = 0;
-    for (int q = 0; q < nq; ++q)
+    for (int q = 0; q < nq; ++q) {
+        const uint64_t qpq = batched ? (uint64_t)I.hqp[q].qProbes : (uint64_t)I.qProbes;
+        const uint64_t flag_cap = std::min<uint64_t>(qpq * I.max_bucket, (uint64_t)I.n_points);
         off[q + 1] = off[q] + (flag[q] ? flag_cap : std::min<uint64_t>(ct[q], (uint64_t)I.n_points));
+    }
     I.cand_off.reserve(nq + 1);
     I.cand.reserve(off[nq]);
     I.score.reserve(off[nq]);
@@ -467,11 +567,15 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     ga.qProbes = I.qProbes;
     ga.k = k;
     ga.part = reinterpret_cast<int32_t*>(I.score.get());
+    if (batched) {
+        ga.qp = I.qparams.get();
+        ga.ldo = k;
+    }
     const bool lowp = I.precision != 0;
     ga.window_bits = gather_window_bits(I.n_points, lowp ? lowprec_extra_words(I.refine_count(k))
                                                          : (int64_t)I.gather_threads + 256 + 2048);
     const uint32_t* order = nullptr;
-    if (I.order_queries && nq > 1) {
+    if (I.order_queries && nq > 1 && !I.persistent) {
         const int bits = ilog2_exact(I.NB());
         const int64_t rh = radix_hist_elems(nq);
         I.okey.reserve(nq);
@@ -489,7 +593,18 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         launch_xcd_order(sv, nq, 8, I.order.get(), st);
         order = I.order.get();
     }
-    if (nl > 0) {
+    if (I.persistent) {
+        // work queue, longest first: the unflagged queries by C_tot
+        I.h_order.clear();
+        for (int q = 0; q < nq; ++q)
+            if (!flag[q]) I.h_order.push_back((uint32_t)q);
+        std::stable_sort(I.h_order.begin(), I.h_order.end(), [&](uint32_t a, uint32_t b) { return ct[a] > ct[b]; });
+        I.order_u.reserve(nq);
+        if (nu > 0)
+            FPP_HIP(hipMemcpyAsync(I.order_u.get(), I.h_order.data(), (size_t)nu * 4, hipMemcpyHostToDevice, st));
+        order = I.order_u.get();
+        I.queue.reserve(2);
+    } else if (nl > 0) {
         I.order_u.reserve(nq);
         launch_split_order(order, nq, I.qflag.get(), I.order_u.get(), st);
         order = I.order_u.get();
@@ -503,6 +618,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     if (lowp) {
         lp.format = I.precision;
         lp.refine = I.refine_count(k);
+        lp.refine_per_k = batched && I.refine == 0;
         lp.ldl = I.ldl;
         lp.xb = I.Xb.get();
         lp.x8 = I.X8.get();
@@ -536,6 +652,10 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         I.mark(4, I.side);
         GatherArgs gf = ga;
         gf.order = I.qlist.get();
+        if (I.persistent) {
+            gf.queue = I.queue.get() + 1;
+            gf.nitems = nl;
+        }
         gather(gf, (int)nl, I.side);
         FPP_HIP(hipEventRecord(I.ev_join, I.side));
     } else {
@@ -544,6 +664,10 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     }
     I.mark(5, st);
     ga.order = order;
+    if (I.persistent) {
+        ga.queue = I.queue.get();
+        ga.nitems = (uint32_t)nu;
+    }
     gather(ga, nu, st);
     I.mark(6, st);
     if (nl > 0) FPP_HIP(hipStreamWaitEvent(st, I.ev_join, 0));
@@ -569,7 +693,8 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
             FPP_HIP(hipGetDevice(&dev));
             FPP_HIP(hipDeviceGetAttribute(&I.wall_khz, hipDeviceAttributeWallClockRate, dev));
         }
-        std::vector<uint64_t> ts((size_t)nq * 5);
+        std::vector<uint64_t>& ts = I.last_ts;
+        ts.resize((size_t)nq * 5);
         FPP_HIP(hipMemcpy(ts.data(), I.phase_ts.get(), ts.size() * 8, hipMemcpyDeviceToHost));
         double acc[4] = {0, 0, 0, 0};
         for (int q = 0; q < nq; ++q)
@@ -691,6 +816,75 @@ int fpp_query_device(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, 
     return guard([&] {
         FPP_REQUIRE(h && (nq == 0 || (dQ && d_out)), FPP_EINVAL, "null argument");
         do_query(h->I, dQ, ldq, nq, k, d_out, d_scores, pick_stream(h->I, stream));
+    });
+}
+
+int fpp_query_batch_device(fpp_index* h, const float* dQ, int64_t ldq, int nq, const int32_t* qprobes,
+                           const int32_t* ks, int kmax, int32_t* d_out, float* d_scores, void* stream) {
+    return guard([&] {
+        FPP_REQUIRE(h && (nq == 0 || (dQ && d_out && qprobes && ks)), FPP_EINVAL, "null argument");
+        do_query(h->I, dQ, ldq, nq, kmax, d_out, d_scores, pick_stream(h->I, stream), qprobes, ks);
+    });
+}
+
+int fpp_autotune(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, void* stream, int32_t* chosen) {
+    return guard([&] {
+        FPP_REQUIRE(h && dQ && nq >= 1, FPP_EINVAL, "null argument or empty sample");
+        Index& I = h->I;
+        hipStream_t st = pick_stream(I, stream);
+        DevBuf<int32_t> out;
+        out.reserve((size_t)nq * k);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        FPP_HIP(hipEventCreate(&e0));
+        FPP_HIP(hipEventCreate(&e1));
+        const bool profiling = I.profiling, stamps = I.phase_stamps;
+        I.profiling = I.phase_stamps = false;
+        // workgroup size applies to the FP32 gather; the reduced-precision
+        // gather has one shape
+        const std::vector<int> threads = I.precision == 0 ? std::vector<int>{256, 512, 1024} : std::vector<int>{512};
+        float best = 0.0f;
+        int bt = I.gather_threads, bo = I.order_queries ? 1 : 0, bp = I.persistent ? 1 : 0;
+        try {
+            for (int t : threads)
+                for (int o = 0; o < 2; ++o)
+                    for (int pers = 0; pers < 2; ++pers) {
+                        I.gather_threads = t;
+                        I.order_queries = o != 0;
+                        I.persistent = pers != 0;
+                        do_query(I, dQ, ldq, nq, k, out.get(), nullptr, st);
+                        FPP_HIP(hipEventRecord(e0, st));
+                        for (int r = 0; r < 2; ++r) do_query(I, dQ, ldq, nq, k, out.get(), nullptr, st);
+                        FPP_HIP(hipEventRecord(e1, st));
+                        FPP_HIP(hipEventSynchronize(e1));
+                        float ms = 0.0f;
+                        FPP_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        if (best == 0.0f || ms < best) {
+                            best = ms;
+                            bt = t;
+                            bo = o;
+                            bp = pers;
+                        }
+                    }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            I.profiling = profiling;
+            I.phase_stamps = stamps;
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        I.profiling = profiling;
+        I.phase_stamps = stamps;
+        I.gather_threads = bt;
+        I.order_queries = bo != 0;
+        I.persistent = bp != 0;
+        if (chosen) {
+            chosen[0] = bt;
+            chosen[1] = bo;
+            chosen[2] = bp;
+            chosen[3] = (int32_t)(best / 2.0f * 1000.0f);  // microseconds per batch
+        }
     });
 }
 
@@ -858,8 +1052,32 @@ int fpp_set_tuning(fpp_index* h, int which, int value) {
                             "gather threads must be 256, 512 or 1024");
                 h->I.gather_threads = value;
                 break;
+            case FPP_TUNE_PERSISTENT:
+                h->I.persistent = value != 0;
+                break;
+            case FPP_TUNE_GRAPH:
+                h->I.use_graph = value != 0;
+                if (!h->I.use_graph) h->I.drop_graph();
+                break;
             default:
                 throw Error(FPP_EINVAL, "unknown tuning knob");
+        }
+    });
+}
+
+int fpp_get_query_times(fpp_index* h, double* service_us, double* done_us) {
+    return guard([&] {
+        FPP_REQUIRE(h && service_us && done_us, FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        const size_t nq = I.last_ts.size() / 5;
+        FPP_REQUIRE(nq > 0 && nq == (size_t)I.last_nq, FPP_ESTATE,
+                    "no per-query stamps: make the last query call with profiling level 2");
+        uint64_t t0 = ~0ull;
+        for (size_t q = 0; q < nq; ++q) t0 = std::min(t0, I.last_ts[q * 5]);
+        const double us = 1000.0 / std::max(1, I.wall_khz);
+        for (size_t q = 0; q < nq; ++q) {
+            service_us[q] = (double)(I.last_ts[q * 5 + 4] - I.last_ts[q * 5]) * us;
+            done_us[q] = (double)(I.last_ts[q * 5 + 4] - t0) * us;
         }
     });
 }
@@ -869,7 +1087,7 @@ int fpp_get_last_probes(fpp_index* h, uint32_t* out) {
         FPP_REQUIRE(h && out, FPP_EINVAL, "null argument");
         Index& I = h->I;
         if (I.last_nq)
-            FPP_HIP(hipMemcpy(out, I.probes.get(), (size_t)I.last_nq * I.qProbes * 4, hipMemcpyDeviceToHost));
+            FPP_HIP(hipMemcpy(out, I.probes.get(), (size_t)I.last_nprobes * 4, hipMemcpyDeviceToHost));
     });
 }
 

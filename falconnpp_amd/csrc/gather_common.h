@@ -259,12 +259,11 @@ __device__ __forceinline__ bool bitset_insert(uint32_t* bits, int64_t rel) {
 // Callers rank by (score, id), which does not depend on the order of cq.
 // wc holds window_counter_words(N, W) words, sh at least TB words.
 template <int TB>
-__device__ uint32_t collect_candidates(const GatherArgs& a, const uint32_t* __restrict__ pr,
+__device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32_t* __restrict__ pr,
                                        const uint64_t* __restrict__ bucket_off, const int32_t* __restrict__ ids,
                                        uint32_t* bits, uint32_t* wc, uint32_t* sh, int32_t* __restrict__ cq,
                                        int32_t* __restrict__ part, uint64_t ccap, uint64_t* ts) {
     const int64_t W = a.window_bits;
-    const int np = a.qProbes;
     if (a.N <= W) {
         const int words = (int)(W / 32);
         for (int i = threadIdx.x; i < words; i += TB) bits[i] = 0u;

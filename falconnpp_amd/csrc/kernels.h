@@ -70,10 +70,40 @@ void launch_seg_exact(const uint32_t* tie_list, uint32_t ntie, const uint64_t* w
                       const uint64_t* bucket_off, int32_t* ids, void* work, hipStream_t st);
 
 // ---- query (query.hip) ----------------------------------------------------
-struct ProbeGeom {
-    int L, NB, PL, perTable, qProbes;
-    int two_np;  // 2 * n_proj: codes per layer
+// Per-query parameters of a batch that mixes probe counts and k (perTable
+// and PL follow from qProbes by the reference's formulas).
+struct QueryParams {
+    int32_t qProbes, perTable, PL, k;
+    uint64_t probe_off;  // first probe of the query in the probe array
 };
+
+struct ProbeGeom {
+    int L, NB, PL, perTable, qProbes;  // PL: stride of the per-layer top lists
+    int two_np;  // 2 * n_proj: codes per layer
+    const QueryParams* qp = nullptr;  // per query (null: the uniform values above)
+};
+
+// Parameters of query q: its own when the batch has per-query parameters.
+struct QueryView {
+    int qProbes, perTable, PL, k;
+    uint64_t probe_off;
+};
+__device__ __forceinline__ QueryView query_view(const QueryParams* qp, int q, int qProbes, int perTable, int PL,
+                                                int k) {
+    // q is the same for the whole workgroup: keep the fields scalar
+    QueryView v{qProbes, perTable, PL, k, (uint64_t)q * (uint64_t)qProbes};
+    if (qp) {
+        const QueryParams p = qp[q];
+        v = QueryView{p.qProbes, p.perTable, p.PL, p.k, p.probe_off};
+    }
+    v.qProbes = __builtin_amdgcn_readfirstlane(v.qProbes);
+    v.perTable = __builtin_amdgcn_readfirstlane(v.perTable);
+    v.PL = __builtin_amdgcn_readfirstlane(v.PL);
+    v.k = __builtin_amdgcn_readfirstlane(v.k);
+    v.probe_off = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v.probe_off >> 32)) << 32) |
+                  (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v.probe_off);
+    return v;
+}
 // Global top-qProbes bucket set per query; ctot[q] = sum of probed sizes;
 // qflag[q] |= 2 when a tie straddles a per-table or the global boundary.
 void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const float* qv,
@@ -98,6 +128,12 @@ struct GatherArgs {
     const uint32_t* order = nullptr;  // block -> query (null: identity)
     uint64_t* phase_ts = nullptr;     // per query 5 wall-clock stamps (null: off)
     int32_t* part = nullptr;          // dedup-window scratch, per-query segments at cand_off
+    const QueryParams* qp = nullptr;  // per-query probe counts and k (null: qProbes, k above)
+    int ldo = 0;                      // row stride of the outputs (0: k)
+    // persistent launch: workgroups take order[item] for item = atomicAdd(queue)
+    // until nitems (queue zeroed beforehand); null: one workgroup per query
+    uint32_t* queue = nullptr;
+    uint32_t nitems = 0;
 };
 // Query order for cache reuse: sort key per query, then the XCD-aware map.
 void launch_order_keys(const int32_t* qc, int nq, int L, int PL, int two_np, uint32_t* key, hipStream_t st);
@@ -105,6 +141,11 @@ void launch_xcd_order(const uint32_t* sorted, int nq, int nxcd, uint32_t* order,
 // The queries of order (identity when null) whose qflag is clear, in order,
 // into order_u.
 void launch_split_order(const uint32_t* order, int nq, const uint32_t* qflag, uint32_t* order_u, hipStream_t st);
+// p[0 .. n) = v (a kernel, so it can be captured in a graph).
+void launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
+// Workgroups of a persistent launch over nitems: every CU holds as many as
+// fit in its LDS, no more than there are items.
+unsigned persistent_grid(int nitems, size_t lds_bytes);
 // Largest bucket of a built index (synchronises st); scratch: one word.
 uint32_t max_bucket_size(const uint64_t* off, int64_t nbk, uint32_t* scratch, hipStream_t st);
 // Dedup windows over the ids [0, N) and the LDS words of their counters
@@ -127,7 +168,8 @@ void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes
 // ---- reduced-precision scoring + FP32 refine (lowprec.hip) ----------------
 struct LowPrecArgs {
     int format;              // 1 = BF16, 2 = FP8-e4m3 with per-row scale
-    int refine;              // survivors rescored in FP32 (R), 1..256
+    int refine;              // survivors rescored in FP32 (R), 1..256; with per-query k the largest
+    int refine_per_k = 0;    // 1: query q keeps min(refine, 4 * k_q) survivors
     int64_t ldl;             // row stride of the low-precision copy (elements, multiple of 32)
     const uint16_t* xb;      // BF16 copy (bit patterns)
     const uint8_t* x8;       // FP8 copy

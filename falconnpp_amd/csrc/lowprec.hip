@@ -80,14 +80,14 @@ constexpr int lp_tiles(int fmt, int kt) {
 }
 
 template <int TB, int FMT, int KT>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB / 256))) void k_gather_score_lp(GatherArgs a, LowPrecArgs lp,
+__device__ __forceinline__ void k_gather_score_lp_one(GatherArgs a, LowPrecArgs lp,
                                                         const uint32_t* __restrict__ probes,
                                                         const uint64_t* __restrict__ bucket_off,
                                                         const int32_t* __restrict__ ids, const float* __restrict__ Xc,
                                                         const float* __restrict__ Q, int64_t ldq,
                                                         const uint64_t* __restrict__ cand_off,
                                                         int32_t* __restrict__ cand, uint64_t* __restrict__ ncand,
-                                                        int32_t* __restrict__ out_ids, float* __restrict__ out_scores) {
+                                                        int32_t* __restrict__ out_ids, float* __restrict__ out_scores, int q) {
     constexpr int NW = TB / WAVE;
     extern __shared__ uint32_t lds[];
     const int64_t W = a.window_bits;
@@ -103,14 +103,16 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB
     uint32_t* count = reinterpret_cast<uint32_t*>(rid + R);   // 1
     __shared__ int s_digit;
     __shared__ uint32_t s_below;
-    const int q = a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
     uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * 5 : nullptr;
     if (ts && threadIdx.x == 0) ts[0] = wall_clock64();
-    const uint32_t* pr = probes + (size_t)q * a.qProbes;
+    const QueryView qv_ = query_view(a.qp, q, a.qProbes, 0, 0, a.k);
+    const int kq = qv_.k;
+    const int Rq = lp.refine_per_k ? min(R, 4 * kq) : R;
+    const uint32_t* pr = probes + qv_.probe_off;
     const uint64_t cbase = cand_off[q];
     const uint64_t ccap = cand_off[q + 1] - cbase;
     int32_t* cq = cand + cbase;
-    const uint32_t nc = collect_candidates<TB>(a, pr, bucket_off, ids, bits, wc, sh, cq, a.part + cbase, ccap, ts);
+    const uint32_t nc = collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, a.part + cbase, ccap, ts);
     if (threadIdx.x == 0) ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
 
@@ -236,9 +238,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB
     __syncthreads();
     // the R best approximate scores (ties: smaller id)
     const uint32_t rr = nc <= lds_cap
-                            ? select_smallest<TB>(bits, cq, nc, (uint32_t)R, hist, sh, &s_digit, &s_below, count, rid)
+                            ? select_smallest<TB>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid)
                             : select_smallest<TB>(reinterpret_cast<const uint32_t*>(a.part + cbase), cq, nc,
-                                                  (uint32_t)R, hist, sh, &s_digit, &s_below, count, rid);
+                                                  (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid);
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
     // exact FP32 rescoring of the survivors, 8 lanes per row
     const int g8 = threadIdx.x & 7;
@@ -264,9 +266,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB
         if (v && g8 == 0) rkeys[i] = score_key(s, id);
     }
     __syncthreads();
-    int32_t* oq = out_ids + (size_t)q * a.k;
-    float* osq = out_scores ? out_scores + (size_t)q * a.k : nullptr;
-    const uint32_t kk = min((uint32_t)a.k, rr);
+    const int ldo = a.ldo ? a.ldo : a.k;
+    int32_t* oq = out_ids + (size_t)q * ldo;
+    float* osq = out_scores ? out_scores + (size_t)q * ldo : nullptr;
+    const uint32_t kk = min((uint32_t)kq, rr);
     for (uint32_t e = threadIdx.x; e < rr; e += TB) {
         const uint64_t ke = rkeys[e];
         uint32_t r = 0;
@@ -276,11 +279,41 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB
             if (osq) osq[r] = desc_key_any_value((uint32_t)(ke >> 32));
         }
     }
-    for (uint32_t i = kk + threadIdx.x; i < (uint32_t)a.k; i += TB) {
+    for (uint32_t i = kk + threadIdx.x; i < (uint32_t)ldo; i += TB) {
         oq[i] = 0;
         if (osq) osq[i] = 0.0f;
     }
     if (ts && threadIdx.x == 0) ts[4] = wall_clock64();
+}
+
+template <int TB, int FMT, int KT, bool PERSIST>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB / 256))) void k_gather_score_lp(GatherArgs a, LowPrecArgs lp,
+                                                        const uint32_t* __restrict__ probes,
+                                                        const uint64_t* __restrict__ bucket_off,
+                                                        const int32_t* __restrict__ ids, const float* __restrict__ Xc,
+                                                        const float* __restrict__ Q, int64_t ldq,
+                                                        const uint64_t* __restrict__ cand_off,
+                                                        int32_t* __restrict__ cand, uint64_t* __restrict__ ncand,
+                                                        int32_t* __restrict__ out_ids, float* __restrict__ out_scores) {
+    if constexpr (!PERSIST) {
+        const int q = a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
+        k_gather_score_lp_one<TB, FMT, KT>(a, lp, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand,
+                                           out_ids, out_scores, q);
+    } else {
+        // persistent: take queries from the work queue until it is empty
+        __shared__ uint32_t s_item;
+        for (;;) {
+            if (threadIdx.x == 0) s_item = atomicAdd(a.queue, 1u);
+            __syncthreads();
+            const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+            __syncthreads();
+            if (item >= a.nitems) break;
+            const int q = __builtin_amdgcn_readfirstlane((int)a.order[item]);
+            k_gather_score_lp_one<TB, FMT, KT>(a, lp, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand,
+                                               out_ids, out_scores, q);
+            __syncthreads();
+        }
+    }
 }
 
 constexpr int LP_TB = 512;
@@ -290,8 +323,12 @@ void dispatch_lp_launch(const GatherArgs& a, const LowPrecArgs& lp, int nq, size
                         const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
                         const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids,
                         float* out_scores, hipStream_t st) {
-    k_gather_score_lp<LP_TB, FMT, KT><<<nq, LP_TB, lds, st>>>(a, lp, probes, bucket_off, ids, Xc, Q, ldq, cand_off,
-                                                             cand, ncand, out_ids, out_scores);
+    if (a.queue)
+        k_gather_score_lp<LP_TB, FMT, KT, true><<<persistent_grid(nq, lds), LP_TB, lds, st>>>(
+            a, lp, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids, out_scores);
+    else
+        k_gather_score_lp<LP_TB, FMT, KT, false><<<nq, LP_TB, lds, st>>>(a, lp, probes, bucket_off, ids, Xc, Q, ldq,
+                                                                         cand_off, cand, ncand, out_ids, out_scores);
 }
 
 template <int FMT>
@@ -349,6 +386,7 @@ void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, int nq, 
     const size_t lds =
         (size_t)(wwords + window_counter_words(a.N, a.window_bits) + lowprec_extra_words(lp.refine)) * 4;
     FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "LDS budget exceeded (refine count or qProbes too large)");
+    if (a.queue) launch_fill_u32(a.queue, 0u, 1, st);
     if (lp.format == BF16)
         dispatch_lp_kt<BF16>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
                              out_scores, st);

@@ -32,10 +32,12 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
                                                        uint32_t* __restrict__ probes, uint64_t* __restrict__ ctot,
                                                        uint32_t* __restrict__ qflag) {
     extern __shared__ uint32_t lds[];
-    const int PL = g.PL, np = PL * PL;
+    const int q = blockIdx.x;
+    const QueryView qv_ = query_view(g.qp, q, g.qProbes, g.perTable, g.PL, 0);
+    const int PL = qv_.PL, np = PL * PL, perTable = qv_.perTable, qProbes = qv_.qProbes;
     // FULLPOOL: every pair of every table enters the pool (per-table counts
     // checked below); else the top-perTable pairs of each table
-    const int per = FULLPOOL ? np : g.perTable;
+    const int per = FULLPOOL ? np : perTable;
     const int pool_n = g.L * per;
     uint32_t* pkey = lds;
     uint32_t* pbkt = lds + pool_n;
@@ -46,7 +48,6 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     __shared__ uint32_t s_below;
     __shared__ uint32_t s_tie;
     __shared__ unsigned long long s_ctot;
-    const int q = blockIdx.x;
     const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
     const uint64_t bmask = lanes_below_mask();
     if (threadIdx.x == 0) {
@@ -58,15 +59,16 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     __syncthreads();
     uint32_t* hw = whist + w * 256;
     bool tie = false;
-    // a table's two top lists sit in lanes: v1/c1 in lanes [0, PL), v2/c2 in
-    // [PL, 2 PL); the wave's next table is fetched while this one is selected
+    // a table's two top lists (stored at stride g.PL) sit in lanes: v1/c1 in
+    // lanes [0, PL), v2/c2 in [PL, 2 PL); the wave's next table is fetched
+    // while this one is selected
     auto fetch = [&](int l, float& v, int32_t& c) {
         v = 0.0f;
         c = 0;
         if (l < g.L && lane < 2 * PL) {
-            const size_t base = ((size_t)q * g.L + l) * 2 * PL;
-            v = qv[base + lane];
-            c = qc[base + lane];
+            const size_t base = ((size_t)q * g.L + l) * 2 * g.PL + (lane < PL ? lane : g.PL + lane - PL);
+            v = qv[base];
+            c = qc[base];
         }
     };
     constexpr int KR = 16;  // pairs per lane: PL <= 31, so PL * PL <= 16 * 64
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
         }
         if constexpr (FULLPOOL) continue;
         // perTable-th largest pair sum
-        uint32_t prefix = 0, pmask = 0, k = (uint32_t)g.perTable, eqc = 0;
+        uint32_t prefix = 0, pmask = 0, k = (uint32_t)perTable, eqc = 0;
         for (int shift = 24; shift >= 0; shift -= 8) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) hw[lane * 4 + i] = 0;
@@ -119,7 +121,7 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
             k -= below;
             __builtin_amdgcn_wave_barrier();
         }
-        if (np > g.perTable && eqc != k) tie = true;
+        if (np > perTable && eqc != k) tie = true;
         const uint32_t T = prefix;
         uint32_t written = 0, eq_seen = 0;
 #pragma unroll
@@ -136,8 +138,8 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
             const bool take = lt || take_eq;
             const uint64_t mtake = __ballot(take);
             const uint32_t slot = written + (uint32_t)__popcll(mtake & bmask);
-            if (take && slot < (uint32_t)g.perTable) {
-                const int idx = l * g.perTable + (int)slot;
+            if (take && slot < (uint32_t)perTable) {
+                const int idx = l * perTable + (int)slot;
                 pkey[idx] = key[r];
                 pbkt[idx] = (uint32_t)l * (uint32_t)g.NB + (uint32_t)ci * (uint32_t)g.two_np + (uint32_t)cj;
             }
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     if (tie) atomicOr(&s_tie, 1u);
     __syncthreads();
     // global top-qProbes of the pool: radix select, the digit found by wave 0
-    uint32_t prefix = 0, pmask = 0, k = (uint32_t)g.qProbes, eqc = 0;
+    uint32_t prefix = 0, pmask = 0, k = (uint32_t)qProbes, eqc = 0;
     __shared__ uint32_t s_eqc, s_cnt;
     for (int shift = 24; shift >= 0; shift -= 8) {
         bhist[threadIdx.x] = 0;
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     unsigned long long my_ctot = 0;
-    uint32_t* out = probes + (size_t)q * g.qProbes;
+    uint32_t* out = probes + qv_.probe_off;
     for (int c0 = 0; c0 < pool_n; c0 += PS_T) {
         const int i = c0 + threadIdx.x;
         const bool take = i < pool_n && pkey[i] <= T;
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
         if (lane == leader) base = atomicAdd(&s_cnt, (uint32_t)__popcll(m));
         base = __shfl(base, leader, WAVE);
         const uint32_t pos = base + lane_rank(m);
-        if (take && pos < (uint32_t)g.qProbes) {
+        if (take && pos < (uint32_t)qProbes) {
             const uint32_t b = pbkt[i];
             out[pos] = b;
             my_ctot += bucket_off[b + 1] - bucket_off[b];
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
         // above the global cut, else the query is replayed exactly
         __syncthreads();
         bool over = false;
-        for (int l = threadIdx.x; l < g.L; l += PS_T) over |= tcnt[l] > (uint32_t)g.perTable;
+        for (int l = threadIdx.x; l < g.L; l += PS_T) over |= tcnt[l] > (uint32_t)perTable;
         if (over) atomicOr(&s_tie, 1u);
     }
     atomicAdd(&s_ctot, my_ctot);
@@ -294,7 +296,8 @@ __global__ __launch_bounds__(WAVE) void k_probe_exact_tables(HashGeom hg, ProbeG
     const uint32_t f = blockIdx.x / g.L;
     const int l = (int)(blockIdx.x % g.L);
     const uint32_t q = qlist[f];
-    const int F = hg.F, PL = g.PL, PT = g.perTable;
+    const QueryView qv_ = query_view(g.qp, (int)q, g.qProbes, g.perTable, g.PL, 0);
+    const int F = hg.F, PL = qv_.PL, PT = qv_.perTable;
     const int lane = threadIdx.x;
     float* x1 = xs;
     float* x2 = x1 + F;
@@ -311,10 +314,10 @@ __global__ __launch_bounds__(WAVE) void k_probe_exact_tables(HashGeom hg, ProbeG
         if (lane == 0) layer_heap(x1, hg.n_proj, hg.log2_nproj, PL, hs1, v1);
         if (lane == 1) layer_heap(x2, hg.n_proj, hg.log2_nproj, PL, hs2, v2);
     } else {
-        const size_t base = ((size_t)q * g.L + l) * 2 * PL;
+        const size_t base = ((size_t)q * g.L + l) * 2 * g.PL;
         for (int i = lane; i < PL; i += WAVE) {
             v1[i] = IFPair{qc[base + i], qvals[base + i]};
-            v2[i] = IFPair{qc[base + PL + i], qvals[base + PL + i]};
+            v2[i] = IFPair{qc[base + g.PL + i], qvals[base + g.PL + i]};
         }
     }
     __syncthreads();
@@ -363,11 +366,12 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_select(ProbeGeom g, const 
     __shared__ unsigned long long s_ct;
     const uint32_t f = blockIdx.x;
     const uint32_t q = qlist[f];
-    const int n = g.L * g.perTable;
+    const QueryView qv_ = query_view(g.qp, (int)q, g.qProbes, g.perTable, g.PL, 0);
+    const int n = g.L * qv_.perTable;
     const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
     const IFPair* pool = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
     // descending order: smaller key = larger value
-    uint32_t prefix = 0, pmask = 0, k = (uint32_t)g.qProbes, eqc = 0;
+    uint32_t prefix = 0, pmask = 0, k = (uint32_t)qv_.qProbes, eqc = 0;
     for (int shift = 24; shift >= 0; shift -= 8) {
         hist[threadIdx.x] = 0;
         __syncthreads();
@@ -399,7 +403,7 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_select(ProbeGeom g, const 
         s_ct = 0;
     }
     __syncthreads();
-    uint32_t* out = probes + (size_t)q * g.qProbes;
+    uint32_t* out = probes + qv_.probe_off;
     unsigned long long ct = 0;
     for (int c0 = 0; c0 < n; c0 += MG_T) {
         const int i = c0 + threadIdx.x;
@@ -416,7 +420,7 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_select(ProbeGeom g, const 
         if (lane == leader) base = atomicAdd(&s_cnt, (uint32_t)__popcll(m));
         base = __shfl(base, leader, WAVE);
         const uint32_t pos = base + lane_rank(m);
-        if (take && pos < (uint32_t)g.qProbes) {
+        if (take && pos < (uint32_t)qv_.qProbes) {
             const uint32_t b = (uint32_t)e.idx;
             out[pos] = b;
             ct += bucket_off[b + 1] - bucket_off[b];
@@ -442,7 +446,8 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const u
     const uint32_t f = blockIdx.x;
     const uint32_t q = qlist[f];
     if (qflag[q] & RESOLVED) return;
-    const int L = g.L, PT = g.perTable;
+    const QueryView qv_ = query_view(g.qp, (int)q, g.qProbes, g.perTable, g.PL, 0);
+    const int L = g.L, PT = qv_.perTable, QP = qv_.qProbes;
     const IFPair* list_g = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
     IFPair* list = mlds;
     IFPair* mq = list + (int64_t)L * PT;
@@ -450,12 +455,12 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const u
     for (int64_t i = threadIdx.x; i < (int64_t)L * PT; i += MG_T) list[i] = list_g[i];
     for (int l = threadIdx.x; l < L; l += MG_T) track[l] = 0;
     __syncthreads();
-    uint32_t* out = probes + (size_t)q * g.qProbes;
+    uint32_t* out = probes + qv_.probe_off;
     if (threadIdx.x == 0) {
         MaxHeap mh;
         mh.init(mq);
         for (int l = 0; l < L; ++l) mh.push(list[(int64_t)l * PT]);
-        for (int c = 0; c < g.qProbes; ++c) {
+        for (int c = 0; c < QP; ++c) {
             const IFPair pr = mh.top();
             mh.pop();
             out[c] = (uint32_t)pr.idx;
@@ -470,7 +475,7 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const u
     if (threadIdx.x == 0) s_ct = 0;
     __syncthreads();
     unsigned long long ct = 0;
-    for (int c = threadIdx.x; c < g.qProbes; c += MG_T) {
+    for (int c = threadIdx.x; c < QP; c += MG_T) {
         const uint32_t b = out[c];
         ct += bucket_off[b + 1] - bucket_off[b];
     }
@@ -485,14 +490,14 @@ size_t merge_lds_bytes(const ProbeGeom& g) {
 
 // ---- gather + dedup + FP32 rerank + top-k ---------------------------------
 template <int TB, int CPL>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB / 256))) void k_gather_score_f32(GatherArgs a, const uint32_t* __restrict__ probes,
+__device__ __forceinline__ void k_gather_score_f32_one(GatherArgs a, const uint32_t* __restrict__ probes,
                                                           const uint64_t* __restrict__ bucket_off,
                                                           const int32_t* __restrict__ ids,
                                                           const float* __restrict__ Xc, const float* __restrict__ Q,
                                                           int64_t ldq, const uint64_t* __restrict__ cand_off,
                                                           int32_t* __restrict__ cand, float* __restrict__ score,
                                                           uint64_t* __restrict__ ncand, int32_t* __restrict__ out_ids,
-                                                          float* __restrict__ out_scores) {
+                                                          float* __restrict__ out_scores, int q) {
     extern __shared__ uint32_t lds[];
     const int64_t W = a.window_bits;
     const int Wwords = (int)(W / 32);
@@ -505,15 +510,16 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB
     uint32_t* tk_id = tk_key + 1024;    // kcap
     __shared__ int s_digit;
     __shared__ uint32_t s_below;
-    const int q = a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
     uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * 5 : nullptr;
     if (ts && threadIdx.x == 0) ts[0] = wall_clock64();
-    const uint32_t* pr = probes + (size_t)q * a.qProbes;
+    const QueryView qv_ = query_view(a.qp, q, a.qProbes, 0, 0, a.k);
+    const int kq = qv_.k;
+    const uint32_t* pr = probes + qv_.probe_off;
     const uint64_t cbase = cand_off[q];
     const uint64_t ccap = cand_off[q + 1] - cbase;
     int32_t* cq = cand + cbase;
     float* sq = score + cbase;
-    const uint32_t nc = collect_candidates<TB>(a, pr, bucket_off, ids, bits, wc, sh, cq, a.part + cbase, ccap, ts);
+    const uint32_t nc = collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, a.part + cbase, ccap, ts);
     if (threadIdx.x == 0) ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
     // FP32 scores: groups of 8 lanes per candidate row, CPL float4 chunks per
@@ -538,9 +544,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB
     }
     // wave-level running top-k (k <= 64): lane j holds the wave's j-th best
     // key (desc_key(score) << 32 | id), ascending; misses cost one compare
-    const bool wsel = a.k <= WAVE;
+    const bool wsel = kq <= WAVE;
     WaveTopK<1> tk;
-    tk.init(a.k);
+    tk.init(kq);
     int idn[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -597,16 +603,17 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB
     }
     __syncthreads();
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
-    int32_t* oq = out_ids + (size_t)q * a.k;
-    float* osq = out_scores ? out_scores + (size_t)q * a.k : nullptr;
-    const uint32_t kk = min((uint32_t)a.k, nc);
+    const int ldo = a.ldo ? a.ldo : a.k;
+    int32_t* oq = out_ids + (size_t)q * ldo;
+    float* osq = out_scores ? out_scores + (size_t)q * ldo : nullptr;
+    const uint32_t kk = min((uint32_t)kq, nc);
     if (wsel) {
         // merge the waves' lists: rank each entry among all of them
         constexpr int NW = TB / WAVE;
         uint64_t* wl = reinterpret_cast<uint64_t*>(tk_key);  // NW * k entries (<= 1024 x 8 B)
-        if (lane < a.k) wl[wv * a.k + lane] = tk.w[0];
+        if (lane < kq) wl[wv * kq + lane] = tk.w[0];
         __syncthreads();
-        const int tot = NW * a.k;
+        const int tot = NW * kq;
         for (int e = threadIdx.x; e < tot; e += TB) {
             const uint64_t ke = wl[e];
             if (ke == ~0ull) continue;
@@ -688,11 +695,41 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB
             if (osq) osq[r] = desc_key_any_value(ki);
         }
     }
-    for (uint32_t i = kk + threadIdx.x; i < (uint32_t)a.k; i += TB) {
+    for (uint32_t i = kk + threadIdx.x; i < (uint32_t)ldo; i += TB) {
         oq[i] = 0;
         if (osq) osq[i] = 0.0f;
     }
     if (ts && threadIdx.x == 0) ts[4] = wall_clock64();
+}
+
+template <int TB, int CPL, bool PERSIST>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB / 256))) void k_gather_score_f32(GatherArgs a, const uint32_t* __restrict__ probes,
+                                                          const uint64_t* __restrict__ bucket_off,
+                                                          const int32_t* __restrict__ ids,
+                                                          const float* __restrict__ Xc, const float* __restrict__ Q,
+                                                          int64_t ldq, const uint64_t* __restrict__ cand_off,
+                                                          int32_t* __restrict__ cand, float* __restrict__ score,
+                                                          uint64_t* __restrict__ ncand, int32_t* __restrict__ out_ids,
+                                                          float* __restrict__ out_scores) {
+    if constexpr (!PERSIST) {
+        const int q = a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
+        k_gather_score_f32_one<TB, CPL>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand,
+                                        out_ids, out_scores, q);
+    } else {
+        // persistent: take queries from the work queue until it is empty
+        __shared__ uint32_t s_item;
+        for (;;) {
+            if (threadIdx.x == 0) s_item = atomicAdd(a.queue, 1u);
+            __syncthreads();
+            const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+            __syncthreads();
+            if (item >= a.nitems) break;
+            const int q = __builtin_amdgcn_readfirstlane((int)a.order[item]);
+            k_gather_score_f32_one<TB, CPL>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand,
+                                            out_ids, out_scores, q);
+            __syncthreads();
+        }
+    }
 }
 
 // Sort key of a query: its best bucket of table 0 (layer-1 and layer-2 top
@@ -750,6 +787,11 @@ __global__ __launch_bounds__(SO_T) void k_split_order(const uint32_t* __restrict
     }
 }
 
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 size_t gather_lds_bytes(const GatherArgs& a, int TB) {
     const int64_t wwords = (a.window_bits / 32 + 1) & ~1;
     return (size_t)(wwords + window_counter_words(a.N, a.window_bits) + TB + 256 + 2048) * 4;
@@ -764,7 +806,12 @@ void dispatch_gather_cpl(const GatherArgs& a, int nq, const uint32_t* probes, co
     const int cpl = (int)ceil_div(nchunk, 8);
     const size_t lds = gather_lds_bytes(a, TB);
 #define FPP_G(C) \
-    k_gather_score_f32<TB, C><<<nq, TB, lds, st>>>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids, out_scores)
+    if (a.queue)                                                                                                    \
+        k_gather_score_f32<TB, C, true><<<persistent_grid(nq, lds), TB, lds, st>>>(                                   \
+            a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids, out_scores);               \
+    else                                                                                                            \
+        k_gather_score_f32<TB, C, false><<<nq, TB, lds, st>>>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, \
+                                                             score, ncand, out_ids, out_scores)
     if (cpl <= 1)
         FPP_G(1);
     else if (cpl <= 2)
@@ -802,6 +849,23 @@ void launch_order_keys(const int32_t* qc, int nq, int L, int PL, int two_np, uin
     if (nq == 0) return;
     k_order_keys<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(qc, nq, L, PL, two_np, key);
     FPP_HIP(hipGetLastError());
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st) {
+    if (n <= 0) return;
+    k_fill_u32<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(p, v, n);
+    FPP_HIP(hipGetLastError());
+}
+
+unsigned persistent_grid(int nitems, size_t lds_bytes) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        FPP_HIP(hipGetDevice(&dev));
+        FPP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const int per_cu = std::max<int>(1, (int)(160 * 1024 / (lds_bytes + 1024)));
+    return (unsigned)std::max(1, std::min(nitems, cus * per_cu));
 }
 
 void launch_split_order(const uint32_t* order, int nq, const uint32_t* qflag, uint32_t* order_u, hipStream_t st) {
@@ -860,6 +924,7 @@ void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes
     FPP_REQUIRE(a.ldx % 4 == 0, FPP_EINTERNAL, "row stride must be a multiple of 4");
     FPP_REQUIRE(a.ldx <= 1024, FPP_EINVAL, "n_features must be <= 1024");
     if (nq == 0) return;
+    if (a.queue) launch_fill_u32(a.queue, 0u, 1, st);
     if (threads <= 256)
         dispatch_gather_cpl<256>(a, nq, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids,
                                  out_scores, st);

@@ -21,10 +21,11 @@ EXPORTS = (
     "fpp_clear", "fpp_build", "fpp_build_device", "fpp_query", "fpp_query_device", "fpp_get_shape",
     "fpp_get_center", "fpp_get_sign_bits", "fpp_get_buckets", "fpp_hash_points", "fpp_get_stats",
     "fpp_get_last_probes", "fpp_get_last_candidates", "fpp_bruteforce_f64", "fpp_set_profiling",
-    "fpp_get_timings", "fpp_set_tuning", "fpp_set_precision",
+    "fpp_get_timings", "fpp_set_tuning", "fpp_set_precision", "fpp_autotune", "fpp_query_batch_device",
+    "fpp_get_query_times",
 )
 PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
-TUNE_QUERY_ORDER, TUNE_GATHER_THREADS = 1, 2
+TUNE_QUERY_ORDER, TUNE_GATHER_THREADS, TUNE_PERSISTENT, TUNE_GRAPH = 1, 2, 3, 4
 
 FPP_EINVAL, FPP_ESTATE, FPP_EHIP, FPP_ENOMEM, FPP_EINTERNAL = 1, 2, 3, 4, 5
 
@@ -70,6 +71,9 @@ def load():
     L.fpp_get_timings.argtypes = [vp, vp]
     L.fpp_set_tuning.argtypes = [vp, i32, i32]
     L.fpp_set_precision.argtypes = [vp, i32, i32]
+    L.fpp_autotune.argtypes = [vp, vp, i64, i32, i32, vp, vp]
+    L.fpp_get_query_times.argtypes = [vp, vp, vp]
+    L.fpp_query_batch_device.argtypes = [vp, vp, i64, i32, vp, vp, i32, vp, vp, vp]
     L.fpp_get_last_candidates.restype = i64
     L.fpp_get_last_candidates.argtypes = [vp, i32, vp, i64]
     L.fpp_bruteforce_f64.argtypes = [vp, i64, i64, i32, vp, i64, i32, i32, vp, vp, vp]
@@ -139,6 +143,23 @@ class Index:
         out = np.zeros((nq, k), np.int32)
         _check(load().fpp_query(self.h, _ptr(Q), self.d, nq, k, _ptr(out), 1 if verbose else 0))
         return out
+
+    def query_batch_device(self, dQ_ptr: int, ldq: int, nq: int, qprobes, ks, kmax: int, dout_ptr: int,
+                           dscores_ptr: int = 0, stream: int = 0):
+        """Per-query qProbes and k (host arrays); outputs nq x kmax on the device."""
+        qp = np.ascontiguousarray(qprobes, dtype=np.int32)
+        kk = np.ascontiguousarray(ks, dtype=np.int32)
+        assert len(qp) == nq and len(kk) == nq
+        _check(load().fpp_query_batch_device(self.h, C.c_void_p(dQ_ptr), ldq, nq, _ptr(qp), _ptr(kk), kmax,
+                                             C.c_void_p(dout_ptr), C.c_void_p(dscores_ptr or None),
+                                             C.c_void_p(stream or None)))
+
+    def autotune(self, dQ_ptr: int, ldq: int, nq: int, k: int, stream: int = 0) -> dict:
+        """Time the launch shapes on these queries and keep the fastest."""
+        ch = np.zeros(4, np.int32)
+        _check(load().fpp_autotune(self.h, C.c_void_p(dQ_ptr), ldq, nq, k, C.c_void_p(stream or None), _ptr(ch)))
+        return {"gather_threads": int(ch[0]), "query_order": int(ch[1]), "persistent": int(ch[2]),
+                "us_per_batch": int(ch[3])}
 
     def query_device(self, dQ_ptr: int, ldq: int, nq: int, k: int, dout_ptr: int, dscores_ptr: int = 0,
                      stream: int = 0):
@@ -210,6 +231,20 @@ class Index:
         out = np.zeros((nq, self.qprobes), np.uint32)
         _check(load().fpp_get_last_probes(self.h, _ptr(out)))
         return out
+
+    def query_times(self, nq: int):
+        """(service_us, done_us) per query of the last level-2 profiled call."""
+        a = np.zeros(nq, np.float64)
+        b = np.zeros(nq, np.float64)
+        _check(load().fpp_get_query_times(self.h, _ptr(a), _ptr(b)))
+        return a, b
+
+    def last_probes_batch(self, qprobes) -> list:
+        """Probes of the last batched call, one array per query."""
+        qp = np.asarray(qprobes, np.int64)
+        flat = np.zeros(max(int(qp.sum()), 1), np.uint32)
+        _check(load().fpp_get_last_probes(self.h, _ptr(flat)))
+        return np.split(flat[:int(qp.sum())], np.cumsum(qp)[:-1])
 
     def last_candidates(self, j: int) -> np.ndarray:
         n = load().fpp_get_last_candidates(self.h, j, None, 0)

@@ -65,6 +65,10 @@ CONFIGS = {
     # Plain FP32 GPU drop-in: N=1e6 x d=200 mixture of 1000 centres.
     "c2": dict(n=1_000_000, d=200, kind="mixture", n_centres=1000, nq=10_000, L=100, n_proj=256, iProbes=3,
                alpha=0.005, bucketLimit=20, qProbes=4000, k=20),
+    # Batched serving: the N=1e7 x d=128 index, 50k queries, each with its own
+    # qProbes (uniform in [100, 8000]) and k (one of 1, 10, 100).
+    "c4": dict(n=10_000_000, d=128, kind="mixture", n_centres=5000, nq=50_000, L=64, n_proj=128, iProbes=2,
+               alpha=0.01, bucketLimit=20, qProbes=2000, k=100, qp_range=(100, 8000), k_choices=(1, 10, 100)),
     # Precision variant: N=1e7 x d=128, 5000 centres.
     "c3": dict(n=10_000_000, d=128, kind="mixture", n_centres=5000, nq=10_000, L=64, n_proj=128, iProbes=2,
                alpha=0.01, bucketLimit=20, qProbes=2000, k=20),
@@ -86,3 +90,13 @@ def make_config(name: str, scale: float = 1.0):
         Q = gaussian_mixture(nq, d, cfg["n_centres"], 0.3, 0.25, QUERY_SEED, C)
     cfg["n"], cfg["nq"] = n, nq
     return X, Q, cfg
+
+
+def per_query_params(cfg: dict, nq: int, seed: int = QUERY_SEED + 1):
+    """Per-query qProbes (uniform integers in cfg["qp_range"]) and k (uniform
+    over cfg["k_choices"]) of a batched-serving configuration."""
+    rng = np.random.default_rng(seed)
+    lo, hi = cfg["qp_range"]
+    qps = rng.integers(lo, hi + 1, nq).astype(np.int32)
+    ks = rng.choice(np.asarray(cfg["k_choices"], np.int32), nq).astype(np.int32)
+    return qps, ks

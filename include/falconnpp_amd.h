@@ -110,11 +110,33 @@ int fpp_get_timings(fpp_index* h, double* ms);
 int fpp_set_precision(fpp_index* h, int mode, int refine);
 /* launch-shape knobs (no reference counterpart): FPP_TUNE_QUERY_ORDER 0/1
  * (process queries grouped by their best table-0 bucket, XCD-aware),
- * FPP_TUNE_GATHER_THREADS 256/512/1024 (workgroup of the gather kernel) */
-enum { FPP_TUNE_QUERY_ORDER = 1, FPP_TUNE_GATHER_THREADS = 2 };
+ * FPP_TUNE_GATHER_THREADS 256/512/1024 (workgroup of the gather kernel),
+ * FPP_TUNE_PERSISTENT 0/1 (one persistent gather launch whose workgroups take
+ * queries from a work queue, longest first), FPP_TUNE_GRAPH 0/1 (hashing and
+ * probe selection of a batch captured once in a hipGraph and replayed while
+ * the batch size and buffers stay the same).  None changes any result. */
+enum { FPP_TUNE_QUERY_ORDER = 1, FPP_TUNE_GATHER_THREADS = 2, FPP_TUNE_PERSISTENT = 3, FPP_TUNE_GRAPH = 4 };
 int fpp_set_tuning(fpp_index* h, int which, int value);
-/* probes of the last query call (nq x qProbes global bucket ids, as a set per
- * query) and, for query j, its unique candidates (ascending; returns count) */
+/* Autotuner (no reference counterpart): times every combination of the
+ * gather workgroup size, query order and persistent launch on the given
+ * device queries and keeps the fastest.  chosen (may be NULL) receives
+ * {threads, order, persistent, microseconds per batch}. */
+int fpp_autotune(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, void* stream, int32_t* chosen);
+/* Batched serving (no reference counterpart; each query is
+ * query2Layers_1D, src/FalconnPP.cpp:310-707, with its own qProbes and k):
+ * qprobes and ks are HOST arrays of nq, dQ/d_out/d_scores device pointers.
+ * Row q of d_out (nq x kmax int32) holds the ks[q] best ids, then zeros;
+ * d_scores (may be NULL) likewise. */
+int fpp_query_batch_device(fpp_index* h, const float* dQ, int64_t ldq, int nq, const int32_t* qprobes,
+                           const int32_t* ks, int kmax, int32_t* d_out, float* d_scores, void* stream);
+/* Per-query times of the last query call made with profiling level 2 (no
+ * reference counterpart): service_us[q] = time query q spent in the gather
+ * kernel, done_us[q] = its completion after the first query of the call
+ * entered the gather.  Both arrays hold nq doubles. */
+int fpp_get_query_times(fpp_index* h, double* service_us, double* done_us);
+/* probes of the last query call (global bucket ids, qProbes per query in
+ * query order, as a set per query) and, for query j, its unique candidates
+ * (ascending; returns count) */
 int fpp_get_last_probes(fpp_index* h, uint32_t* out);
 int64_t fpp_get_last_candidates(fpp_index* h, int j, int32_t* out, int64_t cap);
 

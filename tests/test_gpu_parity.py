@@ -332,3 +332,124 @@ def test_low_precision_rejects_k_above_refine():
         assert e.value.code == E.FPP_EINVAL
     finally:
         g.set_precision(E.PREC_FP32, 0)
+
+
+# ---- batched serving: per-query qProbes and k, persistent gather, graph ----
+
+def _batch_case(seed=3):
+    X, Q, g, o = _built("mix_d128_p2")
+    rng = np.random.default_rng(seed)
+    qps = rng.choice([20, 60, 160, 400], len(Q)).astype(np.int32)
+    ks = rng.choice([1, 5, 10], len(Q)).astype(np.int32)
+    return X, Q, g, qps, ks
+
+
+def _run_batch(g, Q, qps, ks, kmax):
+    import torch
+    dQ = torch.from_numpy(np.ascontiguousarray(Q)).cuda()
+    out = torch.zeros((len(Q), kmax), dtype=torch.int32, device="cuda")
+    sc = torch.zeros((len(Q), kmax), dtype=torch.float32, device="cuda")
+    g.query_batch_device(dQ.data_ptr(), Q.shape[1], len(Q), qps, ks, kmax, out.data_ptr(), sc.data_ptr())
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), sc.cpu().numpy()
+
+
+def _run_single(g, Q, qp, k):
+    import torch
+    g.set_qProbes(int(qp))
+    dQ = torch.from_numpy(np.ascontiguousarray(Q)).cuda()
+    out = torch.zeros((len(Q), k), dtype=torch.int32, device="cuda")
+    sc = torch.zeros((len(Q), k), dtype=torch.float32, device="cuda")
+    g.query_device(dQ.data_ptr(), Q.shape[1], len(Q), k, out.data_ptr(), sc.data_ptr())
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), sc.cpu().numpy()
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_batch_equals_unbatched(precision, persistent):
+    """Every query of a batch with mixed qProbes and k returns what the plain
+    call with that query's qProbes and k returns (ids, scores, probes)."""
+    X, Q, g, qps, ks = _batch_case()
+    kmax = int(ks.max())
+    g.set_precision(precision, 0)
+    g.set_tuning(E.TUNE_PERSISTENT, persistent)
+    try:
+        out, sc = _run_batch(g, Q, qps, ks, kmax)
+        probes = g.last_probes_batch(qps)
+        for qp in np.unique(qps):
+            for k in np.unique(ks):
+                sel = np.where((qps == qp) & (ks == k))[0]
+                if len(sel) == 0:
+                    continue
+                ro, rs = _run_single(g, Q[sel], qp, int(k))
+                rp = g.last_probes(len(sel))
+                for i, j in enumerate(sel):
+                    assert np.array_equal(out[j, :k], ro[i]), (qp, k, j)
+                    assert np.array_equal(sc[j, :k], rs[i]), (qp, k, j)
+                    assert np.all(out[j, k:] == 0) and np.all(sc[j, k:] == 0)
+                    assert np.array_equal(np.sort(probes[j]), np.sort(rp[i])), (qp, k, j)
+    finally:
+        g.set_precision(0, 0)
+        g.set_tuning(E.TUNE_PERSISTENT, 0)
+
+
+def test_persistent_equals_one_workgroup_per_query():
+    X, Q, g, o = _built("multiwindow_n1.3m")
+    qp, k = CASES["multiwindow_n1.3m"][6], CASES["multiwindow_n1.3m"][7]
+    ref, rsc = _run_single(g, Q, qp, k)
+    g.set_tuning(E.TUNE_PERSISTENT, 1)
+    try:
+        out, sc = _run_single(g, Q, qp, k)
+    finally:
+        g.set_tuning(E.TUNE_PERSISTENT, 0)
+    assert np.array_equal(out, ref) and np.array_equal(sc, rsc)
+
+
+def test_graph_replays_equal_eager():
+    """Hashing + probe selection captured once and replayed: every replay,
+    with eager calls and synchronisations in between, equals the eager run."""
+    X, Q, g, qps, ks = _batch_case(seed=5)
+    kmax = int(ks.max())
+    ref, rsc = _run_batch(g, Q, qps, ks, kmax)
+    g.set_tuning(E.TUNE_GRAPH, 1)
+    try:
+        for rep in range(4):
+            out, sc = _run_batch(g, Q, qps, ks, kmax)
+            assert np.array_equal(out, ref) and np.array_equal(sc, rsc), rep
+            g.set_tuning(E.TUNE_GRAPH, 0)
+            e_out, _ = _run_single(g, Q, 160, 10)     # eager work between replays
+            g.set_tuning(E.TUNE_GRAPH, 1)
+            # the graph was dropped by the knob and is captured again
+    finally:
+        g.set_tuning(E.TUNE_GRAPH, 0)
+    # replays without re-capture: the same batch four times in a row, with
+    # eager device work of the program and synchronisations in between
+    import torch
+    g.set_tuning(E.TUNE_GRAPH, 1)
+    try:
+        for rep in range(4):
+            out, sc = _run_batch(g, Q, qps, ks, kmax)
+            assert np.array_equal(out, ref) and np.array_equal(sc, rsc), rep
+            t = torch.arange(1 << 20, device="cuda", dtype=torch.float32)
+            assert float((t * 2).max().item()) == 2.0 * ((1 << 20) - 1)
+            torch.cuda.synchronize()
+    finally:
+        g.set_tuning(E.TUNE_GRAPH, 0)
+
+
+def test_autotune_keeps_results():
+    import torch
+    X, Q, g, o = _built("mix_d128_p2")
+    qp, k = CASES["mix_d128_p2"][6], CASES["mix_d128_p2"][7]
+    ref, rsc = _run_single(g, Q, qp, k)
+    dQ = torch.from_numpy(np.ascontiguousarray(Q)).cuda()
+    ch = g.autotune(dQ.data_ptr(), Q.shape[1], len(Q), k)
+    assert ch["gather_threads"] in (256, 512, 1024) and ch["us_per_batch"] > 0
+    try:
+        out, sc = _run_single(g, Q, qp, k)
+        assert np.array_equal(out, ref) and np.array_equal(sc, rsc)
+    finally:
+        g.set_tuning(E.TUNE_PERSISTENT, 0)
+        g.set_tuning(E.TUNE_QUERY_ORDER, 1)
+        g.set_tuning(E.TUNE_GATHER_THREADS, 512)
