@@ -455,6 +455,8 @@ void dispatch_query_f(const HashGeom& g, const float* Q, int64_t ldq, int nq, in
         FPP_V(12)
     } else if (PL + 1 <= 16) {
         FPP_V(16)
+    } else if (PL + 1 <= 24) {
+        FPP_V(24)
     } else {
         FPP_V(32)
     }

@@ -134,6 +134,7 @@ struct GatherArgs {
     // until nitems (queue zeroed beforehand); null: one workgroup per query
     uint32_t* queue = nullptr;
     uint32_t nitems = 0;
+    int spare_cus = 0;  // persistent launch: CUs left free for concurrent work
 };
 // Query order for cache reuse: sort key per query, then the XCD-aware map.
 void launch_order_keys(const int32_t* qc, int nq, int L, int PL, int two_np, uint32_t* key, hipStream_t st);
@@ -143,9 +144,9 @@ void launch_xcd_order(const uint32_t* sorted, int nq, int nxcd, uint32_t* order,
 void launch_split_order(const uint32_t* order, int nq, const uint32_t* qflag, uint32_t* order_u, hipStream_t st);
 // p[0 .. n) = v (a kernel, so it can be captured in a graph).
 void launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
-// Workgroups of a persistent launch over nitems: every CU holds as many as
-// fit in its LDS, no more than there are items.
-unsigned persistent_grid(int nitems, size_t lds_bytes);
+// Workgroups of a persistent launch over nitems: every CU but spare_cus holds
+// as many as fit in its LDS, no more than there are items.
+unsigned persistent_grid(int nitems, size_t lds_bytes, int spare_cus);
 // Largest bucket of a built index (synchronises st); scratch: one word.
 uint32_t max_bucket_size(const uint64_t* off, int64_t nbk, uint32_t* scratch, hipStream_t st);
 // Dedup windows over the ids [0, N) and the LDS words of their counters

@@ -324,7 +324,7 @@ void dispatch_lp_launch(const GatherArgs& a, const LowPrecArgs& lp, int nq, size
                         const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids,
                         float* out_scores, hipStream_t st) {
     if (a.queue)
-        k_gather_score_lp<LP_TB, FMT, KT, true><<<persistent_grid(nq, lds), LP_TB, lds, st>>>(
+        k_gather_score_lp<LP_TB, FMT, KT, true><<<persistent_grid(nq, lds, a.spare_cus), LP_TB, lds, st>>>(
             a, lp, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids, out_scores);
     else
         k_gather_score_lp<LP_TB, FMT, KT, false><<<nq, LP_TB, lds, st>>>(a, lp, probes, bucket_off, ids, Xc, Q, ldq,

@@ -807,7 +807,7 @@ void dispatch_gather_cpl(const GatherArgs& a, int nq, const uint32_t* probes, co
     const size_t lds = gather_lds_bytes(a, TB);
 #define FPP_G(C) \
     if (a.queue)                                                                                                    \
-        k_gather_score_f32<TB, C, true><<<persistent_grid(nq, lds), TB, lds, st>>>(                                   \
+        k_gather_score_f32<TB, C, true><<<persistent_grid(nq, lds, a.spare_cus), TB, lds, st>>>(                                   \
             a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids, out_scores);               \
     else                                                                                                            \
         k_gather_score_f32<TB, C, false><<<nq, TB, lds, st>>>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, \
@@ -838,7 +838,7 @@ void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const fl
     const size_t lds = (size_t)2 * g.L * g.perTable * 4 + fixed;
     FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "probe pool (n_tables * perTable) too large for one workgroup");
     if (nq == 0) return;
-    if (lds_full <= 80 * 1024)
+    if (lds_full <= 160 * 1024)
         k_probe_select<true><<<nq, PS_T, lds_full, st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
     else
         k_probe_select<false><<<nq, PS_T, lds, st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
@@ -857,7 +857,7 @@ void launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st) {
     FPP_HIP(hipGetLastError());
 }
 
-unsigned persistent_grid(int nitems, size_t lds_bytes) {
+unsigned persistent_grid(int nitems, size_t lds_bytes, int spare_cus) {
     static int cus = 0;
     if (!cus) {
         int dev = 0;
@@ -865,7 +865,8 @@ unsigned persistent_grid(int nitems, size_t lds_bytes) {
         FPP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
     const int per_cu = std::max<int>(1, (int)(160 * 1024 / (lds_bytes + 1024)));
-    return (unsigned)std::max(1, std::min(nitems, cus * per_cu));
+    const int use = std::max(1, cus - std::max(0, spare_cus));
+    return (unsigned)std::max(1, std::min(nitems, use * per_cu));
 }
 
 void launch_split_order(const uint32_t* order, int nq, const uint32_t* qflag, uint32_t* order_u, hipStream_t st) {
