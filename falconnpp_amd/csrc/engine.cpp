@@ -132,6 +132,7 @@ struct Index {
     DevBuf<uint16_t> Xb;
     DevBuf<uint8_t> X8;
     DevBuf<float> s8;
+    float s8_shared = 0.0f;  // FP8 scale shared by every row, 0 when rows keep their own
     int refine_count(int k) const { return refine > 0 ? refine : std::min(256, 4 * k); }
 
     // optional per-stage timing of query calls (HIP events on the call's stream)
@@ -409,7 +410,9 @@ void ensure_lowprec(Index& I, hipStream_t st) {
         I.Xb.reset();
         I.X8.reserve((size_t)N * I.ldl);
         I.s8.reserve(N);
-        launch_quant_fp8(I.Xc.get(), I.ldx, N, I.n_features, I.ldl, I.X8.get(), I.s8.get(), st);
+        I.scratch_word.reserve(2);
+        I.s8_shared = launch_quant_fp8(I.Xc.get(), I.ldx, N, I.n_features, I.ldl, I.X8.get(), I.s8.get(),
+                                       I.scratch_word.get(), st);
     }
     I.lp_ready = I.precision;
 }
@@ -623,6 +626,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         lp.xb = I.Xb.get();
         lp.x8 = I.X8.get();
         lp.scale = I.s8.get();
+        lp.uscale = I.precision == 2 ? I.s8_shared : 0.0f;
     }
     auto gather = [&](const GatherArgs& a, int n, hipStream_t s2) {
         if (lowp)

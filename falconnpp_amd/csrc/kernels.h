@@ -175,12 +175,14 @@ struct LowPrecArgs {
     const uint16_t* xb;      // BF16 copy (bit patterns)
     const uint8_t* x8;       // FP8 copy
     const float* scale;      // FP8 per-row scale
+    float uscale = 0.0f;     // FP8 scale shared by every row (0: per-row scales)
 };
 int lowprec_threads();
 int64_t lowprec_extra_words(int refine);
 void launch_quant_bf16(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint16_t* Xb, hipStream_t st);
-void launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint8_t* X8, float* scale,
-                      hipStream_t st);
+// returns the scale shared by every row, or 0 when rows keep their own
+float launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint8_t* X8, float* scale,
+                       uint32_t* range, hipStream_t st);
 void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, int nq, const uint32_t* probes,
                             const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q,
                             int64_t ldq, const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids,

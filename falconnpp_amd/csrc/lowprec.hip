@@ -2,15 +2,18 @@
 // reference; its FP32 rerank is src/FalconnPP.cpp:627-673).
 //
 // The centred data gets a second copy in BF16, or in FP8-e4m3 with one FP32
-// scale per row.  A query gathers and dedups its candidates exactly as the
-// FP32 path does.  It then scores them on MFMA (v_mfma_f32_16x16x32_bf16 /
-// _fp8_fp8, FP32 accumulation), keeps the best R by approximate score,
-// rescores those R exactly in FP32 from the master copy and returns the top-k.
+// scale per row (one scale shared by all rows when their magnitudes allow).
+// A query gathers and dedups its candidates exactly as the FP32 path does.  It
+// then scores them on MFMA (v_mfma_f32_16x16x32_bf16 / _fp8_fp8, FP32
+// accumulation), keeps the best R by approximate score, rescores those R
+// exactly in FP32 from the master copy and returns the top-k.
 //
-// MFMA use: 16 candidate rows form the A operand (lane l holds row l%16,
-// elements 32t + 8*(l/16) .. +7); the query, converted once, is the B operand
-// broadcast to all 16 columns, so every column of the 16x16 result holds the
-// 16 row scores and lane l reads rows 4*(l/16) + i from its 4 accumulators.
+// MFMA use: the A operand holds 8 candidates as 16 half-rows of a 128-byte
+// chunk, the B operand the two matching query halves in columns 0 and 1, so
+// every load instruction reads whole 128-byte lines (the layout is spelled out
+// at the scoring loop).
+#include <cstring>
+
 #include "common.h"
 #include "gather_common.h"
 #include "kernels.h"
@@ -44,9 +47,33 @@ __global__ void k_quant_bf16(const float* __restrict__ Xc, int64_t ldx, int64_t 
     }
 }
 
-// One wave per row: scale = amax / 240, codes = e4m3(x / scale), zero padded.
+// Largest row magnitude (range[0]) and smallest nonzero one (range[1]) as bit
+// patterns (non-negative floats order as their bits); rows grid-strided over
+// the waves, one atomic pair per wave.
+__global__ __launch_bounds__(256) void k_amax_range(const float* __restrict__ Xc, int64_t ldx, int64_t N, int D,
+                                                    uint32_t* __restrict__ range) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    float hi = 0.0f, lo = INFINITY;
+    for (int64_t n = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; n < N; n += nw) {
+        const float* x = Xc + n * ldx;
+        float amax = 0.0f;
+        for (int d = lane; d < D; d += WAVE) amax = fmaxf(amax, fabsf(x[d]));
+        for (int off = 32; off > 0; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off, WAVE));
+        hi = fmaxf(hi, amax);
+        if (amax > 0.0f) lo = fminf(lo, amax);
+    }
+    if (lane == 0) {
+        atomicMax(range, __float_as_uint(hi));
+        if (lo < INFINITY) atomicMin(range + 1, __float_as_uint(lo));
+    }
+}
+
+// One wave per row: scale = uscale when given, else amax / 240; codes =
+// e4m3(x / scale), zero padded.
 __global__ __launch_bounds__(256) void k_quant_fp8(const float* __restrict__ Xc, int64_t ldx, int64_t N, int D,
-                                                   int64_t ldl, uint8_t* __restrict__ X8, float* __restrict__ scale) {
+                                                   int64_t ldl, float uscale, uint8_t* __restrict__ X8,
+                                                   float* __restrict__ scale) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t n = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (n >= N) return;
@@ -54,7 +81,7 @@ __global__ __launch_bounds__(256) void k_quant_fp8(const float* __restrict__ Xc,
     float amax = 0.0f;
     for (int d = lane; d < D; d += WAVE) amax = fmaxf(amax, fabsf(x[d]));
     for (int off = 32; off > 0; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off, WAVE));
-    const float s = amax > 0.0f ? amax / FP8_TARGET : 1.0f;
+    const float s = uscale > 0.0f ? uscale : (amax > 0.0f ? amax / FP8_TARGET : 1.0f);
     const float inv = 1.0f / s;
     if (lane == 0) scale[n] = s;
     // four codes per lane per step
@@ -71,12 +98,16 @@ __global__ __launch_bounds__(256) void k_quant_fp8(const float* __restrict__ Xc,
 
 enum Fmt { BF16 = 1, FP8 = 2 };
 
-// Tiles of 16 rows per wave step: about 16 KB of row data in flight per wave,
-// at least one tile.
+// 128-byte chunks of a row for kt steps of 32 elements (at least one).
+constexpr int lp_chunks(int fmt, int kt) {
+    const int c = fmt == BF16 ? kt / 2 : kt / 4;
+    return c < 1 ? 1 : c;
+}
+// Tiles of 8 candidates per wave step: about 16 KB of row data in flight per
+// wave (a tile reads 1 KB per chunk), at least one tile.
 constexpr int lp_tiles(int fmt, int kt) {
-    const int row_bytes = kt * 32 * (fmt == BF16 ? 2 : 1);
-    const int t = 16384 / (16 * row_bytes);
-    return t < 1 ? 1 : (t > 8 ? 8 : t);
+    const int t = 16 / lp_chunks(fmt, kt);
+    return t < 1 ? 1 : t;
 }
 
 template <int TB, int FMT, int KT>
@@ -121,8 +152,17 @@ __device__ __forceinline__ void k_gather_score_lp_one(GatherArgs a, LowPrecArgs 
     const int g16 = lane >> 4;  // k-group of this lane in an MFMA operand
     const int r16 = lane & 15;  // row (A) / column (B) of this lane
     const float* qf32 = Q + (int64_t)q * ldq;
-    const int kt = (int)(lp.ldl / 32);
-    // query in the scoring format, as the broadcast B operand
+    // A candidate enters the MFMA as two half-rows: tile row 2c + h holds half
+    // h of candidate c's current 128-byte chunk and column n of the B operand
+    // holds the query's half n of that chunk (columns 2..15 are zero), so
+    // candidate c's score is the sum over chunks of C[2c][0] + C[2c+1][1].
+    // Lane (g16, r16) loads the 16 bytes at 64 h + 16 g16 of the chunk of
+    // candidate r16 / 2 (h = r16 % 2): one load instruction reads the whole
+    // 128-byte lines of 8 candidates.  Bytes past the row end re-read its last
+    // 32 bytes against a zero query (adding exactly 0).
+    constexpr int KC = lp_chunks(FMT, KT);
+    const int64_t rb = FMT == BF16 ? 2 * lp.ldl : lp.ldl;  // row bytes, a multiple of 32
+    // query in the scoring format, as the B operand
     float qscale = 1.0f;
     if constexpr (FMT == FP8) {
         float m = 0.0f;
@@ -135,42 +175,57 @@ __device__ __forceinline__ void k_gather_score_lp_one(GatherArgs a, LowPrecArgs 
         qscale = m > 0.0f ? m / FP8_TARGET : 1.0f;
         __syncthreads();
     }
-    bf16x8 qb[KT];
-    long q8[KT];
+    // B operand: lane (g16, n = r16) holds query elements 8 g16 .. 8 g16 + 7
+    // of half n of each chunk (FP8: two k-steps per chunk half, s = 0, 1)
+    bf16x8 qb[KC];
+    long q8[KC][2];
+    const float inv = 1.0f / qscale;
 #pragma unroll
-    for (int t = 0; t < KT; ++t) {
-        float v[8];
+    for (int ch = 0; ch < KC; ++ch) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int d = 32 * t + 8 * g16 + j;
-            v[j] = (t < kt && d < a.D) ? qf32[d] : 0.0f;
-        }
-        if constexpr (FMT == BF16) {
+        for (int s = 0; s < (FMT == BF16 ? 1 : 2); ++s) {
+            float v[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) qb[t][j] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(v[j]));
-        } else {
-            const float inv = 1.0f / qscale;
-            int lo = 0, hi = 0;
-            lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, lo, false);
-            lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, lo, true);
-            hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4] * inv, v[5] * inv, hi, false);
-            hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6] * inv, v[7] * inv, hi, true);
-            q8[t] = (long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+            for (int j = 0; j < 8; ++j) {
+                const int d = FMT == BF16 ? 64 * ch + 32 * r16 + 8 * g16 + j
+                                          : 128 * ch + 64 * r16 + 16 * g16 + 8 * s + j;
+                v[j] = (r16 < 2 && d < a.D) ? qf32[d] : 0.0f;
+            }
+            if constexpr (FMT == BF16) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) qb[ch][j] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(v[j]));
+            } else {
+                int lo = 0, hi = 0;
+                lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, lo, false);
+                lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, lo, true);
+                hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4] * inv, v[5] * inv, hi, false);
+                hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6] * inv, v[7] * inv, hi, true);
+                q8[ch][s] = (long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+            }
         }
     }
-    // approximate scores, NT tiles of 16 candidates per wave step (the ids of
+    // byte offset of this lane's 16 bytes in each chunk, clamped into the row
+    const int c8 = r16 >> 1;
+    int64_t boff[KC];
+#pragma unroll
+    for (int ch = 0; ch < KC; ++ch) {
+        const int64_t o = 128 * ch + 64 * (r16 & 1) + 16 * g16;
+        boff[ch] = o < rb ? o : rb - 32 + (o & 16);
+    }
+    const uint8_t* xbase = FMT == BF16 ? reinterpret_cast<const uint8_t*>(lp.xb) : lp.x8;
+    // approximate scores, NT tiles of 8 candidates per wave step (the ids of
     // the next step fetched one step ahead), as sort keys by candidate
     // position: in LDS over the dedup bitset when they fit, else in the
     // query's scratch segment
     const uint32_t lds_cap = (uint32_t)Wwords;
     uint32_t* keys = nc <= lds_cap ? bits : reinterpret_cast<uint32_t*>(a.part + cbase);
     constexpr int NT = lp_tiles(FMT, KT);
-    constexpr uint32_t SPAN = 16 * NT;
+    constexpr uint32_t SPAN = 8 * NT;
     const uint32_t stride = (uint32_t)NW * SPAN;
     int32_t idn[NT];
 #pragma unroll
     for (int h = 0; h < NT; ++h) {
-        const uint32_t i = (uint32_t)wv * SPAN + h * 16 + r16;
+        const uint32_t i = (uint32_t)wv * SPAN + h * 8 + c8;
         idn[h] = i < nc ? cq[i] : -1;
     }
     for (uint32_t t0 = (uint32_t)wv * SPAN; t0 < nc; t0 += stride) {
@@ -179,60 +234,53 @@ __device__ __forceinline__ void k_gather_score_lp_one(GatherArgs a, LowPrecArgs 
 #pragma unroll
         for (int h = 0; h < NT; ++h) {
             id[h] = idn[h];
-            const uint32_t i = t0 + stride + h * 16 + r16;
+            const uint32_t i = t0 + stride + h * 8 + c8;
             idn[h] = i < nc ? cq[i] : -1;
-            if constexpr (FMT == FP8) rs[h] = id[h] < 0 ? 0.0f : lp.scale[id[h]];
+            if constexpr (FMT == FP8) rs[h] = lp.uscale > 0.0f ? lp.uscale : (id[h] < 0 ? 0.0f : lp.scale[id[h]]);
         }
         f32x4 acc[NT];
 #pragma unroll
         for (int h = 0; h < NT; ++h) acc[h] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // k-steps past the row end re-read its last step against a zero
-        // query fragment (adding exactly 0): no branches, so all NT * KT
-        // loads of a step are in flight before the first MFMA waits
-        if constexpr (FMT == BF16) {
-            bf16x8 av[KT][NT];
+        // no branches: all KC * NT loads of a step are in flight before the
+        // first MFMA waits
+        int4 av[KC][NT];
 #pragma unroll
-            for (int t = 0; t < KT; ++t) {
-                const int tt = t < kt ? t : kt - 1;
+        for (int ch = 0; ch < KC; ++ch)
 #pragma unroll
-                for (int h = 0; h < NT; ++h) {
-                    const int64_t row = id[h] < 0 ? 0 : id[h];
-                    av[t][h] = *reinterpret_cast<const bf16x8*>(lp.xb + row * lp.ldl + 32 * tt + 8 * g16);
+            for (int h = 0; h < NT; ++h) {
+                const int64_t row = id[h] < 0 ? 0 : id[h];
+                av[ch][h] = *reinterpret_cast<const int4*>(xbase + row * rb + boff[ch]);
+            }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int ch = 0; ch < KC; ++ch)
+#pragma unroll
+            for (int h = 0; h < NT; ++h) {
+                if constexpr (FMT == BF16) {
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[ch][h]), qb[ch],
+                                                                      acc[h], 0, 0, 0);
+                } else {
+                    const long lo = (long)(((uint64_t)(uint32_t)av[ch][h].y << 32) | (uint32_t)av[ch][h].x);
+                    const long hi = (long)(((uint64_t)(uint32_t)av[ch][h].w << 32) | (uint32_t)av[ch][h].z);
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(lo, q8[ch][0], acc[h], 0, 0, 0);
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(hi, q8[ch][1], acc[h], 0, 0, 0);
                 }
             }
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int t = 0; t < KT; ++t)
-#pragma unroll
-                for (int h = 0; h < NT; ++h)
-                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t][h], qb[t], acc[h], 0, 0, 0);
-        } else {
-            long av[KT][NT];
-#pragma unroll
-            for (int t = 0; t < KT; ++t) {
-                const int tt = t < kt ? t : kt - 1;
-#pragma unroll
-                for (int h = 0; h < NT; ++h) {
-                    const int64_t row = id[h] < 0 ? 0 : id[h];
-                    av[t][h] = *reinterpret_cast<const long*>(lp.x8 + row * lp.ldl + 32 * tt + 8 * g16);
-                }
-            }
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int t = 0; t < KT; ++t)
-#pragma unroll
-                for (int h = 0; h < NT; ++h)
-                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av[t][h], q8[t], acc[h], 0, 0, 0);
-        }
-        // accumulator i of lane (g16, r16) holds row 4*g16 + i of the tile
-        // (every column alike); lane (g16, r16 < 4) stores row 4*g16 + r16
-        const int m = 4 * g16 + (r16 & 3);
+        // accumulator i of lane (g16, n) holds C[4 g16 + i][n]: candidate
+        // 2 g16 is acc[0] of column 0 + acc[1] of column 1, candidate
+        // 2 g16 + 1 is acc[2] of column 0 + acc[3] of column 1; lane
+        // (g16, n < 2) stores candidate 2 g16 + n
+        const int c = 2 * g16 + (r16 & 1);
 #pragma unroll
         for (int h = 0; h < NT; ++h) {
-            float sc = r16 == 0 ? acc[h][0] : (r16 == 1 ? acc[h][1] : (r16 == 2 ? acc[h][2] : acc[h][3]));
-            if constexpr (FMT == FP8) sc *= __shfl(rs[h], m, WAVE) * qscale;
-            const uint32_t i = t0 + h * 16 + m;
-            if (r16 < 4 && i < nc) keys[i] = desc_key_any(sc);
+            const float v0 = (r16 & 1) ? acc[h][1] : acc[h][0];
+            const float v1 = (r16 & 1) ? acc[h][3] : acc[h][2];
+            const float s0 = v0 + __shfl_xor(v0, 1, WAVE);
+            const float s1 = v1 + __shfl_xor(v1, 1, WAVE);
+            float sc = (r16 & 1) ? s1 : s0;
+            if constexpr (FMT == FP8) sc *= (lp.uscale > 0.0f ? lp.uscale : __shfl(rs[h], 2 * c, WAVE)) * qscale;
+            const uint32_t i = t0 + h * 8 + c;
+            if (r16 < 2 && i < nc) keys[i] = desc_key_any(sc);
         }
     }
     __syncthreads();
@@ -366,11 +414,28 @@ void launch_quant_bf16(const float* Xc, int64_t ldx, int64_t N, int D, int64_t l
     FPP_HIP(hipGetLastError());
 }
 
-void launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint8_t* X8, float* scale,
-                      hipStream_t st) {
+// When every nonzero row's largest magnitude lies within 2^8 of the largest
+// one, all rows share the scale amax_max / 240: each row's largest code is then
+// at least 240 / 256, inside e4m3's normal range, and scoring needs no
+// per-row scale lookup.  Otherwise each row keeps its own.
+float launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint8_t* X8, float* scale,
+                       uint32_t* range, hipStream_t st) {
     FPP_REQUIRE(ldl % 4 == 0, FPP_EINTERNAL, "fp8 row stride must be a multiple of 4");
-    k_quant_fp8<<<(unsigned)ceil_div(N, 4), 256, 0, st>>>(Xc, ldx, N, D, ldl, X8, scale);
+    launch_fill_u32(range, 0u, 1, st);
+    launch_fill_u32(range + 1, 0xffffffffu, 1, st);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(N, 4), 2048));
+    k_amax_range<<<grid, 256, 0, st>>>(Xc, ldx, N, D, range);
     FPP_HIP(hipGetLastError());
+    uint32_t r[2];
+    FPP_HIP(hipMemcpyAsync(r, range, 8, hipMemcpyDeviceToHost, st));
+    FPP_HIP(hipStreamSynchronize(st));
+    float hi, lo;
+    std::memcpy(&hi, &r[0], 4);
+    std::memcpy(&lo, &r[1], 4);
+    const float uscale = (hi > 0.0f && r[1] != 0xffffffffu && hi <= 256.0f * lo) ? hi / FP8_TARGET : 0.0f;
+    k_quant_fp8<<<(unsigned)ceil_div(N, 4), 256, 0, st>>>(Xc, ldx, N, D, ldl, uscale, X8, scale);
+    FPP_HIP(hipGetLastError());
+    return uscale;
 }
 
 void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, int nq, const uint32_t* probes,

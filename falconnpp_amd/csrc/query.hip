@@ -149,38 +149,111 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     }
     if (tie) atomicOr(&s_tie, 1u);
     __syncthreads();
-    // global top-qProbes of the pool: radix select, the digit found by wave 0
-    uint32_t prefix = 0, pmask = 0, k = (uint32_t)qProbes, eqc = 0;
-    __shared__ uint32_t s_eqc, s_cnt;
-    for (int shift = 24; shift >= 0; shift -= 8) {
-        bhist[threadIdx.x] = 0;
-        __syncthreads();
-        for (int i = threadIdx.x; i < pool_n; i += PS_T) {
-            const uint32_t key = pkey[i];
-            if ((key & pmask) == prefix) atomicAdd(&bhist[(key >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        if (w == 0) {
-            int d;
-            uint32_t below, e;
-            wave_find_digit(bhist, k, d, below, e);
-            if (lane == 0) {
-                s_digit = d;
-                s_below = below;
-                s_eqc = e;
+    __shared__ uint32_t s_eqc, s_cnt, s_over;
+    // global top-qProbes of the pool: radix select, the digit found by wave 0;
+    // the cut key (keys <= it are taken) and whether keys on it must be split
+    auto global_cut = [&](bool& amb) -> uint32_t {
+        uint32_t prefix = 0, pmask = 0, k = (uint32_t)qProbes, eqc = 0;
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            bhist[threadIdx.x] = 0;
+            __syncthreads();
+            for (int i = threadIdx.x; i < pool_n; i += PS_T) {
+                const uint32_t key = pkey[i];
+                if ((key & pmask) == prefix) atomicAdd(&bhist[(key >> shift) & 255u], 1u);
             }
+            __syncthreads();
+            if (w == 0) {
+                int d;
+                uint32_t below, e;
+                wave_find_digit(bhist, k, d, below, e);
+                if (lane == 0) {
+                    s_digit = d;
+                    s_below = below;
+                    s_eqc = e;
+                }
+            }
+            __syncthreads();
+            prefix |= (uint32_t)s_digit << shift;
+            pmask |= 255u << shift;
+            k -= s_below;
+            eqc = s_eqc;
         }
-        __syncthreads();
-        prefix |= (uint32_t)s_digit << shift;
-        pmask |= 255u << shift;
-        k -= s_below;
-        eqc = s_eqc;
+        amb = eqc != k;
+        return prefix;
+    };
+    bool amb;
+    uint32_t T = global_cut(amb);
+    if constexpr (FULLPOOL) {
+        // the reference keeps the top-perTable pairs of each table
+        // (src/FalconnPP.cpp:455-486).  A table holding more than perTable
+        // pairs at or above the global cut gets that cut of its own (its
+        // pairs below it leave the pool) and the global cut is found again.
+        // A trimmed table holds exactly perTable pairs, so every round trims
+        // new tables; after kMaxTrim rounds the query is replayed exactly.
+        constexpr int kMaxTrim = 6;
+        for (int it = 0;; ++it) {
+            for (int l = threadIdx.x; l < g.L; l += PS_T) tcnt[l] = 0;
+            if (threadIdx.x == 0) s_over = 0;
+            __syncthreads();
+            for (int i = threadIdx.x; i < pool_n; i += PS_T)
+                if (pkey[i] <= T) atomicAdd(&tcnt[i / np], 1u);
+            __syncthreads();
+            bool over = false;
+            for (int l = threadIdx.x; l < g.L; l += PS_T) over |= tcnt[l] > (uint32_t)perTable;
+            if (over) atomicOr(&s_over, 1u);
+            __syncthreads();
+            if (!s_over) break;
+            if (it == kMaxTrim) {
+                if (threadIdx.x == 0) atomicOr(&s_tie, 1u);
+                break;
+            }
+            bool ttie = false;
+            for (int l = w; l < g.L; l += PS_WAVES) {
+                if (tcnt[l] <= (uint32_t)perTable) continue;
+                uint32_t* tk = pkey + l * np;
+                uint32_t prefix = 0, pmask = 0, k = (uint32_t)perTable, eqc = 0;
+                for (int shift = 24; shift >= 0; shift -= 8) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) hw[lane * 4 + i] = 0;
+                    __builtin_amdgcn_wave_barrier();
+                    for (int t = lane; t < np; t += WAVE) {
+                        const uint32_t key = tk[t];
+                        if ((key & pmask) == prefix) atomicAdd(&hw[(key >> shift) & 255u], 1u);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    int d;
+                    uint32_t below;
+                    wave_find_digit(hw, k, d, below, eqc);
+                    prefix |= (uint32_t)d << shift;
+                    pmask |= 255u << shift;
+                    k -= below;
+                    __builtin_amdgcn_wave_barrier();
+                }
+                // pairs tied on the table's cut: the reference's heap order
+                // decides which stay, so the query is replayed exactly
+                if (eqc != k) ttie = true;
+                const uint32_t Tl = prefix;
+                uint32_t eq_seen = 0;
+                for (int t0 = 0; t0 < np; t0 += WAVE) {
+                    const int t = t0 + lane;
+                    const bool valid = t < np;
+                    const uint32_t key = valid ? tk[t] : 0xffffffffu;
+                    const bool eqk = valid && key == Tl;
+                    const uint64_t meq = __ballot(eqk);
+                    const bool keep = key < Tl || (eqk && eq_seen + (uint32_t)__popcll(meq & bmask) < k);
+                    if (valid && !keep) tk[t] = 0xffffffffu;
+                    eq_seen += (uint32_t)__popcll(meq);
+                }
+            }
+            if (ttie) atomicOr(&s_tie, 1u);
+            __syncthreads();
+            T = global_cut(amb);
+        }
     }
-    if (eqc != k && threadIdx.x == 0) atomicOr(&s_tie, 2u);
+    if (amb && threadIdx.x == 0) atomicOr(&s_tie, 2u);
     // every key above the cut and every key on it: exactly qProbes when the
     // cut is unambiguous; otherwise the first qProbes appended (the query is
     // flagged and its probes replayed exactly)
-    const uint32_t T = prefix;
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     unsigned long long my_ctot = 0;
@@ -200,17 +273,6 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
             out[pos] = b;
             my_ctot += bucket_off[b + 1] - bucket_off[b];
         }
-        if constexpr (FULLPOOL)
-            if (take) atomicAdd(&tcnt[i / np], 1u);
-    }
-    if constexpr (FULLPOOL) {
-        // the reference keeps the top-perTable pairs of each table; that cut
-        // is irrelevant when no table holds more than perTable pairs at or
-        // above the global cut, else the query is replayed exactly
-        __syncthreads();
-        bool over = false;
-        for (int l = threadIdx.x; l < g.L; l += PS_T) over |= tcnt[l] > (uint32_t)perTable;
-        if (over) atomicOr(&s_tie, 1u);
     }
     atomicAdd(&s_ctot, my_ctot);
     __syncthreads();

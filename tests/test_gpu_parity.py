@@ -323,6 +323,54 @@ def test_low_precision_recall_and_exact_scores(case, mode, min_recall):
         assert np.all(got[j, m:] == 0) and np.all(scores[j, m:] == 0)
 
 
+def test_fp8_rows_keep_their_own_scales_when_magnitudes_spread():
+    # rows scaled by 2^-6 .. 2^6, with their negatives so that the centre is
+    # about zero: the centred rows' largest magnitudes span more than 2^8, so
+    # each row keeps its own FP8 scale (the cases above share one)
+    import torch
+    rng = np.random.default_rng(21)
+    A = synth.gaussian_mixture(6000, 128, 60, seed=13)
+    A = A * np.exp2(rng.integers(-6, 7, len(A))).astype(np.float32)[:, None]
+    X = np.concatenate([A, -A])[rng.permutation(2 * len(A))].astype(np.float32)
+    n, d = X.shape
+    g = E.Index(n, d)
+    g.setIndexParam(8, 128, 10, 0.2, 2, 8, 42)
+    g.build(X)
+    am = np.abs(X - g.center()).max(1)
+    am = am[am > 0]
+    assert am.max() > 256 * am.min()
+    Q = X[rng.choice(n, 50, replace=False)] + np.float32(0.05) * rng.standard_normal((50, d)).astype(np.float32)
+    Q = Q.astype(np.float32)
+    # every candidate refined: equal to the FP32 path
+    g.set_qProbes(6)
+    ref = g.query(Q, 10)
+    assert max(len(g.last_candidates(j)) for j in range(len(Q))) <= 256
+    try:
+        g.set_precision(E.PREC_FP8, 256)
+        got = g.query(Q, 10)
+    finally:
+        g.set_precision(E.PREC_FP32, 0)
+    assert np.array_equal(got, ref)
+    # R = 4k survivors: recall against the FP32 path, exact scores
+    g.set_qProbes(160)
+    ref = g.query(Q, 10)
+    dQ = torch.from_numpy(np.ascontiguousarray(Q)).cuda()
+    out = torch.zeros((len(Q), 10), dtype=torch.int32, device="cuda")
+    sc = torch.zeros((len(Q), 10), dtype=torch.float32, device="cuda")
+    try:
+        g.set_precision(E.PREC_FP8, 40)
+        g.query_device(dQ.data_ptr(), d, len(Q), 10, out.data_ptr(), sc.data_ptr())
+        torch.cuda.synchronize()
+    finally:
+        g.set_precision(E.PREC_FP32, 0)
+    got, scores = out.cpu().numpy(), sc.cpu().numpy()
+    assert E.recall(ref, got, 10) >= 0.9
+    Xc = (X - g.center()).astype(np.float64)
+    for j in range(len(Q)):
+        m = min(10, len(g.last_candidates(j)))
+        np.testing.assert_allclose(scores[j, :m], Xc[got[j, :m]] @ Q[j].astype(np.float64), rtol=1e-5, atol=1e-5)
+
+
 def test_low_precision_rejects_k_above_refine():
     X, Q, g, o = _built("mix_d128_p2")
     g.set_precision(E.PREC_BF16, 8)
