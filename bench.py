@@ -60,6 +60,7 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink N and nq (testing only)")
     ap.add_argument("--qprobes", type=int, default=0, help="override qProbes")
+    ap.add_argument("--threads", type=int, default=0, help="gather workgroup size (0: engine default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the BF16/FP8 + refine runs")
@@ -117,6 +118,8 @@ def main():
     log(f"build {build_s:.3f}s, {sh['n_ids']} ids, hash ties {bst['hash_ties']}, bucket ties {bst['bucket_ties']}")
 
     idx.set_qProbes(cfg["qProbes"])
+    if args.threads:
+        idx.set_tuning(E.TUNE_GATHER_THREADS, args.threads)
     out = torch.zeros((nq, k), dtype=torch.int32, device="cuda")
     scores = torch.zeros((nq, k), dtype=torch.float32, device="cuda")
     idx.set_profiling(1)

@@ -575,8 +575,13 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         ga.ldo = k;
     }
     const bool lowp = I.precision != 0;
-    ga.window_bits = gather_window_bits(I.n_points, lowp ? lowprec_extra_words(I.refine_count(k))
-                                                         : (int64_t)I.gather_threads + 256 + 2048);
+    // 256-thread workgroups run two per CU (dedup window sized for half the
+    // LDS), larger ones one per CU
+    const int gthreads = lowp ? (I.gather_threads <= 256 ? 256 : 512) : I.gather_threads;
+    ga.window_bits = gather_window_bits(I.n_points,
+                                        lowp ? lowprec_extra_words(I.refine_count(k), gthreads)
+                                             : (int64_t)I.gather_threads + 256 + 2048,
+                                        gthreads <= 256 ? 2 : 1);
     const uint32_t* order = nullptr;
     if (I.order_queries && nq > 1 && !I.persistent) {
         const int bits = ilog2_exact(I.NB());
@@ -627,6 +632,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         lp.x8 = I.X8.get();
         lp.scale = I.s8.get();
         lp.uscale = I.precision == 2 ? I.s8_shared : 0.0f;
+        lp.threads = gthreads;
     }
     auto gather = [&](const GatherArgs& a, int n, hipStream_t s2) {
         if (lowp)

@@ -117,7 +117,7 @@ __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ 
                                                    const uint64_t* __restrict__ bucket_off,
                                                    const int32_t* __restrict__ ids, int np, F&& f) {
     constexpr int NW = TB / WAVE;
-    constexpr int UNR = 4;
+    constexpr int UNR = 8;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
     int c0 = wv * WAVE;
@@ -324,7 +324,7 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         for (int i = threadIdx.x; i < words; i += TB) bits[i] = 0u;
         __syncthreads();
         const uint32_t base = woff[w];
-        constexpr int UNR = 4;
+        constexpr int UNR = 8;
         for (uint32_t i0 = 0; i0 < cnt; i0 += TB * UNR) {
             int32_t v[UNR];
 #pragma unroll

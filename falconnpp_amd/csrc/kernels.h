@@ -146,7 +146,7 @@ void launch_split_order(const uint32_t* order, int nq, const uint32_t* qflag, ui
 void launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
 // Workgroups of a persistent launch over nitems: every CU but spare_cus holds
 // as many as fit in its LDS, no more than there are items.
-unsigned persistent_grid(int nitems, size_t lds_bytes, int spare_cus);
+unsigned persistent_grid(int nitems, size_t lds_bytes, int spare_cus, int threads);
 // Largest bucket of a built index (synchronises st); scratch: one word.
 uint32_t max_bucket_size(const uint64_t* off, int64_t nbk, uint32_t* scratch, hipStream_t st);
 // Dedup windows over the ids [0, N) and the LDS words of their counters
@@ -158,7 +158,7 @@ __host__ __device__ inline int64_t window_counter_words(int64_t N, int64_t W) {
 // Dedup window (ids): all of N when its bitset fits in LDS beside
 // extra_words of scratch, else the largest power of two that fits together
 // with the window counters.
-int64_t gather_window_bits(int64_t N, int64_t extra_words);
+int64_t gather_window_bits(int64_t N, int64_t extra_words, int per_cu);
 // Dedup (LDS bitset windows), FP32 rerank and top-k of each query.
 // cand/score: per-query segments at cand_off[q]; ncand[q] = unique count.
 void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,
@@ -176,9 +176,9 @@ struct LowPrecArgs {
     const uint8_t* x8;       // FP8 copy
     const float* scale;      // FP8 per-row scale
     float uscale = 0.0f;     // FP8 scale shared by every row (0: per-row scales)
+    int threads = 512;       // workgroup size, 256 or 512
 };
-int lowprec_threads();
-int64_t lowprec_extra_words(int refine);
+int64_t lowprec_extra_words(int refine, int threads);
 void launch_quant_bf16(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint16_t* Xb, hipStream_t st);
 // returns the scale shared by every row, or 0 when rows keep their own
 float launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint8_t* X8, float* scale,

@@ -335,7 +335,7 @@ __device__ __forceinline__ void k_gather_score_lp_one(GatherArgs a, LowPrecArgs 
 }
 
 template <int TB, int FMT, int KT, bool PERSIST>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB / 256))) void k_gather_score_lp(GatherArgs a, LowPrecArgs lp,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gather_score_lp(GatherArgs a, LowPrecArgs lp,
                                                         const uint32_t* __restrict__ probes,
                                                         const uint64_t* __restrict__ bucket_off,
                                                         const int32_t* __restrict__ ids, const float* __restrict__ Xc,
@@ -364,48 +364,57 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB
     }
 }
 
-constexpr int LP_TB = 512;
-
-template <int FMT, int KT>
+template <int TB, int FMT, int KT>
 void dispatch_lp_launch(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t lds, const uint32_t* probes,
                         const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
                         const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids,
                         float* out_scores, hipStream_t st) {
     if (a.queue)
-        k_gather_score_lp<LP_TB, FMT, KT, true><<<persistent_grid(nq, lds, a.spare_cus), LP_TB, lds, st>>>(
+        k_gather_score_lp<TB, FMT, KT, true><<<persistent_grid(nq, lds, a.spare_cus, TB), TB, lds, st>>>(
             a, lp, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids, out_scores);
     else
-        k_gather_score_lp<LP_TB, FMT, KT, false><<<nq, LP_TB, lds, st>>>(a, lp, probes, bucket_off, ids, Xc, Q, ldq,
-                                                                         cand_off, cand, ncand, out_ids, out_scores);
+        k_gather_score_lp<TB, FMT, KT, false><<<nq, TB, lds, st>>>(a, lp, probes, bucket_off, ids, Xc, Q, ldq,
+                                                                      cand_off, cand, ncand, out_ids, out_scores);
 }
 
-template <int FMT>
+template <int TB, int FMT>
 void dispatch_lp_kt(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t lds, const uint32_t* probes,
                     const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
                     const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids, float* out_scores,
                     hipStream_t st) {
     const int64_t kt = lp.ldl / 32;
+#define FPP_LP(KT)                                                                                                  \
+    dispatch_lp_launch<TB, FMT, KT>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, \
+                                    out_ids, out_scores, st)
     if (kt <= 4)
-        dispatch_lp_launch<FMT, 4>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
-                                out_scores, st);
+        FPP_LP(4);
     else if (kt <= 8)
-        dispatch_lp_launch<FMT, 8>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
-                                out_scores, st);
+        FPP_LP(8);
     else if (kt <= 16)
-        dispatch_lp_launch<FMT, 16>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+        FPP_LP(16);
+    else
+        FPP_LP(32);
+#undef FPP_LP
+}
+
+template <int TB>
+void dispatch_lp_fmt(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t lds, const uint32_t* probes,
+                     const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
+                     const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids, float* out_scores,
+                     hipStream_t st) {
+    if (lp.format == BF16)
+        dispatch_lp_kt<TB, BF16>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
                                  out_scores, st);
     else
-        dispatch_lp_launch<FMT, 32>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
-                                 out_scores, st);
+        dispatch_lp_kt<TB, FP8>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+                                out_scores, st);
 }
 
 }  // namespace
 
-int lowprec_threads() { return LP_TB; }
-
 // LDS words beyond the bitset and window counters: scan scratch, reduction,
 // histogram, survivors (exact keys and ids), append counter.
-int64_t lowprec_extra_words(int refine) { return LP_TB + 64 + 256 + 3 * (int64_t)refine + 2; }
+int64_t lowprec_extra_words(int refine, int threads) { return threads + 64 + 256 + 3 * (int64_t)refine + 2; }
 
 void launch_quant_bf16(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint16_t* Xb, hipStream_t st) {
     const int64_t total = N * ldl;
@@ -448,16 +457,17 @@ void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, int nq, 
     FPP_REQUIRE(lp.ldl % 32 == 0 && lp.ldl / 32 <= 32, FPP_EINVAL, "n_features must be <= 1024");
     if (nq == 0) return;
     const int64_t wwords = (a.window_bits / 32 + 1) & ~1;
-    const size_t lds =
-        (size_t)(wwords + window_counter_words(a.N, a.window_bits) + lowprec_extra_words(lp.refine)) * 4;
+    FPP_REQUIRE(lp.threads == 256 || lp.threads == 512, FPP_EINVAL, "low-precision workgroups are 256 or 512 threads");
+    const size_t lds = (size_t)(wwords + window_counter_words(a.N, a.window_bits) +
+                                lowprec_extra_words(lp.refine, lp.threads)) * 4;
     FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "LDS budget exceeded (refine count or qProbes too large)");
     if (a.queue) launch_fill_u32(a.queue, 0u, 1, st);
-    if (lp.format == BF16)
-        dispatch_lp_kt<BF16>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+    if (lp.threads == 256)
+        dispatch_lp_fmt<256>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
                              out_scores, st);
     else
-        dispatch_lp_kt<FP8>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
-                            out_scores, st);
+        dispatch_lp_fmt<512>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
+                             out_scores, st);
     FPP_HIP(hipGetLastError());
 }
 

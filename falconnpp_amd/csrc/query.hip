@@ -765,7 +765,7 @@ __device__ __forceinline__ void k_gather_score_f32_one(GatherArgs a, const uint3
 }
 
 template <int TB, int CPL, bool PERSIST>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB / 256, TB / 256))) void k_gather_score_f32(GatherArgs a, const uint32_t* __restrict__ probes,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB <= 512 ? 2 : TB / 256, TB <= 512 ? 2 : TB / 256))) void k_gather_score_f32(GatherArgs a, const uint32_t* __restrict__ probes,
                                                           const uint64_t* __restrict__ bucket_off,
                                                           const int32_t* __restrict__ ids,
                                                           const float* __restrict__ Xc, const float* __restrict__ Q,
@@ -869,7 +869,7 @@ void dispatch_gather_cpl(const GatherArgs& a, int nq, const uint32_t* probes, co
     const size_t lds = gather_lds_bytes(a, TB);
 #define FPP_G(C) \
     if (a.queue)                                                                                                    \
-        k_gather_score_f32<TB, C, true><<<persistent_grid(nq, lds, a.spare_cus), TB, lds, st>>>(                                   \
+        k_gather_score_f32<TB, C, true><<<persistent_grid(nq, lds, a.spare_cus, TB), TB, lds, st>>>(                                   \
             a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids, out_scores);               \
     else                                                                                                            \
         k_gather_score_f32<TB, C, false><<<nq, TB, lds, st>>>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, \
@@ -919,14 +919,15 @@ void launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st) {
     FPP_HIP(hipGetLastError());
 }
 
-unsigned persistent_grid(int nitems, size_t lds_bytes, int spare_cus) {
+unsigned persistent_grid(int nitems, size_t lds_bytes, int spare_cus, int threads) {
     static int cus = 0;
     if (!cus) {
         int dev = 0;
         FPP_HIP(hipGetDevice(&dev));
         FPP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
-    const int per_cu = std::max<int>(1, (int)(160 * 1024 / (lds_bytes + 1024)));
+    // resident workgroups per CU: LDS, and two waves per SIMD
+    const int per_cu = std::max<int>(1, std::min<int>(threads <= 256 ? 2 : 1, (int)(160 * 1024 / (lds_bytes + 1024))));
     const int use = std::max(1, cus - std::max(0, spare_cus));
     return (unsigned)std::max(1, std::min(nitems, use * per_cu));
 }
@@ -968,9 +969,9 @@ void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, 
 }
 
 // extra_words: LDS words the kernel needs beyond the bitset and window counters.
-int64_t gather_window_bits(int64_t N, int64_t extra_words) {
+int64_t gather_window_bits(int64_t N, int64_t extra_words, int per_cu) {
     const int64_t fixed = (extra_words + 64) * 4;  // + static LDS margin
-    const int64_t budget = 160 * 1024 - fixed;
+    const int64_t budget = 160 * 1024 / std::max(1, per_cu) - 1024 - fixed;
     const int64_t single = (N + 31) / 32 * 32;
     if (single / 8 + 8 <= budget) return single;
     for (int64_t w = int64_t(1) << 20; w >= (int64_t(1) << 16); w >>= 1)
