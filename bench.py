@@ -275,40 +275,53 @@ def serve_batched(args, E, synth, np, torch):
     def call():
         idx.query_batch_device(dQ.data_ptr(), d, nq, qps_q, ks, kmax, out.data_ptr(), sc.data_ptr())
 
-    for _ in range(args.warmup):
+    def timed_run():
+        """Warm-up, the timed steps, then one more call with per-query
+        stamps; returns (QPS, ms per step, stage ms, latency)."""
+        for _ in range(max(1, args.warmup)):
+            call()
+        torch.cuda.synchronize()
+        stage = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            call()
+            stage.append(idx.timings())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        stages = {key: round(float(np.mean([s[key] for s in stage])), 3)
+                  for key in ("hash", "probe", "replay", "gather")}
+        # per-query latency, from the gather's per-query stamps
+        idx.set_profiling(2)
         call()
-    torch.cuda.synchronize()
-    stage = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        call()
-        stage.append(idx.timings())
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    qps = nq * args.steps / dt
+        service, done = idx.query_times(nq)
+        idx.set_profiling(1)
+        pre_ms = stages["hash"] + stages["probe"]
+        lat = {"service_us_p50": round(float(np.percentile(service, 50)), 1),
+               "service_us_p99": round(float(np.percentile(service, 99)), 1),
+               "completion_ms_p50": round(pre_ms + float(np.percentile(done, 50)) / 1e3, 3),
+               "completion_ms_p99": round(pre_ms + float(np.percentile(done, 99)) / 1e3, 3)}
+        return nq * args.steps / dt, dt / args.steps * 1e3, stages, lat
+
+    qps, ms_step, stages, lat = timed_run()
     st = idx.stats()
-    log(f"batch {dt / args.steps * 1e3:.1f} ms, {qps:.0f} QPS, C_tot/q {st['ctot'] / nq:.0f}, "
+    log(f"batch {ms_step:.1f} ms, {qps:.0f} QPS, C_tot/q {st['ctot'] / nq:.0f}, "
         f"C_uniq/q {st['cuniq'] / nq:.0f}, tie-replayed queries {st['query_ties']}")
-    stages = {key: round(float(np.mean([s[key] for s in stage])), 3) for key in ("hash", "probe", "replay", "gather")}
-    # per-query latency, from the gather's per-query stamps (one more call)
-    idx.set_profiling(2)
-    call()
-    service, done = idx.query_times(nq)
-    idx.set_profiling(1)
-    pre_ms = stages["hash"] + stages["probe"]
-    lat = {"service_us_p50": round(float(np.percentile(service, 50)), 1),
-           "service_us_p99": round(float(np.percentile(service, 99)), 1),
-           "completion_ms_p50": round(pre_ms + float(np.percentile(done, 50)) / 1e3, 3),
-           "completion_ms_p99": round(pre_ms + float(np.percentile(done, 99)) / 1e3, 3)}
     log(f"latency: {lat}")
     res = {}
+    truth_by_k = {}
+
+    def recall_by_k(ans):
+        by_k = {}
+        for kv, (sel, truth) in truth_by_k.items():
+            by_k[kv] = round(float(E.recall(truth, ans[sel, :kv], kv)), 4)
+        return by_k
+
     if not args.no_recall:
         # recall@k_q against FP64 brute force, per k on a sample of at most
         # 2000 queries each; the brute force is seeded with the answer's
         # first k ids of those queries (all valid rows)
         ans = out.cpu().numpy()
         t = time.perf_counter()
-        by_k = {}
         for kv in cfg["k_choices"]:
             sel = np.where(ks == kv)[0][:2000]
             if len(sel) == 0:
@@ -316,9 +329,23 @@ def serve_batched(args, E, synth, np, torch):
             dQs = torch.from_numpy(np.ascontiguousarray(Q[sel])).cuda()
             truth, _ = E.bruteforce_f64_device(dX.data_ptr(), d, n, d, dQs.data_ptr(), d, len(sel), int(kv),
                                                np.ascontiguousarray(ans[sel, :kv]))
-            by_k[int(kv)] = round(float(E.recall(truth, ans[sel, :kv], int(kv))), 4)
-        res["recall_at_k_by_k"] = by_k
-        log(f"recall@k by k = {by_k} (FP64 brute force {time.perf_counter() - t:.1f}s)")
+            truth_by_k[int(kv)] = (sel, truth)
+        res["recall_at_k_by_k"] = recall_by_k(ans)
+        log(f"recall@k by k = {res['recall_at_k_by_k']} (FP64 brute force {time.perf_counter() - t:.1f}s)")
+    variants = {}
+    if not args.no_variants:
+        # the same batch scored in BF16 / FP8 on MFMA, min(256, 4 k_q)
+        # survivors per query rescored exactly in FP32
+        for name, mode in (("bf16_refine", E.PREC_BF16), ("fp8_refine", E.PREC_FP8)):
+            idx.set_precision(mode, 0)
+            vq, vms, vst, vlat = timed_run()
+            v = {"value": round(vq, 1), "ms_per_step": round(vms, 3), "refine": "min(256, 4 k_q)",
+                 "speedup_vs_fp32": round(vq / qps, 3), "stage_ms": vst, "latency": vlat}
+            if truth_by_k:
+                v["recall_at_k_by_k"] = recall_by_k(out.cpu().numpy())
+            variants[name] = v
+            log(f"{name}: {v}")
+        idx.set_precision(E.PREC_FP32, 0)
     line = {
         "metric": "QPS of one batched call, per-query qProbes U[100,8000] and k in {1,10,100}; "
                   "p50/p99 per-query latency",
@@ -327,7 +354,7 @@ def serve_batched(args, E, synth, np, torch):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "ms_per_step": round(ms_step, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -344,6 +371,7 @@ def serve_batched(args, E, synth, np, torch):
         "autotune": tuned,
         "stage_ms": stages,
         "candidates_per_query": {"c_tot": round(st["ctot"] / nq, 1), "c_uniq": round(st["cuniq"] / nq, 1)},
+        "variants": variants,
     }
     print(json.dumps(line), flush=True)
 
