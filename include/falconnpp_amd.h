@@ -102,7 +102,10 @@ int fpp_set_profiling(fpp_index* h, int on);
 int fpp_get_timings(fpp_index* h, double* ms);
 /* Scoring precision (no reference counterpart; the reference scores in FP32):
  * mode 0 = FP32 (default, reference results), 1 = BF16 copy scored on MFMA,
- * 2 = FP8-e4m3 copy with a per-row FP32 scale scored on MFMA.  Modes 1-2 keep
+ * 2 = FP8-e4m3 copy with an FP32 scale per row (one scale for all rows when
+ * their largest magnitudes lie within 2^8 of each other) scored on MFMA.  With
+ * per-query k (fpp_query_batch_device) and refine 0, query q keeps
+ * min(256, 4 * k_q).  Modes 1-2 keep
  * the `refine` best approximate candidates (0: 4 * n_neighbors, at most 256)
  * and rescore them exactly in FP32, so returned scores are FP32 scores; the
  * ids can differ from mode 0 when a true top-k row falls outside the refine
@@ -110,7 +113,8 @@ int fpp_get_timings(fpp_index* h, double* ms);
 int fpp_set_precision(fpp_index* h, int mode, int refine);
 /* launch-shape knobs (no reference counterpart): FPP_TUNE_QUERY_ORDER 0/1
  * (process queries grouped by their best table-0 bucket, XCD-aware),
- * FPP_TUNE_GATHER_THREADS 256/512/1024 (workgroup of the gather kernel),
+ * FPP_TUNE_GATHER_THREADS 256/512/1024 (workgroup of the gather kernel; 256
+ * runs two workgroups per CU, the reduced-precision kernels use 256 or 512),
  * FPP_TUNE_PERSISTENT 0/1 (one persistent gather launch whose workgroups take
  * queries from a work queue, longest first), FPP_TUNE_GRAPH 0/1 (hashing and
  * probe selection of a batch captured once in a hipGraph and replayed while
