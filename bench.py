@@ -105,6 +105,7 @@ def main():
     dX = torch.from_numpy(X).cuda()
     dQ = torch.from_numpy(Q).cuda()
     torch.cuda.synchronize()
+    free0, total_mem = torch.cuda.mem_get_info()
 
     idx = E.Index(n, d)
     idx.setIndexParam(cfg["L"], cfg["n_proj"], cfg["bucketLimit"], cfg["alpha"], cfg["iProbes"], 16,
@@ -113,6 +114,7 @@ def main():
     idx.build_device(dX.data_ptr(), d)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
+    free_built = torch.cuda.mem_get_info()[0]
     sh = idx.shape()
     bst = idx.stats()
     log(f"build {build_s:.3f}s, {sh['n_ids']} ids, hash ties {bst['hash_ties']}, bucket ties {bst['bucket_ties']}")
@@ -162,16 +164,21 @@ def main():
     res = {}
     ans32 = out.cpu().numpy()
     truth = None
+    # recall against the FP64 brute force over the first nrec queries (all of
+    # them up to 10,000)
+    nrec = min(nq, 10_000)
     if not args.no_recall:
         t = time.perf_counter()
-        truth, _ = E.bruteforce_f64_device(dX.data_ptr(), d, n, d, dQ.data_ptr(), d, nq, k, ans32)
-        res["recall_at_k"] = round(E.recall(truth, ans32, k), 4)
-        log(f"recall@{k} = {res['recall_at_k']} (FP64 brute force {time.perf_counter() - t:.1f}s)")
+        truth, _ = E.bruteforce_f64_device(dX.data_ptr(), d, n, d, dQ.data_ptr(), d, nrec, k, ans32[:nrec])
+        res["recall_at_k"] = round(E.recall(truth, ans32[:nrec], k), 4)
+        if nrec < nq:
+            res["recall_queries"] = nrec
+        log(f"recall@{k} = {res['recall_at_k']} over {nrec} queries (FP64 brute force {time.perf_counter() - t:.1f}s)")
 
     variants = {}
     if not args.no_variants:
         for name, mode in (("bf16_refine", E.PREC_BF16), ("fp8_refine", E.PREC_FP8)):
-            refine = 4 * k
+            refine = int(cfg.get("refine", 4 * k))
             idx.set_precision(mode, refine)
             for _ in range(max(1, args.warmup)):
                 idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
@@ -195,11 +202,21 @@ def main():
             v["phases_us"] = {key: round(float(vph[key]), 2)
                               for key in ("ph_gather_us", "ph_compact_us", "ph_score_us", "ph_topk_us")}
             if truth is not None:
-                v["recall_at_k"] = round(E.recall(truth, out.cpu().numpy(), k), 4)
+                v["recall_at_k"] = round(E.recall(truth, out.cpu().numpy()[:nrec], k), 4)
                 v["score_rel_err"] = approx_score_error(X, Q, idx.center(), truth, mode)
             variants[name] = v
             log(f"{name}: {v}")
         idx.set_precision(E.PREC_FP32, 0)
+    # device memory: the resident data and queries (torch), the index (centred
+    # copy, sign bits, CSR tables), then the query workspace and the BF16/FP8
+    # copies made by the calls above
+    free_end = torch.cuda.mem_get_info()[0]
+    memory = {"device_total_gb": round(total_mem / 1e9, 1),
+              "data_and_queries_gb": round((X.nbytes + Q.nbytes) / 1e9, 2),
+              "index_gb": round((free0 - free_built) / 1e9, 2),
+              "query_workspace_and_copies_gb": round((free_built - free_end) / 1e9, 2),
+              "used_at_end_gb": round((total_mem - free_end) / 1e9, 2)}
+    log(f"memory: {memory}")
 
     cpu = None
     if not args.no_cpu:
@@ -231,6 +248,7 @@ def main():
         **res,
         "stage_ms": stages,
         "candidates_per_query": {"c_tot": round(st["ctot"] / nq, 1), "c_uniq": round(st["cuniq"] / nq, 1)},
+        "memory": memory,
         "roofline": roofline,
         "variants": variants,
         "cpu_baseline": cpu,
@@ -397,6 +415,20 @@ def approx_score_error(X, Q, center, truth, mode, sample=200):
     against exact FP64 scores q.(x - c), on a sample of queries (host
     emulation of the engine's rounding)."""
     import numpy as np
+    shared = 0.0
+    if mode == 2:
+        # the engine's FP8 scale rule: one scale amax_max / 240 for all rows
+        # when every nonzero row's largest magnitude is within 2^8 of the
+        # largest, else amax_row / 240 per row
+        hi, lo = 0.0, np.inf
+        for s0 in range(0, len(X), 1 << 20):
+            am = np.abs(X[s0:s0 + (1 << 20)] - center).max(1)
+            hi = max(hi, float(am.max()))
+            nz = am[am > 0]
+            if len(nz):
+                lo = min(lo, float(nz.min()))
+        if hi > 0 and hi <= 256.0 * lo:
+            shared = hi / 240.0
     errs = []
     for j in range(min(sample, len(Q))):
         rows = (X[truth[j]] - center).astype(np.float32)
@@ -405,7 +437,7 @@ def approx_score_error(X, Q, center, truth, mode, sample=200):
         if mode == 1:
             approx = _bf16(rows).astype(np.float64) @ _bf16(q).astype(np.float64)
         else:
-            sr = np.maximum(np.abs(rows).max(1), 1e-30) / 240.0
+            sr = np.full(len(rows), shared) if shared > 0 else np.maximum(np.abs(rows).max(1), 1e-30) / 240.0
             sq = max(float(np.abs(q).max()), 1e-30) / 240.0
             approx = (_e4m3(rows / sr[:, None]) @ _e4m3(q / sq)) * sr * sq
         errs.append(np.abs(approx - exact) / np.maximum(np.abs(exact), 1e-12))

@@ -400,7 +400,9 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
 void ensure_lowprec(Index& I, hipStream_t st) {
     if (I.lp_ready == I.precision) return;
     const int64_t N = I.n_points;
-    I.ldl = round_up(I.n_features, 32);
+    // rows padded to whole 128-byte lines: a scoring load instruction reads
+    // 8 rows' lines whole (FP8: 128 elements, BF16: 64 per line)
+    I.ldl = round_up(I.n_features, I.precision == 2 ? 128 : 64);
     if (I.precision == 1) {
         I.X8.reset();
         I.s8.reset();

@@ -171,7 +171,7 @@ struct LowPrecArgs {
     int format;              // 1 = BF16, 2 = FP8-e4m3 with per-row scale
     int refine;              // survivors rescored in FP32 (R), 1..256; with per-query k the largest
     int refine_per_k = 0;    // 1: query q keeps min(refine, 4 * k_q) survivors
-    int64_t ldl;             // row stride of the low-precision copy (elements, multiple of 32)
+    int64_t ldl;             // row stride of the low-precision copy (elements; 128 bytes a multiple)
     const uint16_t* xb;      // BF16 copy (bit patterns)
     const uint8_t* x8;       // FP8 copy
     const float* scale;      // FP8 per-row scale

@@ -72,6 +72,11 @@ CONFIGS = {
     # Precision variant: N=1e7 x d=128, 5000 centres.
     "c3": dict(n=10_000_000, d=128, kind="mixture", n_centres=5000, nq=10_000, L=64, n_proj=128, iProbes=2,
                alpha=0.01, bucketLimit=20, qProbes=2000, k=20),
+    # Largest single-GPU case: N=5e7 x d=96 (fhtDim 128), 20000 centres,
+    # 6.4e9 (id, table) entries before the bucket filter; FP8 scoring keeps
+    # 200 survivors for the FP32 refine.
+    "c5": dict(n=50_000_000, d=96, kind="mixture", n_centres=20_000, nq=100_000, L=64, n_proj=128, iProbes=2,
+               alpha=0.01, bucketLimit=20, qProbes=4000, k=20, refine=200),
 }
 
 
