@@ -431,10 +431,16 @@ QueryPlan plan_for(const Index& I, int qProbes) {
     return pl;
 }
 
-void check_plan(const Index& I, const QueryPlan& pl) {
+// Probes per layer that the fast hashing and probe-selection kernels handle
+// (top lists of at most 32 entries, PL^2 pair sums in registers).  A plain
+// query call with more replays every query exactly instead.
+constexpr int FAST_MAX_PL = 31;
+
+void check_plan(const Index& I, const QueryPlan& pl, bool batched) {
     FPP_REQUIRE(pl.perLayer <= I.n_proj, FPP_EINVAL,
                 "qProbes too large: ceil(sqrt(ceil(2*qProbes/n_tables))) exceeds n_proj (the reference pops an empty heap)");
-    FPP_REQUIRE(pl.perLayer <= 31, FPP_EINVAL, "qProbes too large: more than 31 probes per layer");
+    FPP_REQUIRE(!batched || pl.perLayer <= FAST_MAX_PL, FPP_EINVAL,
+                "qProbes too large for a batched call: more than 31 probes per layer");
 }
 
 // One batch of queries.  k is the number of neighbours, or with per-query
@@ -462,7 +468,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
             FPP_REQUIRE(h_qprobes[q] >= 1, FPP_EINVAL, "qProbes must be >= 1");
             FPP_REQUIRE(h_k[q] >= 1 && h_k[q] <= k, FPP_EINVAL, "per-query n_neighbors must be in [1, kmax]");
             const QueryPlan p = plan_for(I, h_qprobes[q]);
-            check_plan(I, p);
+            check_plan(I, p, true);
             I.hqp[q] = QueryParams{h_qprobes[q], p.perTable, p.perLayer, h_k[q], nprobes};
             nprobes += (uint64_t)h_qprobes[q];
             pl.perTable = std::max(pl.perTable, p.perTable);
@@ -472,9 +478,13 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     } else {
         FPP_REQUIRE(I.qProbes >= 1, FPP_EINVAL, "qProbes must be >= 1");
         pl = plan_for(I, I.qProbes);
-        check_plan(I, pl);
+        check_plan(I, pl, false);
         nprobes = (uint64_t)nq * I.qProbes;
     }
+    // more probes per layer than the fast kernels take: every query's probes
+    // come from the exact replay (its per-table lists are rebuilt from the
+    // rotated query, qflag bit 0), on the replay stream
+    const bool exact_only = pl.perLayer > FAST_MAX_PL;
     I.stat_ctot = I.stat_cuniq = I.stat_qties = 0;
     I.last_nq = nq;
     if (nq == 0) return;
@@ -500,15 +510,17 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         launch_fill_u32(reinterpret_cast<uint32_t*>(I.Qpad.get()), 0u, (int)((size_t)nq * I.ldx), st);
         FPP_HIP(hipMemcpy2DAsync(I.Qpad.get(), I.ldx * 4, dQ, ldq * 4, (size_t)I.n_features * 4, nq,
                                  hipMemcpyDeviceToDevice, st));
-        launch_fill_u32(I.qflag.get(), 0u, nq, st);
+        launch_fill_u32(I.qflag.get(), exact_only ? 3u : 0u, nq, st);
         if (marks) I.mark(0, st);
-        launch_hash_query(hg, I.Qpad.get(), I.ldx, nq, PL, I.masks.get(), I.qc.get(), I.qv.get(), I.qflag.get(),
-                          st);
+        if (!exact_only)
+            launch_hash_query(hg, I.Qpad.get(), I.ldx, nq, PL, I.masks.get(), I.qc.get(), I.qv.get(),
+                              I.qflag.get(), st);
         if (marks) I.mark(1, st);
-        launch_probe_select(pg, nq, I.qc.get(), I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(),
-                            I.qflag.get(), st);
+        if (!exact_only)
+            launch_probe_select(pg, nq, I.qc.get(), I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(),
+                                I.qflag.get(), st);
     };
-    if (I.use_graph) {
+    if (I.use_graph && !exact_only) {
         const std::vector<uint64_t> key = {
             (uint64_t)nq, (uint64_t)ldq, (uint64_t)(uintptr_t)dQ, (uint64_t)(uintptr_t)st, I.build_gen,
             (uint64_t)(uintptr_t)I.Qpad.get(), (uint64_t)(uintptr_t)I.qc.get(), (uint64_t)(uintptr_t)I.qv.get(),
@@ -655,12 +667,16 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         FPP_HIP(hipEventRecord(I.ev_fork, st));
         FPP_HIP(hipStreamWaitEvent(I.side, I.ev_fork, 0));
         const int64_t wpq = probe_exact_work_bytes(hg, pg);
+        // the per-table lists of at most 1 GiB of queries at a time
+        const uint32_t slice = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(nl, (int64_t(1) << 30) / wpq));
         I.qlist.reserve(nl);
-        I.qwork.reserve((size_t)wpq * nl);
+        I.qwork.reserve((size_t)wpq * slice);
         FPP_HIP(hipMemcpyAsync(I.qlist.get(), lst.data(), (size_t)nl * 4, hipMemcpyHostToDevice, I.side));
         I.mark(3, I.side);
-        launch_probe_exact(hg, pg, I.Qpad.get(), I.ldx, I.masks.get(), I.qlist.get(), nl, I.qflag.get(), I.qc.get(),
-                           I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(), I.qwork.get(), wpq, I.side);
+        for (uint32_t s0 = 0; s0 < nl; s0 += slice)
+            launch_probe_exact(hg, pg, I.Qpad.get(), I.ldx, I.masks.get(), I.qlist.get() + s0,
+                               std::min(slice, nl - s0), I.qflag.get(), I.qc.get(), I.qv.get(), I.bucket_off.get(),
+                               I.probes.get(), I.ctot.get(), I.qwork.get(), wpq, I.side);
         I.mark(4, I.side);
         GatherArgs gf = ga;
         gf.order = I.qlist.get();

@@ -503,7 +503,8 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const u
                                                             const uint64_t* __restrict__ bucket_off,
                                                             uint32_t* __restrict__ probes, uint64_t* __restrict__ ctot,
                                                             const uint32_t* __restrict__ qflag,
-                                                            const unsigned char* __restrict__ work, int64_t wpq) {
+                                                            const unsigned char* __restrict__ work, int64_t wpq,
+                                                            int staged) {
     extern __shared__ IFPair mlds[];
     const uint32_t f = blockIdx.x;
     const uint32_t q = qlist[f];
@@ -511,10 +512,12 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const u
     const QueryView qv_ = query_view(g.qp, (int)q, g.qProbes, g.perTable, g.PL, 0);
     const int L = g.L, PT = qv_.perTable, QP = qv_.qProbes;
     const IFPair* list_g = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
-    IFPair* list = mlds;
-    IFPair* mq = list + (int64_t)L * PT;
+    // the per-table lists are staged in LDS when they fit, else read in place
+    IFPair* mq = staged ? mlds + (int64_t)L * PT : mlds;
     int* track = reinterpret_cast<int*>(mq + L);
-    for (int64_t i = threadIdx.x; i < (int64_t)L * PT; i += MG_T) list[i] = list_g[i];
+    if (staged)
+        for (int64_t i = threadIdx.x; i < (int64_t)L * PT; i += MG_T) mlds[i] = list_g[i];
+    const IFPair* list = staged ? mlds : list_g;
     for (int l = threadIdx.x; l < L; l += MG_T) track[l] = 0;
     __syncthreads();
     uint32_t* out = probes + qv_.probe_off;
@@ -962,9 +965,12 @@ void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, 
     k_probe_exact_select<<<nlist, MG_T, 0, st>>>(g, qlist, bucket_off, probes, ctot, qflag, w,
                                                    work_per_query);
     FPP_HIP(hipGetLastError());
-    const size_t lds = merge_lds_bytes(g);
-    FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "probe lists (n_tables * perTable) too large for the exact merge");
-    k_probe_exact_merge<<<nlist, MG_T, lds, st>>>(g, qlist, bucket_off, probes, ctot, qflag, w, work_per_query);
+    const size_t full = merge_lds_bytes(g);
+    const bool staged = full <= 160 * 1024;
+    const size_t lds = staged ? full : (size_t)g.L * (sizeof(IFPair) + sizeof(int));
+    FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "n_tables too large for the exact probe merge");
+    k_probe_exact_merge<<<nlist, MG_T, lds, st>>>(g, qlist, bucket_off, probes, ctot, qflag, w, work_per_query,
+                                                  staged ? 1 : 0);
     FPP_HIP(hipGetLastError());
 }
 

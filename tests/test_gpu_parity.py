@@ -40,6 +40,12 @@ CASES = {
     "int_ties_l200": (lambda: _sym_int_grid(2000, 16, -2, 2, 17), 200, 16, 5, 0.3, 2, 300, 10, 20),
     # N above the LDS bitset window (~1.2M ids): the dedup runs over two windows
     "multiwindow_n1.3m": (lambda: synth.gaussian_mixture(1_300_000, 16, 300, seed=16), 2, 16, 30, 0.05, 1, 8, 20, 12),
+    # more than 31 probes per layer: every query's probes come from the exact
+    # replay (32 per layer; the merge stages its lists in LDS) ...
+    "sphere_pl32_exact": (lambda: synth.lognormal_sphere(8000, 50, 0.3, seed=19), 10, 128, 20, 0.01, 1, 5000, 20, 30),
+    # ... and with ties everywhere (40 per layer; 20 x 1600 lists of 256 KB,
+    # read by the merge in place)
+    "int_grid_pl40_exact": (lambda: _sym_int_grid(2000, 24, -2, 2, 18), 20, 64, 5, 0.3, 2, 16000, 10, 20),
 }
 
 _cache = {}
@@ -60,7 +66,7 @@ def _built(name):
     o.build(X, g.center())
     rng = np.random.default_rng(7)
     Q = X[rng.choice(n, nq, replace=False)] + np.float32(0.05) * rng.standard_normal((nq, d)).astype(np.float32)
-    if name == "int_grid_ties":
+    if name.startswith("int_grid"):
         Q = np.round(Q)
     _cache[name] = (X, Q.astype(np.float32), g, o)
     return _cache[name]
