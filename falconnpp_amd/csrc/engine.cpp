@@ -16,6 +16,7 @@
 #include "../../include/falconnpp_amd.h"
 #include "common.h"
 #include "kernels.h"
+#include "textio.h"
 
 namespace fpp {
 namespace {
@@ -927,6 +928,17 @@ int fpp_query(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t*
         FPP_REQUIRE(nq >= 0, FPP_EINVAL, "number of queries must be >= 0");
         hipStream_t st = pick_stream(I, nullptr);
         auto t0 = std::chrono::high_resolution_clock::now();
+        // a verbose call times its stages and phases like the reference's
+        const bool prof = I.profiling, stamps = I.phase_stamps;
+        if (verbose) I.profiling = I.phase_stamps = true;
+        struct Restore {
+            Index& I;
+            bool p, s;
+            ~Restore() {
+                I.profiling = p;
+                I.phase_stamps = s;
+            }
+        } restore{I, prof, stamps};
         DevBuf<float> dQ;
         dQ.reserve((size_t)std::max(nq, 1) * I.n_features);
         if (nq > 0)
@@ -940,13 +952,55 @@ int fpp_query(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t*
         if (verbose) {
             const double ms =
                 std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
-            // src/FalconnPP.cpp:682-692
+            // src/FalconnPP.cpp:682-704.  The gather kernel fuses the bucket
+            // lookup with the distances: its time is split by the mean
+            // per-query phase times (walk + dedup vs scoring + top-k).
+            I.scratch_word.reserve(2);
+            const uint64_t empty = count_empty_probes(I.probes.get(), (int64_t)I.last_nprobes, I.bucket_off.get(),
+                                                      I.scratch_word.get(), st);
+            const double ph_look = I.stage_ms[4] + I.stage_ms[5], ph_dist = I.stage_ms[6] + I.stage_ms[7];
+            const double share = ph_look + ph_dist > 0 ? ph_look / (ph_look + ph_dist) : 0.0;
+            const double dq = nq ? (double)nq : 1.0;
+            printf("Average number of empty buckets per query: %g\n", (double)empty / dq);
             printf("Average number of probes per query: %g\n", (double)I.qProbes);
-            printf("Average number of unique candidates per query: %g\n", nq ? (double)I.stat_cuniq / nq : 0.0);
-            printf("Average number of candidates per query: %g\n", nq ? (double)I.stat_ctot / nq : 0.0);
+            printf("Average number of unique candidates per query: %g\n", (double)I.stat_cuniq / dq);
+            printf("Average number of candidates per query: %g\n", (double)I.stat_ctot / dq);
+            printf("Hash and Probing Time: %g ms\n", I.stage_ms[0] + I.stage_ms[1] + I.stage_ms[2]);
+            printf("Lookup Time: %g ms\n", I.stage_ms[3] * share);
+            printf("Distance Time: %g ms\n", I.stage_ms[3] * (1.0 - share));
             printf("Querying Time: %g ms\n", ms);
             fflush(stdout);
+            const std::string name = topk_file_name(k, I.n_proj, I.n_tables, I.iProbes, I.bucket_minSize,
+                                                    I.bucket_scale, I.qProbes);
+            write_topk_txt(name.c_str(), out, nq, k);
         }
+    });
+}
+
+int fpp_load_txt(const char* path, int n_points, int n_features, float* out, int64_t* n_read) {
+    return guard([&] {
+        FPP_REQUIRE(path && out, FPP_EINVAL, "null argument");
+        FPP_REQUIRE(n_points >= 0 && n_features >= 1, FPP_EINVAL, "n_points must be >= 0 and n_features >= 1");
+        const int64_t n = load_txt(path, (int64_t)n_points * n_features, out);
+        if (n_read) *n_read = n;
+    });
+}
+
+int fpp_write_topk_txt(const char* path, const int32_t* ids, int nq, int k) {
+    return guard([&] {
+        FPP_REQUIRE(path && (nq == 0 || ids), FPP_EINVAL, "null argument");
+        FPP_REQUIRE(nq >= 0 && k >= 1, FPP_EINVAL, "nq must be >= 0 and k >= 1");
+        write_topk_txt(path, ids, nq, k);
+    });
+}
+
+int fpp_topk_file_name(int k, int n_proj, int n_tables, int iProbes, int bucket_minSize, float bucket_scale,
+                       int qProbes, char* buf, int buflen) {
+    return guard([&] {
+        FPP_REQUIRE(buf && buflen >= 1, FPP_EINVAL, "null argument");
+        const std::string s = topk_file_name(k, n_proj, n_tables, iProbes, bucket_minSize, bucket_scale, qProbes);
+        FPP_REQUIRE((int64_t)s.size() < (int64_t)buflen, FPP_EINVAL, "buffer too short for the file name");
+        std::memcpy(buf, s.c_str(), s.size() + 1);
     });
 }
 

@@ -158,6 +158,9 @@ __host__ __device__ inline int64_t window_counter_words(int64_t N, int64_t W) {
 // Dedup window (ids): all of N when its bitset fits in LDS beside
 // extra_words of scratch, else the largest power of two that fits together
 // with the window counters.
+// number of probes[0 .. n) with an empty bucket; scratch2: two device words
+uint64_t count_empty_probes(const uint32_t* probes, int64_t n, const uint64_t* bucket_off, uint32_t* scratch2,
+                            hipStream_t st);
 int64_t gather_window_bits(int64_t N, int64_t extra_words, int per_cu);
 // Dedup (LDS bitset windows), FP32 rerank and top-k of each query.
 // cand/score: per-query segments at cand_off[q]; ncand[q] = unique count.

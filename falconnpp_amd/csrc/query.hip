@@ -549,6 +549,20 @@ __global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const u
     if (threadIdx.x == 0) ctot[q] = s_ct;
 }
 
+// Probes of a call whose bucket is empty (the reference's iNumEmptyBucket,
+// src/FalconnPP.cpp:584-588), summed into *count.
+__global__ __launch_bounds__(256) void k_count_empty(const uint32_t* __restrict__ probes, int64_t n,
+                                                     const uint64_t* __restrict__ bucket_off,
+                                                     unsigned long long* __restrict__ count) {
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = probes[i];
+        c += bucket_off[b + 1] == bucket_off[b] ? 1ull : 0ull;
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && c) atomicAdd(count, c);
+}
+
 size_t merge_lds_bytes(const ProbeGeom& g) {
     return (size_t)g.L * g.perTable * sizeof(IFPair) + (size_t)g.L * (sizeof(IFPair) + sizeof(int));
 }
@@ -945,6 +959,19 @@ void launch_xcd_order(const uint32_t* sorted, int nq, int nxcd, uint32_t* order,
     if (nq == 0) return;
     k_xcd_order<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(sorted, nq, nxcd, order);
     FPP_HIP(hipGetLastError());
+}
+
+uint64_t count_empty_probes(const uint32_t* probes, int64_t n, const uint64_t* bucket_off, uint32_t* scratch2,
+                            hipStream_t st) {
+    if (n <= 0) return 0;
+    launch_fill_u32(scratch2, 0u, 2, st);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 4096));
+    k_count_empty<<<grid, 256, 0, st>>>(probes, n, bucket_off, reinterpret_cast<unsigned long long*>(scratch2));
+    FPP_HIP(hipGetLastError());
+    unsigned long long c = 0;
+    FPP_HIP(hipMemcpyAsync(&c, scratch2, 8, hipMemcpyDeviceToHost, st));
+    FPP_HIP(hipStreamSynchronize(st));
+    return c;
 }
 
 int64_t probe_exact_work_bytes(const HashGeom& hg, const ProbeGeom& g) {

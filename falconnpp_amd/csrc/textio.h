@@ -1,0 +1,19 @@
+// Text formats of the reference's command-line driver (textio.cpp).
+#pragma once
+
+#include <cstdint>
+#include <string>
+
+namespace fpp {
+
+// Reads up to count floats, whitespace separated, in file order (fscanf "%f"
+// semantics); out[count] is zeroed first, so values the file lacks, or values
+// past the first token that is not a number, stay 0.  Returns the number read.
+int64_t load_txt(const char* path, int64_t count, float* out);
+// nq lines, each of the query's k ids followed by a space.
+void write_topk_txt(const char* path, const int32_t* ids, int64_t nq, int k);
+// Output file name of a verbose query.
+std::string topk_file_name(int k, int n_proj, int n_tables, int iProbes, int bucket_minSize, float bucket_scale,
+                           int qProbes);
+
+}  // namespace fpp

@@ -22,12 +22,12 @@ EXPORTS = (
     "fpp_get_center", "fpp_get_sign_bits", "fpp_get_buckets", "fpp_hash_points", "fpp_get_stats",
     "fpp_get_last_probes", "fpp_get_last_candidates", "fpp_bruteforce_f64", "fpp_set_profiling",
     "fpp_get_timings", "fpp_set_tuning", "fpp_set_precision", "fpp_autotune", "fpp_query_batch_device",
-    "fpp_get_query_times",
+    "fpp_get_query_times", "fpp_load_txt", "fpp_write_topk_txt", "fpp_topk_file_name",
 )
 PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
 TUNE_QUERY_ORDER, TUNE_GATHER_THREADS, TUNE_PERSISTENT, TUNE_GRAPH = 1, 2, 3, 4
 
-FPP_EINVAL, FPP_ESTATE, FPP_EHIP, FPP_ENOMEM, FPP_EINTERNAL = 1, 2, 3, 4, 5
+FPP_EINVAL, FPP_ESTATE, FPP_EHIP, FPP_ENOMEM, FPP_EINTERNAL, FPP_EIO = 1, 2, 3, 4, 5, 6
 
 _lib = None
 
@@ -77,6 +77,9 @@ def load():
     L.fpp_get_last_candidates.restype = i64
     L.fpp_get_last_candidates.argtypes = [vp, i32, vp, i64]
     L.fpp_bruteforce_f64.argtypes = [vp, i64, i64, i32, vp, i64, i32, i32, vp, vp, vp]
+    L.fpp_load_txt.argtypes = [C.c_char_p, i32, i32, vp, vp]
+    L.fpp_write_topk_txt.argtypes = [C.c_char_p, vp, i32, i32]
+    L.fpp_topk_file_name.argtypes = [i32, i32, i32, i32, i32, C.c_float, i32, C.c_char_p, i32]
     _lib = L
     return L
 
@@ -93,6 +96,31 @@ def _ptr(a: np.ndarray):
 def _points(a) -> np.ndarray:
     """N x D row-per-point float32 (== the reference's D x N column-major)."""
     return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def load_txt(path: str, n_points: int, n_features: int):
+    """The reference's loadtxtData (src/Utilities.cpp:41-60): n_points x
+    n_features float32 (row per point) and the number of values read; values
+    the file lacks stay 0.  No device use."""
+    out = np.zeros((n_points, n_features), np.float32)
+    n = np.zeros(1, np.int64)
+    _check(load().fpp_load_txt(os.fsencode(path), n_points, n_features, _ptr(out), _ptr(n)))
+    return out, int(n[0])
+
+
+def write_topk_txt(path: str, ids: np.ndarray):
+    """The reference's outputFile (src/Utilities.cpp:13-31): one line per
+    query, each id followed by a space.  No device use."""
+    a = np.ascontiguousarray(ids, np.int32)
+    _check(load().fpp_write_topk_txt(os.fsencode(path), _ptr(a), a.shape[0], a.shape[1]))
+
+
+def topk_file_name(k: int, n_proj: int, n_tables: int, iProbes: int, bucket_minSize: int, bucket_scale: float,
+                   qProbes: int) -> str:
+    """File name a verbose query writes (src/FalconnPP.cpp:694-700)."""
+    buf = C.create_string_buffer(512)
+    _check(load().fpp_topk_file_name(k, n_proj, n_tables, iProbes, bucket_minSize, bucket_scale, qProbes, buf, 512))
+    return buf.value.decode()
 
 
 class Index:

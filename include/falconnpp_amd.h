@@ -30,7 +30,8 @@ enum {
     FPP_ESTATE = 2,    /* call out of order (query before build, ...) */
     FPP_EHIP = 3,      /* HIP runtime error */
     FPP_ENOMEM = 4,    /* allocation failed */
-    FPP_EINTERNAL = 5  /* engine invariant broken */
+    FPP_EINTERNAL = 5, /* engine invariant broken */
+    FPP_EIO = 6        /* file cannot be opened, read or written */
 };
 
 typedef struct fpp_index fpp_index;
@@ -65,6 +66,9 @@ int fpp_build_device(fpp_index* h, const float* dX, int64_t ld, void* stream);
 
 /* query2Layers_1D(Q, k, verbose) -- src/FalconnPP.cpp:310-707, bound as
  * query(queries, n_neighbors, verbose=False) at python/python_wrapper.cpp:29-30.
+ * verbose != 0 prints the reference's per-call statistics and writes the ids
+ * to fpp_topk_file_name(...) in the working directory, as the reference does
+ * (src/FalconnPP.cpp:682-704).
  * Q: host D x nq col-major (ldq >= D); out: host nq x k. */
 int fpp_query(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t* out, int verbose);
 /* Same with Q and outputs in device memory; scores (nq x k, FP32 q.(x-c)) may
@@ -150,6 +154,26 @@ int64_t fpp_get_last_candidates(fpp_index* h, int j, int32_t* out, int64_t cap);
  * Outputs on the host, best first.  For recall checks only. */
 int fpp_bruteforce_f64(const float* dX, int64_t ldx, int64_t N, int D, const float* dQ, int64_t ldq, int nq, int k,
                        const int32_t* seed_ids, int32_t* out_ids, double* out_scores);
+
+/* ---- text formats of the reference's command-line driver (no device use) */
+/* loadtxtData(dataset, n, d, X) -- src/Utilities.cpp:41-60: n_points *
+ * n_features floats, whitespace separated (one point per line in the
+ * reference's files), read in file order into out, i.e. n_points x n_features
+ * row-major = the D x N column-major layout fpp_build takes.  Values the file
+ * lacks, or values past the first token that is not a number, stay 0, as with
+ * the reference's fscanf into a zeroed matrix.  FPP_EIO when the file cannot
+ * be opened.  *n_read (may be NULL) receives the number of values read. */
+int fpp_load_txt(const char* path, int n_points, int n_features, float* out, int64_t* n_read);
+/* outputFile(matKNN, file) -- src/Utilities.cpp:13-31: one line per query,
+ * each of its k ids followed by a space.  ids: nq x k row-major. */
+int fpp_write_topk_txt(const char* path, const int32_t* ids, int nq, int k);
+/* Name of the file query2Layers_1D writes when verbose
+ * (src/FalconnPP.cpp:694-700): 1D_Top_<k>_D_<n_proj>_L_<n_tables>_iProbe_<iProbes>
+ * _bucketMinSize_<bucket_minSize>_bucketScale_<(int)(bucket_scale*100)>_qProbe_<qProbes>.txt.
+ * buf receives at most buflen bytes including the terminating 0 (FPP_EINVAL
+ * when it is too short). */
+int fpp_topk_file_name(int k, int n_proj, int n_tables, int iProbes, int bucket_minSize, float bucket_scale,
+                       int qProbes, char* buf, int buflen);
 
 #ifdef __cplusplus
 }
