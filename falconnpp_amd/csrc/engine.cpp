@@ -737,6 +737,167 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     I.last_cand_off = off;
 }
 
+// ---- index files ------------------------------------------------------------
+// A built index as a little-endian file: a fixed header (magic, version, the
+// parameters and the section sizes), then the sections in order: sign bits of
+// layer 1 and 2 (one byte each), flip masks (u32), centre (f32), CSR offsets
+// (u64), CSR ids (i32), centred data (f32, rows of ldx).  Loading restores the
+// built state without hashing again.
+constexpr char kIndexMagic[8] = {'F', 'P', 'P', 'A', 'M', 'D', 'I', 'X'};
+constexpr int64_t kIndexVersion = 1;
+
+struct IndexFileHeader {
+    char magic[8];
+    int64_t version;
+    int64_t n_points, n_features, n_tables, n_proj, fhtDim, n_rotate, bucket_minSize, iProbes, seed;
+    int64_t ldx, n_ids, max_bucket, hash_ties, bucket_ties;
+    int64_t n_bits, n_mask_words, n_offsets;
+    float bucket_scale;
+    int32_t reserved;
+};
+
+struct File {
+    FILE* f = nullptr;
+    std::string path;
+    File(const char* p, const char* mode) : f(std::fopen(p, mode)), path(p) {
+        FPP_REQUIRE(f != nullptr, FPP_EIO, std::string("cannot open index file ") + p);
+    }
+    ~File() {
+        if (f) std::fclose(f);
+    }
+    void write(const void* src, size_t bytes) {
+        FPP_REQUIRE(std::fwrite(src, 1, bytes, f) == bytes, FPP_EIO, "cannot write index file " + path);
+    }
+    void read(void* dst, size_t bytes) {
+        FPP_REQUIRE(std::fread(dst, 1, bytes, f) == bytes, FPP_EIO, "index file " + path + " is truncated");
+    }
+};
+
+// device <-> file through a host staging buffer of at most 64 MiB
+void write_device(File& out, const void* d, size_t bytes) {
+    std::vector<unsigned char> buf(std::min<size_t>(bytes, size_t(64) << 20));
+    for (size_t o = 0; o < bytes; o += buf.size()) {
+        const size_t c = std::min(buf.size(), bytes - o);
+        FPP_HIP(hipMemcpy(buf.data(), static_cast<const unsigned char*>(d) + o, c, hipMemcpyDeviceToHost));
+        out.write(buf.data(), c);
+    }
+}
+void read_device(File& in, void* d, size_t bytes) {
+    std::vector<unsigned char> buf(std::min<size_t>(bytes, size_t(64) << 20));
+    for (size_t o = 0; o < bytes; o += buf.size()) {
+        const size_t c = std::min(buf.size(), bytes - o);
+        in.read(buf.data(), c);
+        FPP_HIP(hipMemcpy(static_cast<unsigned char*>(d) + o, buf.data(), c, hipMemcpyHostToDevice));
+    }
+}
+
+void save_index(const Index& I, const char* path) {
+    FPP_REQUIRE(I.built, FPP_ESTATE, "save called before build (or after clear)");
+    IndexFileHeader h{};
+    std::memcpy(h.magic, kIndexMagic, 8);
+    h.version = kIndexVersion;
+    h.n_points = I.n_points;
+    h.n_features = I.n_features;
+    h.n_tables = I.n_tables;
+    h.n_proj = I.n_proj;
+    h.fhtDim = I.fhtDim;
+    h.n_rotate = I.n_rotate;
+    h.bucket_minSize = I.bucket_minSize;
+    h.iProbes = I.iProbes;
+    h.seed = I.seed;
+    h.ldx = I.ldx;
+    h.n_ids = I.n_ids;
+    h.max_bucket = I.max_bucket;
+    h.hash_ties = I.stat_hash_ties;
+    h.bucket_ties = I.stat_bucket_ties;
+    h.n_bits = (int64_t)I.bit1.size();
+    h.n_mask_words = (int64_t)I.masks.n;
+    h.n_offsets = (int64_t)I.n_tables * I.NB() + 1;
+    h.bucket_scale = I.bucket_scale;
+    File out(path, "wb");
+    out.write(&h, sizeof h);
+    out.write(I.bit1.data(), I.bit1.size());
+    out.write(I.bit2.data(), I.bit2.size());
+    write_device(out, I.masks.get(), (size_t)h.n_mask_words * 4);
+    write_device(out, I.center.get(), (size_t)I.n_features * 4);
+    write_device(out, I.bucket_off.get(), (size_t)h.n_offsets * 8);
+    write_device(out, I.ids.get(), (size_t)I.n_ids * 4);
+    write_device(out, I.Xc.get(), (size_t)I.n_points * I.ldx * 4);
+}
+
+void load_index(Index& I, const char* path) {
+    File in(path, "rb");
+    IndexFileHeader h{};
+    in.read(&h, sizeof h);
+    FPP_REQUIRE(std::memcmp(h.magic, kIndexMagic, 8) == 0, FPP_EINVAL, std::string(path) + " is not an index file");
+    FPP_REQUIRE(h.version == kIndexVersion, FPP_EINVAL, "unsupported index file version");
+    FPP_REQUIRE(h.n_points == I.n_points && h.n_features == I.n_features, FPP_EINVAL,
+                "index file holds n_points = " + std::to_string(h.n_points) + ", n_features = " +
+                    std::to_string(h.n_features) + ", not those of this index");
+    Index T;  // parameters checked on a scratch index first
+    T.n_points = (int)h.n_points;
+    T.n_features = (int)h.n_features;
+    T.n_tables = (int)h.n_tables;
+    T.n_proj = (int)h.n_proj;
+    T.fhtDim = (int)h.fhtDim;
+    T.iProbes = (int)h.iProbes;
+    T.params = true;
+    check_params(T);
+    FPP_REQUIRE(h.n_rotate == I.n_rotate, FPP_EINVAL, "index file: unsupported number of rotations");
+    FPP_REQUIRE(h.n_bits == h.fhtDim * h.n_tables * h.n_rotate &&
+                    h.n_mask_words == h.n_tables * 2 * 3 * ((h.fhtDim + 31) / 32) &&
+                    h.n_offsets == h.n_tables * T.NB() + 1 && h.ldx == round_up(h.n_features, 4) && h.n_ids >= 0 &&
+                    h.max_bucket >= 0,
+                FPP_EINVAL, "index file: inconsistent sizes");
+    // section bytes against the file length
+    const int64_t need = (int64_t)sizeof h + 2 * h.n_bits + 4 * h.n_mask_words + 4 * h.n_features + 8 * h.n_offsets +
+                         4 * h.n_ids + 4 * h.n_points * h.ldx;
+    const long here = std::ftell(in.f);
+    FPP_REQUIRE(std::fseek(in.f, 0, SEEK_END) == 0, FPP_EIO, "cannot seek in index file");
+    const int64_t len = (int64_t)std::ftell(in.f);
+    FPP_REQUIRE(std::fseek(in.f, here, SEEK_SET) == 0, FPP_EIO, "cannot seek in index file");
+    FPP_REQUIRE(len == need, FPP_EINVAL, "index file " + std::string(path) + " has " + std::to_string(len) +
+                                             " bytes, its header describes " + std::to_string(need));
+    I.built = false;
+    I.lp_ready = 0;
+    I.Xb.reset();
+    I.X8.reset();
+    I.s8.reset();
+    I.n_tables = T.n_tables;
+    I.n_proj = T.n_proj;
+    I.fhtDim = T.fhtDim;
+    I.iProbes = T.iProbes;
+    I.bucket_minSize = (int)h.bucket_minSize;
+    I.bucket_scale = h.bucket_scale;
+    I.seed = (int)h.seed;
+    I.params = true;
+    I.bit1.resize((size_t)h.n_bits);
+    I.bit2.resize((size_t)h.n_bits);
+    in.read(I.bit1.data(), I.bit1.size());
+    in.read(I.bit2.data(), I.bit2.size());
+    I.masks.reserve((size_t)h.n_mask_words);
+    read_device(in, I.masks.get(), (size_t)h.n_mask_words * 4);
+    I.center.reserve((size_t)h.n_features);
+    read_device(in, I.center.get(), (size_t)h.n_features * 4);
+    I.bucket_off.reserve((size_t)h.n_offsets);
+    read_device(in, I.bucket_off.get(), (size_t)h.n_offsets * 8);
+    I.ids.reserve((size_t)std::max<int64_t>(h.n_ids, 1));
+    read_device(in, I.ids.get(), (size_t)h.n_ids * 4);
+    I.ldx = h.ldx;
+    I.Xc.reserve((size_t)h.n_points * h.ldx);
+    read_device(in, I.Xc.get(), (size_t)h.n_points * h.ldx * 4);
+    uint64_t last = 0;
+    FPP_HIP(hipMemcpy(&last, I.bucket_off.get() + (h.n_offsets - 1), 8, hipMemcpyDeviceToHost));
+    FPP_REQUIRE((int64_t)last == h.n_ids, FPP_EINVAL, "index file: bucket offsets do not end at the id count");
+    I.n_ids = h.n_ids;
+    I.max_bucket = (uint32_t)h.max_bucket;
+    I.stat_hash_ties = h.hash_ties;
+    I.stat_bucket_ties = h.bucket_ties;
+    I.build_gen++;
+    I.drop_graph();
+    I.built = true;
+}
+
 }  // namespace
 }  // namespace fpp
 
@@ -974,6 +1135,20 @@ int fpp_query(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t*
                                                     I.bucket_scale, I.qProbes);
             write_topk_txt(name.c_str(), out, nq, k);
         }
+    });
+}
+
+int fpp_save(fpp_index* h, const char* path) {
+    return guard([&] {
+        FPP_REQUIRE(h && path, FPP_EINVAL, "null argument");
+        save_index(h->I, path);
+    });
+}
+
+int fpp_load(fpp_index* h, const char* path) {
+    return guard([&] {
+        FPP_REQUIRE(h && path, FPP_EINVAL, "null argument");
+        load_index(h->I, path);
     });
 }
 

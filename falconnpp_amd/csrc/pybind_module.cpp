@@ -21,6 +21,10 @@ using FArray = py::array_t<float, py::array::f_style | py::array::forcecast>;
 
 void rethrow(const falconnpp::Error& e) {
     if (e.code == FPP_EINVAL || e.code == FPP_ESTATE) throw py::value_error(e.what());
+    if (e.code == FPP_EIO) {
+        PyErr_SetString(PyExc_OSError, e.what());
+        throw py::error_already_set();
+    }
     throw std::runtime_error(e.what());
 }
 
@@ -84,6 +88,29 @@ PYBIND11_MODULE(FalconnPP, m) {
                      rethrow(e);
                  }
              })
+        // index files (no reference counterpart)
+        .def(
+            "save",
+            [](FalconnPP& f, const std::string& path) {
+                try {
+                    py::gil_scoped_release nogil;
+                    f.save(path);
+                } catch (const falconnpp::Error& e) {
+                    rethrow(e);
+                }
+            },
+            py::arg("path"))
+        .def(
+            "load",
+            [](FalconnPP& f, const std::string& path) {
+                try {
+                    py::gil_scoped_release nogil;
+                    f.load(path);
+                } catch (const falconnpp::Error& e) {
+                    rethrow(e);
+                }
+            },
+            py::arg("path"))
         .def(
             "build",
             [](FalconnPP& f, const py::handle& ds) {

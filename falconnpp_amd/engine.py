@@ -22,7 +22,7 @@ EXPORTS = (
     "fpp_get_center", "fpp_get_sign_bits", "fpp_get_buckets", "fpp_hash_points", "fpp_get_stats",
     "fpp_get_last_probes", "fpp_get_last_candidates", "fpp_bruteforce_f64", "fpp_set_profiling",
     "fpp_get_timings", "fpp_set_tuning", "fpp_set_precision", "fpp_autotune", "fpp_query_batch_device",
-    "fpp_get_query_times", "fpp_load_txt", "fpp_write_topk_txt", "fpp_topk_file_name",
+    "fpp_get_query_times", "fpp_load_txt", "fpp_write_topk_txt", "fpp_topk_file_name", "fpp_save", "fpp_load",
 )
 PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
 TUNE_QUERY_ORDER, TUNE_GATHER_THREADS, TUNE_PERSISTENT, TUNE_GRAPH = 1, 2, 3, 4
@@ -78,6 +78,8 @@ def load():
     L.fpp_get_last_candidates.argtypes = [vp, i32, vp, i64]
     L.fpp_bruteforce_f64.argtypes = [vp, i64, i64, i32, vp, i64, i32, i32, vp, vp, vp]
     L.fpp_load_txt.argtypes = [C.c_char_p, i32, i32, vp, vp]
+    L.fpp_save.argtypes = [vp, C.c_char_p]
+    L.fpp_load.argtypes = [vp, C.c_char_p]
     L.fpp_write_topk_txt.argtypes = [C.c_char_p, vp, i32, i32]
     L.fpp_topk_file_name.argtypes = [i32, i32, i32, i32, i32, C.c_float, i32, C.c_char_p, i32]
     _lib = L
@@ -146,6 +148,14 @@ class Index:
         self.L, self.n_proj, self.iProbes = n_tables, n_proj, iProbes
         _check(load().fpp_set_index_param(self.h, n_tables, n_proj, bucket_minSize, bucket_scale, iProbes,
                                           n_threads, random_seed))
+
+    def save(self, path: str):
+        """Write the built index to path (no reference counterpart)."""
+        _check(load().fpp_save(self.h, os.fsencode(path)))
+
+    def load(self, path: str):
+        """Restore an index written by save (same n_points and n_features)."""
+        _check(load().fpp_load(self.h, os.fsencode(path)))
 
     def set_qProbes(self, p: int):
         self.qprobes = p

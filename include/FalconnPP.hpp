@@ -58,6 +58,10 @@ public:
         return query2Layers_1D(Q, ldq, nq, k, verbose);
     }
 
+    // index files (no reference counterpart)
+    void save(const std::string& path) { check(fpp_save(h_, path.c_str())); }
+    void load(const std::string& path) { check(fpp_load(h_, path.c_str())); }
+
     int n_points() const { return n_points_; }
     int n_features() const { return n_features_; }
     fpp_index* handle() const { return h_; }

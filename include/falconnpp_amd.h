@@ -155,6 +155,19 @@ int64_t fpp_get_last_candidates(fpp_index* h, int j, int32_t* out, int64_t cap);
 int fpp_bruteforce_f64(const float* dX, int64_t ldx, int64_t N, int D, const float* dQ, int64_t ldq, int nq, int k,
                        const int32_t* seed_ids, int32_t* out_ids, double* out_scores);
 
+/* ---- index files (no reference counterpart: the reference rebuilds its
+ * index in every process) */
+/* Writes the built index (parameters, sign bits, centre, CSR tables, centred
+ * data) to path.  FPP_ESTATE before build; FPP_EIO when the file cannot be
+ * written. */
+int fpp_save(fpp_index* h, const char* path);
+/* Restores an index written by fpp_save into h, which must have been created
+ * with the file's n_points and n_features; afterwards h answers queries as the
+ * saved index did, without building.  FPP_EIO when the file cannot be read,
+ * FPP_EINVAL when it is not an index file, is truncated or does not match.  On
+ * a failure after the checks, h is left unbuilt. */
+int fpp_load(fpp_index* h, const char* path);
+
 /* ---- text formats of the reference's command-line driver (no device use) */
 /* loadtxtData(dataset, n, d, X) -- src/Utilities.cpp:41-60: n_points *
  * n_features floats, whitespace separated (one point per line in the
