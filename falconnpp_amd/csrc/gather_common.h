@@ -105,19 +105,24 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 }
 
 // Calls f(id, true) for every id of the probed buckets pr[0 .. np), with all
-// lanes of the wave present (lanes without an id pass false).  A wave takes
-// 64 probes at a time and walks the concatenation of their bucket lists with
-// one lane per id, so neighbouring lanes read neighbouring ids and a load
-// instruction touches a few cache lines rather than one per lane.  A binary
-// search over the wave's prefix sums of bucket lengths maps a position to its
-// probe.  The next 64 probes' buckets are fetched while the current ones are
-// walked.
+// lanes of the wave present (lanes without an id pass false), in no particular
+// order.  A wave takes 64 probes at a time; the next 64 probes' buckets are
+// fetched while the current ones are walked.
+//  * Lane per bucket (the longest of the 64 buckets holds at most LANE_MAX
+//    ids): lane p reads bucket p's ids itself, UNR at a time, so nothing
+//    crosses lanes and a step is a load and f.  Lanes whose bucket is done
+//    pass false until the longest one is.
+//  * Flat (a longer bucket): the concatenation of the 64 lists is walked one
+//    lane per id; a binary search over the wave's prefix sums of bucket
+//    lengths maps a position to its probe (six dependent lane shuffles per
+//    id, so only where lane per bucket would leave most lanes idle).
 template <int TB, class F>
 __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ pr,
                                                    const uint64_t* __restrict__ bucket_off,
                                                    const int32_t* __restrict__ ids, int np, F&& f) {
     constexpr int NW = TB / WAVE;
     constexpr int UNR = 8;
+    constexpr uint32_t LANE_MAX = 64;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
     int c0 = wv * WAVE;
@@ -131,24 +136,38 @@ __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ 
     for (; c0 < np; c0 += NW * WAVE) {
         const int cn = c0 + NW * WAVE;
         const uint32_t nb = cn + lane < np ? pr[cn + lane] : 0u;
-        const uint32_t incl = wave_incl_scan(len);
-        const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
-        const uint64_t sx = s - (uint64_t)(incl - len);  // ids index of flat position f is sx + f
-        for (uint32_t f0 = 0; f0 < total; f0 += WAVE * UNR) {
-            int32_t v[UNR];
+        uint32_t mx = len;
 #pragma unroll
-            for (int r = 0; r < UNR; ++r) {
-                const uint32_t fp = f0 + r * WAVE + lane;
-                // probe of position fp: number of lanes whose inclusive sum is <= fp
-                int lo = 0;
+        for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, WAVE));
+        mx = __builtin_amdgcn_readfirstlane(mx);
+        if (mx <= LANE_MAX) {
+            for (uint32_t j0 = 0; j0 < mx; j0 += UNR) {
+                int32_t v[UNR];
 #pragma unroll
-                for (int st = WAVE / 2; st > 0; st >>= 1)
-                    if (__shfl(incl, lo + st - 1, WAVE) <= fp) lo += st;
-                const uint64_t base = __shfl(sx, lo, WAVE);
-                v[r] = fp < total ? ids[base + fp] : 0;
+                for (int r = 0; r < UNR; ++r) v[r] = j0 + r < len ? ids[s + j0 + r] : 0;
+#pragma unroll
+                for (int r = 0; r < UNR; ++r) f(v[r], j0 + r < len);
             }
+        } else {
+            const uint32_t incl = wave_incl_scan(len);
+            const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+            const uint64_t sx = s - (uint64_t)(incl - len);  // ids index of flat position f is sx + f
+            for (uint32_t f0 = 0; f0 < total; f0 += WAVE * UNR) {
+                int32_t v[UNR];
 #pragma unroll
-            for (int r = 0; r < UNR; ++r) f(v[r], f0 + r * WAVE + lane < total);
+                for (int r = 0; r < UNR; ++r) {
+                    const uint32_t fp = f0 + r * WAVE + lane;
+                    // probe of position fp: number of lanes whose inclusive sum is <= fp
+                    int lo = 0;
+#pragma unroll
+                    for (int st = WAVE / 2; st > 0; st >>= 1)
+                        if (__shfl(incl, lo + st - 1, WAVE) <= fp) lo += st;
+                    const uint64_t base = __shfl(sx, lo, WAVE);
+                    v[r] = fp < total ? ids[base + fp] : 0;
+                }
+#pragma unroll
+                for (int r = 0; r < UNR; ++r) f(v[r], f0 + r * WAVE + lane < total);
+            }
         }
         s = 0;
         len = 0;
