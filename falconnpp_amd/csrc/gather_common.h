@@ -109,19 +109,19 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 // order.  A wave takes 64 probes at a time; the next 64 probes' buckets are
 // fetched while the current ones are walked.
 //  * Lane per bucket (the longest of the 64 buckets holds at most LANE_MAX
-//    ids): lane p reads bucket p's ids itself, UNR at a time, so nothing
+//    ids): lane p reads bucket p's ids itself, ULANE at a time, so nothing
 //    crosses lanes and a step is a load and f.  Lanes whose bucket is done
 //    pass false until the longest one is.
 //  * Flat (a longer bucket): the concatenation of the 64 lists is walked one
 //    lane per id; a binary search over the wave's prefix sums of bucket
 //    lengths maps a position to its probe (six dependent lane shuffles per
 //    id, so only where lane per bucket would leave most lanes idle).
-template <int TB, class F>
+template <int TB, int ULANE, class F>
 __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ pr,
                                                    const uint64_t* __restrict__ bucket_off,
                                                    const int32_t* __restrict__ ids, int np, F&& f) {
     constexpr int NW = TB / WAVE;
-    constexpr int UNR = 8;
+    constexpr int UNR = 8;  // flat walk: ids per lane in flight
     constexpr uint32_t LANE_MAX = 64;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
@@ -141,12 +141,12 @@ __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ 
         for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, WAVE));
         mx = __builtin_amdgcn_readfirstlane(mx);
         if (mx <= LANE_MAX) {
-            for (uint32_t j0 = 0; j0 < mx; j0 += UNR) {
-                int32_t v[UNR];
+            for (uint32_t j0 = 0; j0 < mx; j0 += ULANE) {
+                int32_t v[ULANE];
 #pragma unroll
-                for (int r = 0; r < UNR; ++r) v[r] = j0 + r < len ? ids[s + j0 + r] : 0;
+                for (int r = 0; r < ULANE; ++r) v[r] = j0 + r < len ? ids[s + j0 + r] : 0;
 #pragma unroll
-                for (int r = 0; r < UNR; ++r) f(v[r], j0 + r < len);
+                for (int r = 0; r < ULANE; ++r) f(v[r], j0 + r < len);
             }
         } else {
             const uint32_t incl = wave_incl_scan(len);
@@ -287,7 +287,9 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         const int words = (int)(W / 32);
         for (int i = threadIdx.x; i < words; i += TB) bits[i] = 0u;
         __syncthreads();
-        for_each_bucket_id<TB>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+        // 24 ids per lane in flight: one step covers a group of buckets of
+        // the usual sizes (the bucket filter keeps them near bucketLimit)
+        for_each_bucket_id<TB, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
             if (ok) atomicOr(&bits[v >> 5], 1u << (v & 31));
         });
         __syncthreads();
@@ -304,7 +306,7 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         wfill[i] = 0u;
     }
     __syncthreads();
-    for_each_bucket_id<TB>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+    for_each_bucket_id<TB, 8>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
         if (ok) atomicAdd(&wcnt[v >> shift], 1u);
     });
     __syncthreads();
@@ -324,7 +326,7 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
             }
     }
     __syncthreads();
-    for_each_bucket_id<TB>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+    for_each_bucket_id<TB, 8>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
         if (ok) {
             const int w = v >> shift;
             const uint32_t pos = woff[w] + atomicAdd(&wfill[w], 1u);
