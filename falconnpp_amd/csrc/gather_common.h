@@ -183,62 +183,41 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Exclusive wave prefix sum of p (0 <= p <= 32) and its total, from ballots of
-// the bits of p: no cross-lane data movement, so no LDS round trips.
-__device__ __forceinline__ uint32_t wave_excl_prefix_small(uint32_t p, uint32_t& total) {
-    if (__ballot(p > 1u) == 0ull) {
-        const uint64_t m = __ballot(p != 0u);
-        total = (uint32_t)__popcll(m);
-        return lane_rank(m);
-    }
-    uint32_t ex = 0, tot = 0;
-#pragma unroll
-    for (int b = 0; b < 6; ++b) {
-        const uint64_t m = __ballot((p >> b) & 1u);
-        ex += lane_rank(m) << b;
-        tot += (uint32_t)__popcll(m) << b;
-    }
-    total = tot;
-    return ex;
-}
-
 // Ordered compaction of one window's bitset (words words, ids from w0) into
-// cq[nc ...]: wave v owns a contiguous run of words and its lanes read
-// consecutive words.  red holds TB / 64 words.  Returns the new count.
+// cq[nc ...]: thread t owns the contiguous run of r words from t * r (r odd,
+// so at every step the lanes of a wave read 32 different LDS banks).  It
+// counts its bits; a wave scan and the waves' totals give its first output
+// position, and it writes its ids in order.  red holds TB / 64 words.
+// Returns the new count.
 template <int TB>
 __device__ __forceinline__ uint32_t compact_window(const uint32_t* bits, int words, int64_t w0, uint32_t nc,
                                                    uint32_t* red, int32_t* __restrict__ cq, uint64_t ccap) {
     constexpr int NW = TB / WAVE;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
-    const int per = (int)ceil_div(ceil_div(words, NW), WAVE) * WAVE;
-    const int wb = wv * per;
-    const int we = min(words, wb + per);
+    const int r = (int)ceil_div(words, TB) | 1;
+    const int b = min(words, (int)threadIdx.x * r);
+    const int e = min(words, b + r);
     uint32_t c = 0;
-    for (int i = wb + lane; i < we; i += WAVE) c += __popc(bits[i]);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, WAVE);
-    if (lane == 0) red[wv] = c;
+    for (int i = b; i < e; ++i) c += __popc(bits[i]);
+    const uint32_t incl = wave_incl_scan(c);
+    if (lane == WAVE - 1) red[wv] = incl;
     __syncthreads();
-    uint32_t base = nc, tot = 0;
+    uint32_t base = nc + incl - c, tot = 0;
 #pragma unroll
     for (int v = 0; v < NW; ++v) {
         const uint32_t x = red[v];
         base += v < wv ? x : 0u;
         tot += x;
     }
-    for (int i0 = wb; i0 < we; i0 += WAVE) {
-        const int i = i0 + lane;
-        uint32_t word = i < we ? bits[i] : 0u;
-        uint32_t slab;
-        uint32_t pos = base + wave_excl_prefix_small(__popc(word), slab);
+    for (int i = b; i < e; ++i) {
+        uint32_t word = bits[i];
         while (word) {
             const int bit = __ffs(word) - 1;
             word &= word - 1;
-            if (pos < ccap) cq[pos] = (int32_t)(w0 + (int64_t)i * 32 + bit);
-            pos++;
+            if (base < ccap) cq[base] = (int32_t)(w0 + (int64_t)i * 32 + bit);
+            base++;
         }
-        base += slab;
     }
     __syncthreads();
     return nc + tot;
