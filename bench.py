@@ -61,6 +61,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="shrink N and nq (testing only)")
     ap.add_argument("--qprobes", type=int, default=0, help="override qProbes")
     ap.add_argument("--threads", type=int, default=0, help="gather workgroup size (0: engine default)")
+    ap.add_argument("--persistent", type=int, default=-1, help="persistent gather launch 0/1 (-1: engine default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the BF16/FP8 + refine runs")
@@ -122,6 +123,8 @@ def main():
     idx.set_qProbes(cfg["qProbes"])
     if args.threads:
         idx.set_tuning(E.TUNE_GATHER_THREADS, args.threads)
+    if args.persistent >= 0:
+        idx.set_tuning(E.TUNE_PERSISTENT, args.persistent)
     out = torch.zeros((nq, k), dtype=torch.int32, device="cuda")
     scores = torch.zeros((nq, k), dtype=torch.float32, device="cuda")
     idx.set_profiling(1)

@@ -104,10 +104,11 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
-// Calls f(id, true) for every id of the probed buckets pr[0 .. np), with all
-// lanes of the wave present (lanes without an id pass false), in no particular
-// order.  A wave takes 64 probes at a time; the next 64 probes' buckets are
-// fetched while the current ones are walked.
+// Walk over the ids of the probed buckets pr[0 .. np), one group of 64 probes
+// per step of a wave (wave v takes groups v, v + NW, ...); step(f) calls
+// f(id, true) for every id of its group, with all lanes of the wave present
+// (lanes without an id pass false), in no particular order.  The next group's
+// buckets are fetched while the current ones are walked.
 //  * Lane per bucket (the longest of the 64 buckets holds at most LANE_MAX
 //    ids): lane p reads bucket p's ids itself, ULANE at a time, so nothing
 //    crosses lanes and a step is a load and f.  Lanes whose bucket is done
@@ -116,24 +117,40 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 //    lane per id; a binary search over the wave's prefix sums of bucket
 //    lengths maps a position to its probe (six dependent lane shuffles per
 //    id, so only where lane per bucket would leave most lanes idle).
-template <int TB, int ULANE, class F>
-__device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ pr,
-                                                   const uint64_t* __restrict__ bucket_off,
-                                                   const int32_t* __restrict__ ids, int np, F&& f) {
-    constexpr int NW = TB / WAVE;
-    constexpr int UNR = 8;  // flat walk: ids per lane in flight
-    constexpr uint32_t LANE_MAX = 64;
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wv = threadIdx.x / WAVE;
-    int c0 = wv * WAVE;
-    uint64_t s = 0;
-    uint32_t len = 0;
-    if (c0 + lane < np) {
-        const uint32_t b = pr[c0 + lane];
-        s = bucket_off[b];
-        len = (uint32_t)(bucket_off[b + 1] - s);
+template <int TB, int ULANE>
+struct BucketWalk {
+    static constexpr int NW = TB / WAVE;
+    static constexpr int UNR = 8;  // flat walk: ids per lane in flight
+    static constexpr uint32_t LANE_MAX = 64;
+    const uint32_t* __restrict__ pr;
+    const uint64_t* __restrict__ bucket_off;
+    const int32_t* __restrict__ ids;
+    int np;
+    int c0;
+    uint64_t s;
+    uint32_t len;
+
+    __device__ __forceinline__ void init(const uint32_t* pr_, const uint64_t* off_, const int32_t* ids_, int np_) {
+        pr = pr_;
+        bucket_off = off_;
+        ids = ids_;
+        np = np_;
+        const int lane = threadIdx.x & (WAVE - 1);
+        c0 = (int)(threadIdx.x / WAVE) * WAVE;
+        s = 0;
+        len = 0;
+        if (c0 + lane < np) {
+            const uint32_t b = pr[c0 + lane];
+            s = bucket_off[b];
+            len = (uint32_t)(bucket_off[b + 1] - s);
+        }
     }
-    for (; c0 < np; c0 += NW * WAVE) {
+    // wave-uniform
+    __device__ __forceinline__ bool done() const { return c0 >= np; }
+
+    template <class F>
+    __device__ __forceinline__ void step(F&& f) {
+        const int lane = threadIdx.x & (WAVE - 1);
         const int cn = c0 + NW * WAVE;
         const uint32_t nb = cn + lane < np ? pr[cn + lane] : 0u;
         uint32_t mx = len;
@@ -169,6 +186,7 @@ __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ 
                 for (int r = 0; r < UNR; ++r) f(v[r], f0 + r * WAVE + lane < total);
             }
         }
+        c0 = cn;
         s = 0;
         len = 0;
         if (cn + lane < np) {
@@ -176,6 +194,16 @@ __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ 
             len = (uint32_t)(bucket_off[nb + 1] - s);
         }
     }
+};
+
+// Every id of the probed buckets pr[0 .. np) through f (BucketWalk, all groups).
+template <int TB, int ULANE, class F>
+__device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ pr,
+                                                   const uint64_t* __restrict__ bucket_off,
+                                                   const int32_t* __restrict__ ids, int np, F&& f) {
+    BucketWalk<TB, ULANE> w;
+    w.init(pr, bucket_off, ids, np);
+    while (!w.done()) w.step(f);
 }
 
 // Rank of this lane among the set lanes of m.
