@@ -16,6 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 def __getattr__(name):
     if name == "FalconnPP":
         import sys
+        from .engine import share_hip_runtime
+        share_hip_runtime()
         if HERE not in sys.path:
             sys.path.insert(0, HERE)
         mod = importlib.import_module("FalconnPP")

@@ -899,6 +899,65 @@ void load_index(Index& I, const char* path) {
     I.built = true;
 }
 
+// Times the gather launch shapes with run(st) (one query call on stream st)
+// and keeps the fastest: workgroup size, query order (per-query launches
+// only; the persistent launch takes its own longest-first order) and
+// persistent launch.  chosen receives {threads, order, persistent, us}.
+template <class Run>
+void autotune(Index& I, hipStream_t st, Run run, int32_t* chosen) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    FPP_HIP(hipEventCreate(&e0));
+    FPP_HIP(hipEventCreate(&e1));
+    const bool profiling = I.profiling, stamps = I.phase_stamps;
+    I.profiling = I.phase_stamps = false;
+    // workgroup size applies to the FP32 gather; the reduced-precision
+    // gather has one shape
+    const std::vector<int> threads = I.precision == 0 ? std::vector<int>{256, 512, 1024} : std::vector<int>{512};
+    float best = 0.0f;
+    int bt = I.gather_threads, bo = I.order_queries ? 1 : 0, bp = I.persistent ? 1 : 0;
+    try {
+        for (int t : threads)
+            for (int pers = 0; pers < 2; ++pers)
+                for (int o = 0; o < (pers ? 1 : 2); ++o) {
+                    I.gather_threads = t;
+                    I.order_queries = o != 0;
+                    I.persistent = pers != 0;
+                    run(st);
+                    FPP_HIP(hipEventRecord(e0, st));
+                    for (int r = 0; r < 2; ++r) run(st);
+                    FPP_HIP(hipEventRecord(e1, st));
+                    FPP_HIP(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    FPP_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (best == 0.0f || ms < best) {
+                        best = ms;
+                        bt = t;
+                        bo = o;
+                        bp = pers;
+                    }
+                }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        I.profiling = profiling;
+        I.phase_stamps = stamps;
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    I.profiling = profiling;
+    I.phase_stamps = stamps;
+    I.gather_threads = bt;
+    I.order_queries = bo != 0;
+    I.persistent = bp != 0;
+    if (chosen) {
+        chosen[0] = bt;
+        chosen[1] = bo;
+        chosen[2] = bp;
+        chosen[3] = (int32_t)(best / 2.0f * 1000.0f);  // microseconds per batch
+    }
+}
+
 }  // namespace
 }  // namespace fpp
 
@@ -1025,60 +1084,24 @@ int fpp_autotune(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, void
     return guard([&] {
         FPP_REQUIRE(h && dQ && nq >= 1, FPP_EINVAL, "null argument or empty sample");
         Index& I = h->I;
-        hipStream_t st = pick_stream(I, stream);
         DevBuf<int32_t> out;
         out.reserve((size_t)nq * k);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        FPP_HIP(hipEventCreate(&e0));
-        FPP_HIP(hipEventCreate(&e1));
-        const bool profiling = I.profiling, stamps = I.phase_stamps;
-        I.profiling = I.phase_stamps = false;
-        // workgroup size applies to the FP32 gather; the reduced-precision
-        // gather has one shape
-        const std::vector<int> threads = I.precision == 0 ? std::vector<int>{256, 512, 1024} : std::vector<int>{512};
-        float best = 0.0f;
-        int bt = I.gather_threads, bo = I.order_queries ? 1 : 0, bp = I.persistent ? 1 : 0;
-        try {
-            for (int t : threads)
-                for (int o = 0; o < 2; ++o)
-                    for (int pers = 0; pers < 2; ++pers) {
-                        I.gather_threads = t;
-                        I.order_queries = o != 0;
-                        I.persistent = pers != 0;
-                        do_query(I, dQ, ldq, nq, k, out.get(), nullptr, st);
-                        FPP_HIP(hipEventRecord(e0, st));
-                        for (int r = 0; r < 2; ++r) do_query(I, dQ, ldq, nq, k, out.get(), nullptr, st);
-                        FPP_HIP(hipEventRecord(e1, st));
-                        FPP_HIP(hipEventSynchronize(e1));
-                        float ms = 0.0f;
-                        FPP_HIP(hipEventElapsedTime(&ms, e0, e1));
-                        if (best == 0.0f || ms < best) {
-                            best = ms;
-                            bt = t;
-                            bo = o;
-                            bp = pers;
-                        }
-                    }
-        } catch (...) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            I.profiling = profiling;
-            I.phase_stamps = stamps;
-            throw;
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        I.profiling = profiling;
-        I.phase_stamps = stamps;
-        I.gather_threads = bt;
-        I.order_queries = bo != 0;
-        I.persistent = bp != 0;
-        if (chosen) {
-            chosen[0] = bt;
-            chosen[1] = bo;
-            chosen[2] = bp;
-            chosen[3] = (int32_t)(best / 2.0f * 1000.0f);  // microseconds per batch
-        }
+        autotune(I, pick_stream(I, stream), [&](hipStream_t st) { do_query(I, dQ, ldq, nq, k, out.get(), nullptr, st); },
+                 chosen);
+    });
+}
+
+int fpp_autotune_batch(fpp_index* h, const float* dQ, int64_t ldq, int nq, const int32_t* qprobes,
+                       const int32_t* ks, int kmax, void* stream, int32_t* chosen) {
+    return guard([&] {
+        FPP_REQUIRE(h && dQ && qprobes && ks && nq >= 1, FPP_EINVAL, "null argument or empty sample");
+        Index& I = h->I;
+        FPP_REQUIRE(kmax >= 1, FPP_EINVAL, "kmax must be >= 1");
+        DevBuf<int32_t> out;
+        out.reserve((size_t)nq * kmax);
+        autotune(
+            I, pick_stream(I, stream),
+            [&](hipStream_t st) { do_query(I, dQ, ldq, nq, kmax, out.get(), nullptr, st, qprobes, ks); }, chosen);
     });
 }
 

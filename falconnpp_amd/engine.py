@@ -21,7 +21,8 @@ EXPORTS = (
     "fpp_clear", "fpp_build", "fpp_build_device", "fpp_query", "fpp_query_device", "fpp_get_shape",
     "fpp_get_center", "fpp_get_sign_bits", "fpp_get_buckets", "fpp_hash_points", "fpp_get_stats",
     "fpp_get_last_probes", "fpp_get_last_candidates", "fpp_bruteforce_f64", "fpp_set_profiling",
-    "fpp_get_timings", "fpp_set_tuning", "fpp_set_precision", "fpp_autotune", "fpp_query_batch_device",
+    "fpp_get_timings", "fpp_set_tuning", "fpp_set_precision", "fpp_autotune", "fpp_autotune_batch",
+    "fpp_query_batch_device",
     "fpp_get_query_times", "fpp_load_txt", "fpp_write_topk_txt", "fpp_topk_file_name", "fpp_save", "fpp_load",
 )
 PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
@@ -38,6 +39,18 @@ class FppError(RuntimeError):
         self.code = code
 
 
+def share_hip_runtime():
+    """Import torch, when installed, before the library loads.  torch carries
+    its own HIP runtime (loaded as libamdhip64.so); the library asks for
+    libamdhip64.so.7.  Loaded first, torch's runtime serves both.  Loaded
+    after the library, torch would load a second runtime into the process,
+    and that one finds no GPU."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load():
     """Load libfalconnpp_amd.so (raises if it has not been built)."""
     global _lib
@@ -46,6 +59,7 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `make -C falconnpp_amd/csrc` "
                           "(or __graft_entry__.build())")
+    share_hip_runtime()
     L = C.CDLL(LIB_PATH)
     vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int
     L.fpp_last_error.restype = C.c_char_p
@@ -72,6 +86,7 @@ def load():
     L.fpp_set_tuning.argtypes = [vp, i32, i32]
     L.fpp_set_precision.argtypes = [vp, i32, i32]
     L.fpp_autotune.argtypes = [vp, vp, i64, i32, i32, vp, vp]
+    L.fpp_autotune_batch.argtypes = [vp, vp, i64, i32, vp, vp, i32, vp, vp]
     L.fpp_get_query_times.argtypes = [vp, vp, vp]
     L.fpp_query_batch_device.argtypes = [vp, vp, i64, i32, vp, vp, i32, vp, vp, vp]
     L.fpp_get_last_candidates.restype = i64
@@ -196,6 +211,17 @@ class Index:
         """Time the launch shapes on these queries and keep the fastest."""
         ch = np.zeros(4, np.int32)
         _check(load().fpp_autotune(self.h, C.c_void_p(dQ_ptr), ldq, nq, k, C.c_void_p(stream or None), _ptr(ch)))
+        return {"gather_threads": int(ch[0]), "query_order": int(ch[1]), "persistent": int(ch[2]),
+                "us_per_batch": int(ch[3])}
+
+    def autotune_batch(self, dQ_ptr: int, ldq: int, nq: int, qprobes, ks, kmax: int, stream: int = 0) -> dict:
+        """As autotune, timing batched calls with these per-query qProbes and k."""
+        qp = np.ascontiguousarray(qprobes, dtype=np.int32)
+        kk = np.ascontiguousarray(ks, dtype=np.int32)
+        assert len(qp) == nq and len(kk) == nq
+        ch = np.zeros(4, np.int32)
+        _check(load().fpp_autotune_batch(self.h, C.c_void_p(dQ_ptr), ldq, nq, _ptr(qp), _ptr(kk), kmax,
+                                         C.c_void_p(stream or None), _ptr(ch)))
         return {"gather_threads": int(ch[0]), "query_order": int(ch[1]), "persistent": int(ch[2]),
                 "us_per_batch": int(ch[3])}
 

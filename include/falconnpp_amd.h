@@ -125,11 +125,16 @@ int fpp_set_precision(fpp_index* h, int mode, int refine);
  * the batch size and buffers stay the same).  None changes any result. */
 enum { FPP_TUNE_QUERY_ORDER = 1, FPP_TUNE_GATHER_THREADS = 2, FPP_TUNE_PERSISTENT = 3, FPP_TUNE_GRAPH = 4 };
 int fpp_set_tuning(fpp_index* h, int which, int value);
-/* Autotuner (no reference counterpart): times every combination of the
- * gather workgroup size, query order and persistent launch on the given
- * device queries and keeps the fastest.  chosen (may be NULL) receives
- * {threads, order, persistent, microseconds per batch}. */
+/* Autotuner (no reference counterpart): times the gather workgroup sizes,
+ * with one workgroup per query (query order off and on) and with the
+ * persistent launch, on the given device queries, and keeps the fastest.
+ * chosen (may be NULL) receives {threads, order, persistent, microseconds
+ * per batch}.  fpp_autotune_batch times batched calls (qprobes, ks: HOST
+ * arrays of nq, as fpp_query_batch_device takes), so that a sample of a
+ * mixed batch tunes for that mix. */
 int fpp_autotune(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, void* stream, int32_t* chosen);
+int fpp_autotune_batch(fpp_index* h, const float* dQ, int64_t ldq, int nq, const int32_t* qprobes,
+                       const int32_t* ks, int kmax, void* stream, int32_t* chosen);
 /* Batched serving (no reference counterpart; each query is
  * query2Layers_1D, src/FalconnPP.cpp:310-707, with its own qProbes and k):
  * qprobes and ks are HOST arrays of nq, dQ/d_out/d_scores device pointers.

@@ -67,3 +67,18 @@ def test_pybind_module_imports_with_reference_names():
     if _no_gpu():
         with pytest.raises(RuntimeError):
             cls(100, 8)
+
+
+@pytest.mark.parametrize("entry", ["from falconnpp_amd import engine; engine.load()",
+                                   "import falconnpp_amd; falconnpp_amd.FalconnPP"])
+def test_one_hip_runtime_per_process(entry):
+    """Loading the library before torch leaves one HIP runtime mapped (torch's,
+    imported first by the loaders); a second copy would see no GPU."""
+    _ensure_built()
+    pytest.importorskip("torch")
+    code = (entry + "\nimport torch\n"
+            "maps = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
+            "print(len(maps))\n")
+    out = subprocess.run(["python3", "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split()[-1] == "1", out.stdout

@@ -507,3 +507,24 @@ def test_autotune_keeps_results():
         g.set_tuning(E.TUNE_PERSISTENT, 0)
         g.set_tuning(E.TUNE_QUERY_ORDER, 1)
         g.set_tuning(E.TUNE_GATHER_THREADS, 512)
+
+
+def test_autotune_batch_keeps_results():
+    """The batched autotuner (sample with its own qProbes and k per query)
+    changes launch shapes only: the batch answers as before."""
+    import torch
+    X, Q, g, qps, ks = _batch_case()
+    kmax = int(ks.max())
+    ref, rsc = _run_batch(g, Q, qps, ks, kmax)
+    dQ = torch.from_numpy(np.ascontiguousarray(Q)).cuda()
+    m = min(len(Q), 256)
+    ch = g.autotune_batch(dQ.data_ptr(), Q.shape[1], m, qps[:m], ks[:m], kmax)
+    assert ch["gather_threads"] in (256, 512, 1024) and ch["us_per_batch"] > 0
+    assert ch["persistent"] in (0, 1) and (ch["persistent"] == 0 or ch["query_order"] == 0)
+    try:
+        out, sc = _run_batch(g, Q, qps, ks, kmax)
+        assert np.array_equal(out, ref) and np.array_equal(sc, rsc)
+    finally:
+        g.set_tuning(E.TUNE_PERSISTENT, 0)
+        g.set_tuning(E.TUNE_QUERY_ORDER, 1)
+        g.set_tuning(E.TUNE_GATHER_THREADS, 512)
