@@ -23,6 +23,14 @@ using namespace gather;
 constexpr int PS_T = 256;
 constexpr int PS_WAVES = PS_T / WAVE;
 
+__device__ __forceinline__ void wave_min_max(uint32_t& mn, uint32_t& mx) {
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor(mn, off, WAVE));
+        mx = max(mx, (uint32_t)__shfl_xor(mx, off, WAVE));
+    }
+}
+
 // LDS (dynamic): pool keys + buckets (L*perTable each), per-wave histograms,
 // block histogram.
 template <bool FULLPOOL>
@@ -54,8 +62,6 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
         s_tie = 0;
         s_ctot = 0;
     }
-    if constexpr (FULLPOOL)
-        for (int l = threadIdx.x; l < g.L; l += PS_T) tcnt[l] = 0;
     __syncthreads();
     uint32_t* hw = whist + w * 256;
     bool tie = false;
@@ -73,93 +79,147 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     };
     constexpr int KR = 16;  // pairs per lane: PL <= 31, so PL * PL <= 16 * 64
     const int nr = (np + WAVE - 1) / WAVE;
-    float vn;
-    int32_t cn;
-    fetch(w, vn, cn);
-    for (int l = w; l < g.L; l += PS_WAVES) {
-        const float vv = vn;
-        const int32_t cc = cn;
-        fetch(l + PS_WAVES, vn, cn);
-        // key of pair t = r * 64 + lane: fAbsHashValue1 + fAbsHashValue2
-        // (src/FalconnPP.cpp:457-488), in the same order of addition
-        uint32_t key[KR];
+    // t / PL for t < 1024 and PL <= 31 as (t * pl_mul) >> 16 (exact there)
+    const uint32_t pl_mul = 65536u / (uint32_t)PL + 1u;
+    if constexpr (FULLPOOL) {
+        // key of pair t = i * PL + j: fAbsHashValue1 + fAbsHashValue2
+        // (src/FalconnPP.cpp:457-488), in the same order of addition.  The
+        // lists of PF tables are fetched together, so a wave waits for
+        // memory once per PF tables
+        constexpr int PF = 8;
+        for (int l0 = w; l0 < g.L; l0 += PF * PS_WAVES) {
+            float vb[PF];
+            int32_t cb[PF];
 #pragma unroll
-        for (int r = 0; r < KR; ++r) {
-            key[r] = 0xffffffffu;
-            if (r < nr) {
-                const int t = r * WAVE + lane;
-                const int i = t < np ? t / PL : 0, j = t < np ? t - i * PL : 0;
-                const float a = __shfl(vv, i, WAVE), b = __shfl(vv, PL + j, WAVE);
-                if (t < np) key[r] = desc_key(a + b);
-                if constexpr (FULLPOOL) {
-                    const int32_t ci = __shfl(cc, i, WAVE), cj = __shfl(cc, PL + j, WAVE);
+            for (int u = 0; u < PF; ++u) fetch(l0 + u * PS_WAVES, vb[u], cb[u]);
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int l = l0 + u * PS_WAVES;
+                if (l >= g.L) break;
+                for (int r = 0; r < nr; ++r) {
+                    const int t = r * WAVE + lane;
+                    const int i = t < np ? (int)(((uint32_t)t * pl_mul) >> 16) : 0, j = t < np ? t - i * PL : 0;
+                    const float a = __shfl(vb[u], i, WAVE), b = __shfl(vb[u], PL + j, WAVE);
+                    const int32_t ci = __shfl(cb[u], i, WAVE), cj = __shfl(cb[u], PL + j, WAVE);
                     if (t < np) {
-                        pkey[l * np + t] = key[r];
+                        pkey[l * np + t] = desc_key(a + b);
                         pbkt[l * np + t] =
                             (uint32_t)l * (uint32_t)g.NB + (uint32_t)ci * (uint32_t)g.two_np + (uint32_t)cj;
                     }
                 }
             }
         }
-        if constexpr (FULLPOOL) continue;
-        // perTable-th largest pair sum
-        uint32_t prefix = 0, pmask = 0, k = (uint32_t)perTable, eqc = 0;
-        for (int shift = 24; shift >= 0; shift -= 8) {
+    }
+    if constexpr (!FULLPOOL) {
+        float vn;
+        int32_t cn;
+        fetch(w, vn, cn);
+        for (int l = w; l < g.L; l += PS_WAVES) {
+            const float vv = vn;
+            const int32_t cc = cn;
+            fetch(l + PS_WAVES, vn, cn);
+            // key of pair t = r * 64 + lane, as above
+            uint32_t key[KR];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) hw[lane * 4 + i] = 0;
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int r = 0; r < KR; ++r)
-                if (r < nr && r * WAVE + lane < np && (key[r] & pmask) == prefix)
-                    atomicAdd(&hw[(key[r] >> shift) & 255u], 1u);
-            __builtin_amdgcn_wave_barrier();
-            int d;
-            uint32_t below;
-            wave_find_digit(hw, k, d, below, eqc);
-            prefix |= (uint32_t)d << shift;
-            pmask |= 255u << shift;
-            k -= below;
-            __builtin_amdgcn_wave_barrier();
-        }
-        if (np > perTable && eqc != k) tie = true;
-        const uint32_t T = prefix;
-        uint32_t written = 0, eq_seen = 0;
-#pragma unroll
-        for (int r = 0; r < KR; ++r) {
-            if (r >= nr) continue;
-            const int t = r * WAVE + lane;
-            const bool valid = t < np;
-            const int i = valid ? t / PL : 0, j = valid ? t - i * PL : 0;
-            const int32_t ci = __shfl(cc, i, WAVE), cj = __shfl(cc, PL + j, WAVE);
-            const bool lt = valid && key[r] < T;
-            const bool eqk = valid && key[r] == T;
-            const uint64_t meq = __ballot(eqk);
-            const bool take_eq = eqk && (eq_seen + (uint32_t)__popcll(meq & bmask)) < k;
-            const bool take = lt || take_eq;
-            const uint64_t mtake = __ballot(take);
-            const uint32_t slot = written + (uint32_t)__popcll(mtake & bmask);
-            if (take && slot < (uint32_t)perTable) {
-                const int idx = l * perTable + (int)slot;
-                pkey[idx] = key[r];
-                pbkt[idx] = (uint32_t)l * (uint32_t)g.NB + (uint32_t)ci * (uint32_t)g.two_np + (uint32_t)cj;
+            for (int r = 0; r < KR; ++r) {
+                key[r] = 0xffffffffu;
+                if (r < nr) {
+                    const int t = r * WAVE + lane;
+                    const int i = t < np ? (int)(((uint32_t)t * pl_mul) >> 16) : 0, j = t < np ? t - i * PL : 0;
+                    const float a = __shfl(vv, i, WAVE), b = __shfl(vv, PL + j, WAVE);
+                    if (t < np) key[r] = desc_key(a + b);
+                }
             }
-            written += (uint32_t)__popcll(mtake);
-            eq_seen += (uint32_t)__popcll(meq);
+            // perTable-th largest pair sum
+            uint32_t prefix = 0, pmask = 0, k = (uint32_t)perTable, eqc = 0;
+            for (int shift = 24; shift >= 0; shift -= 8) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) hw[lane * 4 + i] = 0;
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int r = 0; r < KR; ++r)
+                    if (r < nr && r * WAVE + lane < np && (key[r] & pmask) == prefix)
+                        atomicAdd(&hw[(key[r] >> shift) & 255u], 1u);
+                __builtin_amdgcn_wave_barrier();
+                int d;
+                uint32_t below;
+                wave_find_digit(hw, k, d, below, eqc);
+                prefix |= (uint32_t)d << shift;
+                pmask |= 255u << shift;
+                k -= below;
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (np > perTable && eqc != k) tie = true;
+            const uint32_t T = prefix;
+            uint32_t written = 0, eq_seen = 0;
+#pragma unroll
+            for (int r = 0; r < KR; ++r) {
+                if (r >= nr) continue;
+                const int t = r * WAVE + lane;
+                const bool valid = t < np;
+                const int i = valid ? (int)(((uint32_t)t * pl_mul) >> 16) : 0, j = valid ? t - i * PL : 0;
+                const int32_t ci = __shfl(cc, i, WAVE), cj = __shfl(cc, PL + j, WAVE);
+                const bool lt = valid && key[r] < T;
+                const bool eqk = valid && key[r] == T;
+                const uint64_t meq = __ballot(eqk);
+                const bool take_eq = eqk && (eq_seen + (uint32_t)__popcll(meq & bmask)) < k;
+                const bool take = lt || take_eq;
+                const uint64_t mtake = __ballot(take);
+                const uint32_t slot = written + (uint32_t)__popcll(mtake & bmask);
+                if (take && slot < (uint32_t)perTable) {
+                    const int idx = l * perTable + (int)slot;
+                    pkey[idx] = key[r];
+                    pbkt[idx] = (uint32_t)l * (uint32_t)g.NB + (uint32_t)ci * (uint32_t)g.two_np + (uint32_t)cj;
+                }
+                written += (uint32_t)__popcll(mtake);
+                eq_seen += (uint32_t)__popcll(meq);
+            }
         }
     }
     if (tie) atomicOr(&s_tie, 1u);
     __syncthreads();
-    __shared__ uint32_t s_eqc, s_cnt, s_over;
+    __shared__ uint32_t s_eqc, s_cnt, s_over, s_mn, s_mx;
     // global top-qProbes of the pool: radix select, the digit found by wave 0;
-    // the cut key (keys <= it are taken) and whether keys on it must be split
+    // the cut key (keys <= it are taken) and whether keys on it must be split.
+    // The pool's keys (pair sums of similar size) share their top bits, so
+    // digits of up to 8 bits start at the highest bit where the smallest and
+    // largest key differ: the first histogram spreads over its bins instead
+    // of piling the pool onto one counter.  Trimmed pairs (0xffffffff, above
+    // every key of a pair sum) stay out: there are always at least
+    // L * perTable >= 2 qProbes others.
     auto global_cut = [&](bool& amb) -> uint32_t {
-        uint32_t prefix = 0, pmask = 0, k = (uint32_t)qProbes, eqc = 0;
-        for (int shift = 24; shift >= 0; shift -= 8) {
+        uint32_t mn = 0xffffffffu, mx = 0u;
+        for (int i = threadIdx.x; i < pool_n; i += PS_T) {
+            const uint32_t key = pkey[i];
+            if (key != 0xffffffffu) {
+                mn = min(mn, key);
+                mx = max(mx, key);
+            }
+        }
+        wave_min_max(mn, mx);
+        if (threadIdx.x == 0) {
+            s_mn = 0xffffffffu;
+            s_mx = 0u;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            atomicMin(&s_mn, mn);
+            atomicMax(&s_mx, mx);
+        }
+        __syncthreads();
+        mn = s_mn;
+        mx = s_mx;
+        // keys <= 0x7fffffff, so top <= 30; all keys equal: one pass on bit 0
+        const int top = mn == mx ? 0 : 31 - __clz(mn ^ mx);
+        uint32_t pmask = ~0u << (top + 1), prefix = mn & pmask, k = (uint32_t)qProbes, eqc = 0;
+        for (int hi = top; hi >= 0; hi -= 8) {
+            const int shift = hi >= 7 ? hi - 7 : 0;
+            const uint32_t dm = (2u << (hi - shift)) - 1u;
             bhist[threadIdx.x] = 0;
             __syncthreads();
             for (int i = threadIdx.x; i < pool_n; i += PS_T) {
                 const uint32_t key = pkey[i];
-                if ((key & pmask) == prefix) atomicAdd(&bhist[(key >> shift) & 255u], 1u);
+                if ((key & pmask) == prefix) atomicAdd(&bhist[(key >> shift) & dm], 1u);
             }
             __syncthreads();
             if (w == 0) {
@@ -174,7 +234,7 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
             }
             __syncthreads();
             prefix |= (uint32_t)s_digit << shift;
-            pmask |= 255u << shift;
+            pmask |= dm << shift;
             k -= s_below;
             eqc = s_eqc;
         }
@@ -192,15 +252,22 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
         // new tables; after kMaxTrim rounds the query is replayed exactly.
         constexpr int kMaxTrim = 6;
         for (int it = 0;; ++it) {
-            for (int l = threadIdx.x; l < g.L; l += PS_T) tcnt[l] = 0;
             if (threadIdx.x == 0) s_over = 0;
             __syncthreads();
-            for (int i = threadIdx.x; i < pool_n; i += PS_T)
-                if (pkey[i] <= T) atomicAdd(&tcnt[i / np], 1u);
-            __syncthreads();
-            bool over = false;
-            for (int l = threadIdx.x; l < g.L; l += PS_T) over |= tcnt[l] > (uint32_t)perTable;
-            if (over) atomicOr(&s_over, 1u);
+            // pairs at or above the cut per table: a wave per table, counted
+            // by ballots (the table's wave below reads its own count)
+            for (int l = w; l < g.L; l += PS_WAVES) {
+                const uint32_t* tk = pkey + l * np;
+                uint32_t c = 0;
+                for (int t0 = 0; t0 < np; t0 += WAVE) {
+                    const int t = t0 + lane;
+                    c += (uint32_t)__popcll(__ballot(t < np && tk[t] <= T));
+                }
+                if (lane == 0) {
+                    tcnt[l] = c;
+                    if (c > (uint32_t)perTable) atomicOr(&s_over, 1u);
+                }
+            }
             __syncthreads();
             if (!s_over) break;
             if (it == kMaxTrim) {
@@ -258,21 +325,32 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     __syncthreads();
     unsigned long long my_ctot = 0;
     uint32_t* out = probes + qv_.probe_off;
-    for (int c0 = 0; c0 < pool_n; c0 += PS_T) {
-        const int i = c0 + threadIdx.x;
-        const bool take = i < pool_n && pkey[i] <= T;
-        const uint64_t m = __ballot(take);
-        if (m == 0ull) continue;
-        const int leader = (int)__ffsll((long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&s_cnt, (uint32_t)__popcll(m));
-        base = __shfl(base, leader, WAVE);
-        const uint32_t pos = base + lane_rank(m);
-        if (take && pos < (uint32_t)qProbes) {
-            const uint32_t b = pbkt[i];
-            out[pos] = b;
-            my_ctot += bucket_off[b + 1] - bucket_off[b];
+    // OU chunks of the pool per step, so their bucket-size loads wait for
+    // memory together
+    constexpr int OU = 8;
+    for (int c0 = 0; c0 < pool_n; c0 += OU * PS_T) {
+        uint64_t lo[OU], hi[OU];
+#pragma unroll
+        for (int u = 0; u < OU; ++u) {
+            lo[u] = hi[u] = 0;
+            const int i = c0 + u * PS_T + (int)threadIdx.x;
+            const bool take = i < pool_n && pkey[i] <= T;
+            const uint64_t m = __ballot(take);
+            if (m == 0ull) continue;
+            const int leader = (int)__ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&s_cnt, (uint32_t)__popcll(m));
+            base = __shfl(base, leader, WAVE);
+            const uint32_t pos = base + lane_rank(m);
+            if (take && pos < (uint32_t)qProbes) {
+                const uint32_t b = pbkt[i];
+                out[pos] = b;
+                lo[u] = bucket_off[b];
+                hi[u] = bucket_off[b + 1];
+            }
         }
+#pragma unroll
+        for (int u = 0; u < OU; ++u) my_ctot += hi[u] - lo[u];
     }
     atomicAdd(&s_ctot, my_ctot);
     __syncthreads();
