@@ -229,22 +229,29 @@ __device__ __forceinline__ void hash_layer(const float* __restrict__ src, const 
         for (int e = 0; e < E; ++e)
             x[e] = ((ob[e / 32] >> (e % 32)) & 1u) ? __int_as_float(0x7fc00000) : x[e];
     }
-    // code of the coordinate holding m[j] (index and sign of the match), for
-    // the first ncode (wave-uniform) entries only
+    // code of the coordinate holding m[j] (index and sign of the last
+    // match), for the first ncode (wave-uniform) entries only.  The signs are
+    // packed 32 coordinates to a word first, so the search compares
+    // magnitudes only; sgn(x) < 0 exactly when x < 0 (src/Utilities.h:26-31),
+    // -0.0 is positive
+    uint32_t neg[WPL];
+#pragma unroll
+    for (int w = 0; w < WPL; ++w) neg[w] = 0u;
+#pragma unroll
+    for (int e = 0; e < E; ++e) neg[e / 32] |= x[e] < 0.0f ? (1u << (e % 32)) : 0u;
 #pragma unroll
     for (int j = 0; j < PM - 1; ++j) {
         c[j] = 0;
         if (j >= ncode) continue;
         int cj = -1;
-        float sj = 0.0f;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const bool hit = fabsf(x[e]) == m[j];
-            cj = hit ? e : cj;
-            sj = hit ? x[e] : sj;
+        for (int e = 0; e < E; ++e) cj = fabsf(x[e]) == m[j] ? e : cj;
+        if (cj >= 0) {
+            uint32_t nw = neg[0];
+#pragma unroll
+            for (int w = 1; w < WPL; ++w) nw = (cj >> 5) == w ? neg[w] : nw;
+            cj = base + cj + (((nw >> (cj & 31)) & 1u) ? n_proj : 0);
         }
-        // sgn(x) < 0 exactly when x < 0 (src/Utilities.h:26-31); -0.0 is positive
-        if (cj >= 0) cj = base + cj + ((sj < 0.0f) ? n_proj : 0);
         c[j] = max(0, lanes_max<LPH>(cj));
     }
 #pragma unroll
@@ -331,7 +338,7 @@ __global__ __launch_bounds__(HB_THREADS) void k_hash_build(HashGeom g, const flo
 }
 
 template <int F, int PM, bool FULL>
-__global__ __launch_bounds__(HB_THREADS) void k_hash_query(HashGeom g, const float* __restrict__ Q, int64_t ldq,
+__global__ __launch_bounds__(HB_THREADS) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 1))) void k_hash_query(HashGeom g, const float* __restrict__ Q, int64_t ldq,
                                                             int nq, int PL, const uint32_t* __restrict__ masks,
                                                             int32_t* __restrict__ qc, float* __restrict__ qv,
                                                             uint32_t* __restrict__ qflag) {
