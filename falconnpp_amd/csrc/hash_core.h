@@ -260,7 +260,7 @@ __device__ __forceinline__ void hash_layer(const float* __restrict__ src, const 
 }
 
 template <int F, int PM, bool FULL>
-__global__ __launch_bounds__(HB_THREADS) void k_hash_build(HashGeom g, const float* __restrict__ Xc, int64_t ldx,
+__global__ __launch_bounds__(HB_THREADS) __attribute__((amdgpu_waves_per_eu(FULL && PM <= 8 ? 2 : 1))) void k_hash_build(HashGeom g, const float* __restrict__ Xc, int64_t ldx,
                                                             int64_t N, int p, const uint32_t* __restrict__ masks,
                                                             uint32_t* __restrict__ codes, float* __restrict__ vals,
                                                             uint64_t* __restrict__ flag_list,
