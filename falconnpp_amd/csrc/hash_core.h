@@ -5,8 +5,9 @@
 // flip followed by an unnormalised Walsh-Hadamard transform of length F, then
 // the top-p |projections| among the first n_proj coordinates.
 //
-// Layout: a hash lives in LPH = F/E adjacent lanes, each holding E = min(F,128)
-// consecutive coordinates in VGPRs.  The butterflies run in the reference's
+// Layout: a hash lives in LPH = F/E adjacent lanes, each holding E consecutive
+// coordinates in VGPRs: E = min(F, 128) for the build, and 64 for the query
+// at F = 128 and 256 (query_lane_coords).  The butterflies run in the reference's
 // ascending-stride order (FFHT == ascending radix-2 butterflies, bit for bit):
 // strides 1..E/2 inside the lane, strides E..F/2 across lanes (DPP quad_perm
 // for lane-xor 1 and 2, ds_bpermute beyond).  The 4 waves of a workgroup share
@@ -33,6 +34,11 @@ namespace fpp {
 namespace hashk {
 
 constexpr int HB_THREADS = 256;
+
+// Coordinates per lane of the query hash: 64 at F = 128 and 256 (two and
+// four lanes per hash; half the registers of 128 per lane, so more waves per
+// SIMD), else as the build.
+constexpr int query_lane_coords(int F) { return (F == 128 || F == 256) ? 64 : (F < 128 ? F : 128); }
 
 template <int S>
 __device__ __forceinline__ float xshfl(float v) {
@@ -174,11 +180,10 @@ __device__ __forceinline__ void out_of_range_bits(int base, int n_proj, uint32_t
 // One layer: rotate the staged vector and return the top-PM |values| (m,
 // descending) and the codes of the first PM-1 of them (c); `tie` is set when
 // two of the first `nchk` values are equal.
-template <int F, int PM, bool FULL>
+template <int F, int PM, bool FULL, int E = (F < 128 ? F : 128)>
 __device__ __forceinline__ void hash_layer(const float* __restrict__ src, const uint32_t* __restrict__ masks, int part,
                                            int n_proj, const uint32_t* ob_in, float (&m)[PM], int (&c)[PM], int nchk,
                                            int ncode, bool& tie) {
-    constexpr int E = F < 128 ? F : 128;
     constexpr int LPH = F / E;
     constexpr int MW = (F + 31) / 32;
     constexpr int WPL = (E + 31) / 32;
@@ -342,7 +347,7 @@ __global__ __launch_bounds__(HB_THREADS) __attribute__((amdgpu_waves_per_eu(FULL
                                                             int nq, int PL, const uint32_t* __restrict__ masks,
                                                             int32_t* __restrict__ qc, float* __restrict__ qv,
                                                             uint32_t* __restrict__ qflag) {
-    constexpr int E = F < 128 ? F : 128;
+    constexpr int E = query_lane_coords(F);
     constexpr int LPH = F / E;
     constexpr int HPW = WAVE / LPH;
     constexpr int MW = (F + 31) / 32;
@@ -374,8 +379,8 @@ __global__ __launch_bounds__(HB_THREADS) __attribute__((amdgpu_waves_per_eu(FULL
         for (int layer = 0; layer < 2; ++layer) {
             float m[PM];
             int c[PM];
-            hash_layer<F, PM, FULL>(src, masks + (size_t)(l * 2 + layer) * 3 * MW, part, g.n_proj, ob, m, c, nchk,
-                                    PL, tie);
+            hash_layer<F, PM, FULL, E>(src, masks + (size_t)(l * 2 + layer) * 3 * MW, part, g.n_proj, ob, m, c,
+                                       nchk, PL, tie);
             if (valid && part == 0) {
                 const size_t o = (((size_t)q * g.L + l) * 2 + layer) * PL;
 #pragma unroll
@@ -390,9 +395,8 @@ __global__ __launch_bounds__(HB_THREADS) __attribute__((amdgpu_waves_per_eu(FULL
     if (valid && part == 0 && tie) atomicOr(qflag + q, 1u);
 }
 
-template <int F>
+template <int F, int E = (F < 128 ? F : 128)>
 inline size_t hash_lds_bytes() {
-    constexpr int E = F < 128 ? F : 128;
     return (size_t)(WAVE / (F / E)) * (F + 1) * sizeof(float);
 }
 
@@ -409,7 +413,7 @@ void launch_build_variant(const HashGeom& g, const float* Xc, int64_t ldx, int64
 template <int F, int PM, bool FULL>
 void launch_query_variant(const HashGeom& g, const float* Q, int64_t ldq, int nq, int PL, const uint32_t* masks,
                           int32_t* qc, float* qv, uint32_t* qflag, hipStream_t st) {
-    constexpr int E = F < 128 ? F : 128;
+    constexpr int E = query_lane_coords(F);
     constexpr int HPW = WAVE / (F / E);
     constexpr int NWV = HB_THREADS / WAVE;
     // split the tables over workgroups until about four waves per SIMD of
@@ -418,8 +422,8 @@ void launch_query_variant(const HashGeom& g, const float* Q, int64_t ldq, int nq
     const int64_t want = ceil_div(4 * 4 * 256, bx * NWV);
     const int gy = (int)std::max<int64_t>(1, std::min<int64_t>(want, ceil_div(g.L, NWV)));
     const dim3 grid((unsigned)bx, (unsigned)gy);
-    k_hash_query<F, PM, FULL><<<grid, HB_THREADS, hash_lds_bytes<F>(), st>>>(g, Q, ldq, nq, PL, masks, qc, qv,
-                                                                            qflag);
+    k_hash_query<F, PM, FULL><<<grid, HB_THREADS, hash_lds_bytes<F, E>(), st>>>(g, Q, ldq, nq, PL, masks, qc, qv,
+                                                                               qflag);
 }
 
 // Per-F entry points, explicitly instantiated in hash_f*.hip (one file per
