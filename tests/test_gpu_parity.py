@@ -46,6 +46,9 @@ CASES = {
     # ... and with ties everywhere (40 per layer; 20 x 1600 lists of 256 KB,
     # read by the merge in place)
     "int_grid_pl40_exact": (lambda: _sym_int_grid(2000, 24, -2, 2, 18), 20, 64, 5, 0.3, 2, 16000, 10, 20),
+    # pool of every pair (64 x 18^2 pairs, 171 KB) above the LDS: the
+    # two-stage probe selection (top-290 per table, then the 9280 of 18,560)
+    "two_stage_l64_pl18": (lambda: synth.lognormal_sphere(20000, 64, 0.3, seed=21), 64, 64, 20, 0.01, 1, 9280, 20, 20),
 }
 
 _cache = {}
