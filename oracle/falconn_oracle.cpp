@@ -10,7 +10,7 @@
 // libstdc++, exactly as the reference, so exact float ties resolve the same way.
 //
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
-// this library, and only as the checker / CPU baseline.  The product never
+// this library, and only to check results or as the CPU baseline.  The product never
 // links it.
 //
 // Parity status of each function (how it is pinned):

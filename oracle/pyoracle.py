@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE.
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg use
-this module, as the checker or as the CPU baseline; the product never imports
+this module, to check results or as the CPU baseline; the product never imports
 it.  The C++ behind it restates the reference's 1D path
 (src/FalconnPP.cpp:21-707, src/FalconnPP.h:52-100) with libstdc++ heaps; see
 the header of falconn_oracle.cpp for how each function is pinned.
