@@ -31,8 +31,9 @@ __device__ __forceinline__ void wave_min_max(uint32_t& mn, uint32_t& mx) {
     }
 }
 
-// LDS (dynamic): pool keys + buckets (L*perTable each), per-wave histograms,
-// block histogram.
+// LDS (dynamic): pool keys; their buckets (two-stage: one per key; full pool:
+// the tables' code lists, L*2*PL, from which a pair's bucket is recomputed);
+// per-wave histograms, block histogram, per-table counts.
 template <bool FULLPOOL>
 __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_t* __restrict__ qc,
                                                        const float* __restrict__ qv,
@@ -49,7 +50,7 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     const int pool_n = g.L * per;
     uint32_t* pkey = lds;
     uint32_t* pbkt = lds + pool_n;
-    uint32_t* whist = pbkt + pool_n;           // PS_WAVES * 256
+    uint32_t* whist = pbkt + (FULLPOOL ? g.L * 2 * PL : pool_n);  // PS_WAVES * 256
     uint32_t* bhist = whist + PS_WAVES * 256;  // 256
     uint32_t* tcnt = bhist + 256;              // L (FULLPOOL)
     __shared__ int s_digit;
@@ -81,6 +82,8 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     const int nr = (np + WAVE - 1) / WAVE;
     // t / PL for t < 1024 and PL <= 31 as (t * pl_mul) >> 16 (exact there)
     const uint32_t pl_mul = 65536u / (uint32_t)PL + 1u;
+    // i / np for i < 2^16 as (i * np_mul) >> 32 (exact there for np <= 961)
+    const uint64_t np_mul = ((1ull << 32) + (uint64_t)np - 1) / (uint64_t)np;
     if constexpr (FULLPOOL) {
         // key of pair t = i * PL + j: fAbsHashValue1 + fAbsHashValue2
         // (src/FalconnPP.cpp:457-488), in the same order of addition.  The
@@ -96,16 +99,12 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
             for (int u = 0; u < PF; ++u) {
                 const int l = l0 + u * PS_WAVES;
                 if (l >= g.L) break;
+                if (lane < 2 * PL) pbkt[l * 2 * PL + lane] = (uint32_t)cb[u];
                 for (int r = 0; r < nr; ++r) {
                     const int t = r * WAVE + lane;
                     const int i = t < np ? (int)(((uint32_t)t * pl_mul) >> 16) : 0, j = t < np ? t - i * PL : 0;
                     const float a = __shfl(vb[u], i, WAVE), b = __shfl(vb[u], PL + j, WAVE);
-                    const int32_t ci = __shfl(cb[u], i, WAVE), cj = __shfl(cb[u], PL + j, WAVE);
-                    if (t < np) {
-                        pkey[l * np + t] = desc_key(a + b);
-                        pbkt[l * np + t] =
-                            (uint32_t)l * (uint32_t)g.NB + (uint32_t)ci * (uint32_t)g.two_np + (uint32_t)cj;
-                    }
+                    if (t < np) pkey[l * np + t] = desc_key(a + b);
                 }
             }
         }
@@ -343,7 +342,19 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
             base = __shfl(base, leader, WAVE);
             const uint32_t pos = base + lane_rank(m);
             if (take && pos < (uint32_t)qProbes) {
-                const uint32_t b = pbkt[i];
+                uint32_t b;
+                if constexpr (FULLPOOL) {
+                    // pair i: table l = i / np, t = i % np, codes of list 1
+                    // entry t / PL and list 2 entry t % PL (exact divisions
+                    // by multiply-shift for i < 2^16)
+                    const uint32_t l = (uint32_t)(((uint64_t)(uint32_t)i * np_mul) >> 32);
+                    const uint32_t t = (uint32_t)i - l * (uint32_t)np;
+                    const uint32_t ii = (t * pl_mul) >> 16, jj = t - ii * (uint32_t)PL;
+                    const uint32_t* cl = pbkt + l * 2 * PL;
+                    b = l * (uint32_t)g.NB + cl[ii] * (uint32_t)g.two_np + cl[PL + jj];
+                } else {
+                    b = pbkt[i];
+                }
                 out[pos] = b;
                 lo[u] = bucket_off[b];
                 hi[u] = bucket_off[b + 1];
@@ -991,7 +1002,7 @@ void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const fl
                          const uint64_t* bucket_off, uint32_t* probes, uint64_t* ctot, uint32_t* qflag,
                          hipStream_t st) {
     const size_t fixed = (size_t)(PS_WAVES * 256 + 256 + g.L) * 4;
-    const size_t lds_full = (size_t)2 * g.L * g.PL * g.PL * 4 + fixed;
+    const size_t lds_full = ((size_t)g.L * g.PL * g.PL + (size_t)g.L * 2 * g.PL) * 4 + fixed;
     const size_t lds = (size_t)2 * g.L * g.perTable * 4 + fixed;
     FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "probe pool (n_tables * perTable) too large for one workgroup");
     if (nq == 0) return;
