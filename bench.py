@@ -1,22 +1,28 @@
 #!/usr/bin/env python3
-"""Falconn++ on one MI355X: query throughput at the recall the index reaches,
-plus index build seconds.
+"""Falconn++ on one MI355X: query throughput at recall@20 >= 0.9, FP32 plain
+against BF16 / FP8 + FP32 refine, plus index build seconds.
 
-Workload (default): N = 1,000,000 points x d = 200 (Gaussian mixture of 1,000
-centres, lognormal(0, 0.25) norms), 10,000 queries from the same mixture;
-n_tables = 100, n_proj = 256, iProbes = 3, alpha = 0.005, bucketLimit = 20,
-qProbes = 4,000, k = 20, index seed 42.  One step = one query() call over the
-whole batch with data, queries and outputs already in HBM.  Recall@k is
-measured against an exact FP64 brute force on the GPU.
+Workload (default, the precision configuration the metric is quoted on):
+N = 10,000,000 points x d = 128 (Gaussian mixture of 5,000 centres, within
+sigma 0.3, lognormal(0, 0.25) norms), 10,000 queries from the same mixture;
+n_tables = 64, n_proj = 128, iProbes = 2, alpha = 0.01, bucketLimit = 20,
+qProbes = 2,000, k = 20, BF16 / FP8 refine of the top 80, index seed 42.
+One step = one query() call over the whole batch with data, queries and
+outputs already in HBM.  Recall@k is measured against an exact FP64 brute
+force on the GPU.
 
-Extra fields on the JSON line:
-  roofline      gather/dedup/rerank kernel, algorithmic bytes per launch over
-                its HIP-event time on the library's stream, against 8 TB/s
+JSON fields beside the contract's:
+  variants      BF16 / FP8 + FP32 refine at the same qProbes: QPS, recall,
+                score error against FP64, their own gather roofline
+  sweep         per precision, recall@20 and QPS over a qProbes grid and the
+                first qProbes reaching recall@20 >= 0.9 (the operating point)
+  roofline      FP32 gather/dedup/rerank kernel: algorithmic bytes per launch
+                over its HIP-event time on the library's stream, against 8 TB/s
   cpu_baseline  the CPU oracle (C++/OpenMP restatement of the reference) on
-                this host: query QPS on a sample of the queries against the
-                same index tables, and build seconds extrapolated from a
-                sample of tables (tables are independent; the reference
-                parallelises over them)
+                this host's cores: query QPS on a sample of the queries
+                against the same index tables, build seconds of one table per
+                core (measured), and with --cpu-full-build every table
+  extra         the N = 1e6 x d = 200 workload (FP32 and both variants)
 """
 import argparse
 import json
@@ -30,26 +36,317 @@ if ROOT not in sys.path:
 
 METRIC = "QPS at recall@20≥0.9, BF16/FP8+refine vs FP32 plain, 1 GPU; index build sec"
 HBM_PEAK_GBS = 8000.0
+CPU_THREADS = 16  # host cores of one GPU's share on the benchmark hosts
+T0 = time.perf_counter()
 
 
 def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
-def profiled_traffic(args, cfg, default_cfg):
-    """HBM bytes per launch of the FP32 gather kernel (FETCH_SIZE + WRITE_SIZE, each
-    scaled by a plain copy's reading) as recorded under profiles/ for this
-    workload; None for a workload that was not profiled."""
-    if args.scale != 1.0 or cfg["qProbes"] != default_cfg["qProbes"]:
-        return None
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"hbm_traffic_gather_{args.config}.md")
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def profiled_traffic(config, kernel):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, each scaled by a plain
+    copy's reading) of a gather kernel as recorded under profiles/ for this
+    workload; None when it was not profiled."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_gather_{config}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch k_gather_score_f32"):
+        if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
             return int(line.rsplit(":", 1)[1])
     return None
+
+
+def gather_roofline(kind, nq, qprobes, d, k, st, ms, refine=0, scale_bytes=0, config=None):
+    """Algorithmic bytes of one gather launch over its time: per query
+    8 qProbes + 4 C_tot + C_uniq (d s + s_scale) + 4 R d + 4 d + 4 k, with
+    s = 4 (FP32), 2 (BF16), 1 (FP8) and the measured C_tot, C_uniq sums."""
+    s = {"f32": 4, "bf16": 2, "fp8": 1}[kind]
+    alg = (8 * qprobes * nq + 4 * st["ctot"] + st["cuniq"] * (d * s + scale_bytes)
+           + nq * (4 * refine * d + 4 * d + 4 * k))
+    achieved = alg / (ms * 1e-3) / 1e9
+    kernel = "k_gather_score_f32" if kind == "f32" else f"k_gather_score_lp_{kind}"
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": profiled_traffic(config, kernel) if config else None,
+            "kernel": kernel, "kernel_ms": round(ms, 3), "alg_bytes_per_launch": int(alg)}
+
+
+class Workload:
+    """One configuration built on the GPU: data, queries, index."""
+
+    def __init__(self, name, args, E, synth, torch):
+        self.name, self.E, self.torch = name, E, torch
+        t = time.perf_counter()
+        self.X, self.Q, self.cfg = synth.make_config(name, args.scale)
+        if args.qprobes and name == args.config:
+            self.cfg["qProbes"] = args.qprobes
+        self.n, self.d = self.X.shape
+        self.nq = len(self.Q)
+        self.k = self.cfg["k"]
+        log(f"{name}: data {self.n}x{self.d}, {self.nq} queries generated in {time.perf_counter() - t:.1f}s")
+        self.dX = torch.from_numpy(self.X).cuda()
+        self.dQ = torch.from_numpy(self.Q).cuda()
+        torch.cuda.synchronize()
+        self.free0, self.total_mem = torch.cuda.mem_get_info()
+        c = self.cfg
+        self.idx = E.Index(self.n, self.d)
+        self.idx.setIndexParam(c["L"], c["n_proj"], c["bucketLimit"], c["alpha"], c["iProbes"], CPU_THREADS,
+                               synth.INDEX_SEED)
+        t0 = time.perf_counter()
+        self.idx.build_device(self.dX.data_ptr(), self.d)
+        torch.cuda.synchronize()
+        self.build_s = time.perf_counter() - t0
+        self.free_built = torch.cuda.mem_get_info()[0]
+        sh, bst = self.idx.shape(), self.idx.stats()
+        log(f"{name}: build {self.build_s:.3f}s, {sh['n_ids']} ids, hash ties {bst['hash_ties']}, "
+            f"bucket ties {bst['bucket_ties']}")
+        self.out = torch.zeros((self.nq, self.k), dtype=torch.int32, device="cuda")
+        self.scores = torch.zeros((self.nq, self.k), dtype=torch.float32, device="cuda")
+        if args.threads:
+            self.idx.set_tuning(E.TUNE_GATHER_THREADS, args.threads)
+        if args.persistent >= 0:
+            self.idx.set_tuning(E.TUNE_PERSISTENT, args.persistent)
+        self.truth = None
+
+    def call(self):
+        self.idx.query_device(self.dQ.data_ptr(), self.d, self.nq, self.k, self.out.data_ptr(),
+                              self.scores.data_ptr())
+
+    def timed(self, steps, warmup, events=False):
+        """QPS and ms per step of `steps` calls after `warmup` untimed ones;
+        with events the per-stage HIP-event times of every step."""
+        self.idx.set_profiling(1 if events else 0)
+        for _ in range(max(1, warmup)):
+            self.call()
+        self.torch.cuda.synchronize()
+        stage = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            self.call()
+            if events:
+                stage.append(self.idx.timings())
+        self.torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        self.idx.set_profiling(0)
+        return self.nq * steps / dt, dt / steps * 1e3, stage
+
+    def gather_ms(self):
+        """The gather kernel's HIP-event time of one more call."""
+        self.idx.set_profiling(1)
+        self.call()
+        ms = float(self.idx.timings()["gather"])
+        self.idx.set_profiling(0)
+        return ms
+
+    def phases(self):
+        self.idx.set_profiling(2)
+        self.call()
+        ph = self.idx.timings()
+        self.idx.set_profiling(0)
+        return {key: round(float(ph[key]), 2) for key in ("ph_gather_us", "ph_compact_us", "ph_score_us",
+                                                          "ph_topk_us")}
+
+    def answer(self):
+        return self.out.cpu().numpy()
+
+    def compute_truth(self, seeds):
+        t = time.perf_counter()
+        self.truth, _ = self.E.bruteforce_f64_device(self.dX.data_ptr(), self.d, self.n, self.d,
+                                                     self.dQ.data_ptr(), self.d, self.nq, self.k, seeds)
+        log(f"{self.name}: FP64 brute force over {self.nq} queries in {time.perf_counter() - t:.1f}s")
+
+    def recall(self):
+        return round(float(self.E.recall(self.truth, self.answer(), self.k)), 4) if self.truth is not None else None
+
+    def memory(self):
+        free_end = self.torch.cuda.mem_get_info()[0]
+        return {"device_total_gb": round(self.total_mem / 1e9, 1),
+                "data_and_queries_gb": round((self.X.nbytes + self.Q.nbytes) / 1e9, 2),
+                "index_gb": round((self.free0 - self.free_built) / 1e9, 2),
+                "query_workspace_and_copies_gb": round((self.free_built - free_end) / 1e9, 2),
+                "used_at_end_gb": round((self.total_mem - free_end) / 1e9, 2)}
+
+    def release(self):
+        self.idx.close()
+        del self.dX, self.dQ, self.out, self.scores
+        self.torch.cuda.empty_cache()
+
+
+PRECISIONS = (("fp32", 0), ("bf16_refine", 1), ("fp8_refine", 2))
+
+
+def run_precisions(w, args, config_key, with_truth):
+    """FP32 at the configured qProbes (the headline), then BF16 and FP8 +
+    refine at the same qProbes."""
+    E, c = w.E, w.cfg
+    w.idx.set_qProbes(c["qProbes"])
+    qps, ms, stage = w.timed(args.steps, args.warmup, events=True)
+    st = w.idx.stats()
+    log(f"{w.name} fp32: {ms:.2f} ms/step, {qps:.0f} QPS, C_tot/q {st['ctot'] / w.nq:.0f}, "
+        f"C_uniq/q {st['cuniq'] / w.nq:.0f}, tie-replayed queries {st['query_ties']}")
+    gms = float(sum(s["gather"] for s in stage) / len(stage))
+    res = {"qps": qps, "ms": ms, "stats": st,
+           "stage_ms": {key: round(float(sum(s[key] for s in stage) / len(stage)), 3)
+                        for key in ("hash", "probe", "replay", "gather")},
+           "roofline": gather_roofline("f32", w.nq, c["qProbes"], w.d, w.k, st, gms, config=config_key)}
+    res["stage_ms"].update(w.phases())
+    if with_truth:
+        w.compute_truth(w.answer())
+        res["recall"] = w.recall()
+        log(f"{w.name} fp32: recall@{w.k} = {res['recall']}")
+    variants = {}
+    if not args.no_variants:
+        refine = int(c.get("refine", 4 * w.k))
+        for name, mode in PRECISIONS[1:]:
+            w.idx.set_precision(mode, refine)
+            vq, vms, _ = w.timed(args.steps, args.warmup)
+            vst = w.idx.stats()
+            shared = w.idx.fp8_shared_scale() if mode == 2 else 0.0
+            sb = 4 if (mode == 2 and shared == 0.0) else 0
+            kind = "bf16" if mode == 1 else "fp8"
+            v = {"value": round(vq, 1), "ms_per_step": round(vms, 3), "refine": refine,
+                 "speedup_vs_fp32": round(vq / qps, 3),
+                 "roofline": gather_roofline(kind, w.nq, c["qProbes"], w.d, w.k, vst, w.gather_ms(), refine, sb,
+                                             config=config_key),
+                 "phases_us": w.phases()}
+            if mode == 2:
+                v["fp8_scale"] = "shared" if shared > 0 else "per row"
+            if w.truth is not None:
+                v["recall_at_k"] = w.recall()
+                v["score_rel_err"] = approx_score_error(w.X, w.Q, w.idx.center(), w.truth, mode, shared)
+            variants[name] = v
+            log(f"{w.name} {name}: {v}")
+        w.idx.set_precision(0, 0)
+    res["variants"] = variants
+    return res
+
+
+def run_sweep(w, args):
+    """Recall@k and QPS over a qProbes grid, per precision; the operating
+    point of a precision is the first qProbes reaching recall@k >= 0.9."""
+    grid = [int(x) for x in args.sweep.split(",") if x]
+    refine = int(w.cfg.get("refine", 4 * w.k))
+    out = {}
+    for name, mode in PRECISIONS:
+        w.idx.set_precision(mode, refine if mode else 0)
+        pts = []
+        op = None
+        for qp in grid:
+            w.idx.set_qProbes(qp)
+            qps, ms, _ = w.timed(max(2, args.steps // 2), 1)
+            r = w.recall()
+            pts.append({"qprobes": qp, "recall": r, "qps": round(qps, 1)})
+            log(f"sweep {name} qProbes={qp}: recall@{w.k} {r}, {qps:.0f} QPS")
+            if r >= 0.9:
+                op = pts[-1]
+                break
+        out[name] = {"points": pts, "operating_point": op}
+    w.idx.set_precision(0, 0)
+    w.idx.set_qProbes(w.cfg["qProbes"])
+    base = out["fp32"]["operating_point"]
+    for name, _ in PRECISIONS[1:]:
+        op = out[name]["operating_point"]
+        out[name]["speedup_vs_fp32_at_recall_0.9"] = (round(op["qps"] / base["qps"], 3)
+                                                      if op and base else None)
+    return out
+
+
+def _bf16(a):
+    import numpy as np
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return (u & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+
+
+def _e4m3(a):
+    """Round to the nearest e4m3 value (3 mantissa bits, min normal 2^-6), ties to even."""
+    import numpy as np
+    a = np.asarray(a, np.float64)
+    e = np.floor(np.log2(np.maximum(np.abs(a), 2.0 ** -6)))
+    step = 2.0 ** (e - 3)
+    return np.round(a / step) * step
+
+
+def approx_score_error(X, Q, center, truth, mode, shared, sample=200):
+    """Relative error of the reduced-precision scores of the true top-k rows
+    against exact FP64 scores q.(x - c), on a sample of queries (host
+    emulation of the engine's rounding; shared = the FP8 scale every row
+    uses, 0 when rows keep their own)."""
+    import numpy as np
+    errs = []
+    for j in range(min(sample, len(Q))):
+        rows = (X[truth[j]] - center).astype(np.float32)
+        q = Q[j].astype(np.float32)
+        exact = rows.astype(np.float64) @ q.astype(np.float64)
+        if mode == 1:
+            approx = _bf16(rows).astype(np.float64) @ _bf16(q).astype(np.float64)
+        else:
+            sr = np.full(len(rows), shared) if shared > 0 else np.maximum(np.abs(rows).max(1), 1e-30) / 240.0
+            sq = max(float(np.abs(q).max()), 1e-30) / 240.0
+            approx = (_e4m3(rows / sr[:, None]) @ _e4m3(q / sq)) * sr * sq
+        errs.append(np.abs(approx - exact) / np.maximum(np.abs(exact), 1e-12))
+    e = np.concatenate(errs)
+    return {"mean": round(float(e.mean()), 6), "p99": round(float(np.quantile(e, 0.99)), 6)}
+
+
+def cpu_baseline(args, w, gpu_ans):
+    """The oracle on CPU_THREADS host threads: its query path on the engine's
+    own tables for a sample of the queries, and its build of one table per
+    thread (every table with --cpu-full-build)."""
+    import numpy as np
+
+    from falconnpp_amd import synth
+    from oracle import pyoracle as P
+
+    cores = min(CPU_THREADS, os.cpu_count() or 1)
+    c = w.cfg
+    center = w.idx.center()
+    off, ids = w.idx.buckets()
+    o = P.OracleIndex(w.n, w.d)
+    o.setIndexParam(c["L"], c["n_proj"], c["bucketLimit"], c["alpha"], c["iProbes"], cores, synth.INDEX_SEED)
+    o.load_index(w.X, center, off, ids)
+    o.set_qProbes(c["qProbes"])
+    del off, ids
+    m = min(args.cpu_queries, w.nq)
+    o.query(w.Q[:8], w.k)  # warm-up
+    t = time.perf_counter()
+    topk, _, _ = o.query(w.Q[:m], w.k)
+    qdt = time.perf_counter() - t
+    agree = float(np.mean([len(set(a.tolist()) & set(b.tolist())) / w.k for a, b in zip(topk, gpu_ans[:m])]))
+    log(f"cpu query: {m} queries in {qdt:.2f}s on {cores} threads ({m / qdt:.1f} QPS), top-k agreement {agree:.4f}")
+    del o
+    lt = c["L"] if args.cpu_full_build else min(cores, c["L"])
+    log(f"cpu build: {lt} of {c['L']} tables on {cores} threads ...")
+    ob = P.OracleIndex(w.n, w.d)
+    ob.setIndexParam(lt, c["n_proj"], c["bucketLimit"], c["alpha"], c["iProbes"], cores, synth.INDEX_SEED)
+    ob.build(w.X, center)
+    bs = ob.build_seconds
+    del ob
+    # tables are independent and the reference builds them in parallel
+    # (src/FalconnPP.cpp:62-64): L tables take ceil(L / cores) rounds of a
+    # round of `cores` tables
+    rounds = -(-c["L"] // cores)
+    est = bs if lt == c["L"] else bs * rounds
+    log(f"cpu build: {lt} tables in {bs:.2f}s -> {est:.1f}s for {c['L']} tables")
+    res = {"value": round(m / qdt, 2), "unit": "queries/s", "cores": cores, "kind": "port",
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+           "sample": (f"oracle query path on the engine's tables for {m} of {w.nq} queries at qProbes "
+                      f"{c['qProbes']}; build of {lt} of {c['L']} tables on {cores} threads"
+                      + ("" if lt == c["L"] else f" (one table per thread), x{rounds} rounds for all tables")),
+           "build_s_measured": round(bs, 3), "build_tables_measured": lt, "build_s": round(est, 2),
+           "build_speedup_gpu": round(est / w.build_s, 1), "topk_agreement": round(agree, 4)}
+    return res
 
 
 def main():
@@ -57,25 +354,28 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--config", default="c3")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink N and nq (testing only)")
     ap.add_argument("--qprobes", type=int, default=0, help="override qProbes")
     ap.add_argument("--threads", type=int, default=0, help="gather workgroup size (0: engine default)")
     ap.add_argument("--persistent", type=int, default=-1, help="persistent gather launch 0/1 (-1: engine default)")
+    ap.add_argument("--sweep", default="2,5,10,20,30,50,75,100,150,200,300,500,1000,2000",
+                    help="qProbes grid of the recall sweep")
+    ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the BF16/FP8 + refine runs")
-    ap.add_argument("--cpu-queries", type=int, default=1000)
-    ap.add_argument("--cpu-tables", type=int, default=16)
+    ap.add_argument("--no-extra", action="store_true", help="skip the N = 1e6 x d = 200 workload")
+    ap.add_argument("--cpu-queries", type=int, default=2000)
+    ap.add_argument("--cpu-full-build", action="store_true", help="time the CPU build of every table")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the engine is single-GPU (multi-GPU sharding is design-only); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the engine runs on one GPU (sharding over several is described "
+              "in the design notes, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     if args.calibrate_copy:
@@ -92,182 +392,75 @@ def main():
     from falconnpp_amd import synth
 
     if "qp_range" in synth.CONFIGS[args.config]:
-        serve_batched(args, E, synth, np, torch)
+        serve_batched(args, E, synth, torch)
         return
 
-    t = time.perf_counter()
-    X, Q, cfg = synth.make_config(args.config, args.scale)
-    if args.qprobes:
-        cfg["qProbes"] = args.qprobes
-    n, d = X.shape
-    nq = len(Q)
-    k = cfg["k"]
-    log(f"data {n}x{d}, {nq} queries generated in {time.perf_counter() - t:.1f}s")
-    dX = torch.from_numpy(X).cuda()
-    dQ = torch.from_numpy(Q).cuda()
-    torch.cuda.synchronize()
-    free0, total_mem = torch.cuda.mem_get_info()
-
-    idx = E.Index(n, d)
-    idx.setIndexParam(cfg["L"], cfg["n_proj"], cfg["bucketLimit"], cfg["alpha"], cfg["iProbes"], 16,
-                      synth.INDEX_SEED)
-    t0 = time.perf_counter()
-    idx.build_device(dX.data_ptr(), d)
-    torch.cuda.synchronize()
-    build_s = time.perf_counter() - t0
-    free_built = torch.cuda.mem_get_info()[0]
-    sh = idx.shape()
-    bst = idx.stats()
-    log(f"build {build_s:.3f}s, {sh['n_ids']} ids, hash ties {bst['hash_ties']}, bucket ties {bst['bucket_ties']}")
-
-    idx.set_qProbes(cfg["qProbes"])
-    if args.threads:
-        idx.set_tuning(E.TUNE_GATHER_THREADS, args.threads)
-    if args.persistent >= 0:
-        idx.set_tuning(E.TUNE_PERSISTENT, args.persistent)
-    out = torch.zeros((nq, k), dtype=torch.int32, device="cuda")
-    scores = torch.zeros((nq, k), dtype=torch.float32, device="cuda")
-    idx.set_profiling(1)
-    for _ in range(args.warmup):
-        idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
-    torch.cuda.synchronize()
-    log("warmup done")
-    stage = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
-        stage.append(idx.timings())
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    qps = nq * args.steps / dt
-    st = idx.stats()
-    log(f"query {dt / args.steps * 1e3:.2f} ms/step, {qps:.0f} QPS, C_tot/q {st['ctot'] / nq:.0f}, "
-        f"C_uniq/q {st['cuniq'] / nq:.0f}, tie-replayed queries {st['query_ties']}")
-
-    # roofline of the gather + dedup + rerank kernel (dominant)
-    gather_ms = float(np.mean([s["gather"] for s in stage]))
-    D = d
-    alg_bytes = (8 * cfg["qProbes"] * nq + 4 * st["ctot"] + 4 * D * st["cuniq"] + nq * (4 * D + 4 * k))
-    achieved = alg_bytes / (gather_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": profiled_traffic(args, cfg, synth.CONFIGS[args.config]),
-                "kernel": "k_gather_score_f32", "kernel_ms": round(gather_ms, 3),
-                "alg_bytes_per_launch": int(alg_bytes)}
-    stages = {key: round(float(np.mean([s[key] for s in stage])), 3) for key in ("hash", "probe", "replay", "gather")}
-    # one more call, outside the timed region, with per-query phase stamps
-    idx.set_profiling(2)
-    idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
-    ph = idx.timings()
-    idx.set_profiling(1)
-    stages.update({key: round(float(ph[key]), 2) for key in ("ph_gather_us", "ph_compact_us", "ph_score_us",
-                                                             "ph_topk_us")})
-
-    res = {}
-    ans32 = out.cpu().numpy()
-    truth = None
-    # recall against the FP64 brute force over the first nrec queries (all of
-    # them up to 10,000)
-    nrec = min(nq, 10_000)
-    if not args.no_recall:
-        t = time.perf_counter()
-        truth, _ = E.bruteforce_f64_device(dX.data_ptr(), d, n, d, dQ.data_ptr(), d, nrec, k, ans32[:nrec])
-        res["recall_at_k"] = round(E.recall(truth, ans32[:nrec], k), 4)
-        if nrec < nq:
-            res["recall_queries"] = nrec
-        log(f"recall@{k} = {res['recall_at_k']} over {nrec} queries (FP64 brute force {time.perf_counter() - t:.1f}s)")
-
-    variants = {}
-    if not args.no_variants:
-        for name, mode in (("bf16_refine", E.PREC_BF16), ("fp8_refine", E.PREC_FP8)):
-            refine = int(cfg.get("refine", 4 * k))
-            idx.set_precision(mode, refine)
-            # timed without stage events; the gather time and the per-query
-            # phases (approximate scoring under ph_score_us, exact refine +
-            # ranking under ph_topk_us) come from calls after the timed loop
-            idx.set_profiling(0)
-            for _ in range(max(1, args.warmup)):
-                idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
-            torch.cuda.synchronize()
-            vdt = time.perf_counter() - t0
-            idx.set_profiling(1)
-            idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
-            vgather = idx.timings()["gather"]
-            v = {"value": round(nq * args.steps / vdt, 1), "ms_per_step": round(vdt / args.steps * 1e3, 3),
-                 "refine": refine, "gather_ms": round(float(vgather), 3),
-                 "speedup_vs_fp32": round((nq * args.steps / vdt) / qps, 3)}
-            idx.set_profiling(2)
-            idx.query_device(dQ.data_ptr(), d, nq, k, out.data_ptr(), scores.data_ptr())
-            vph = idx.timings()
-            idx.set_profiling(1)
-            v["phases_us"] = {key: round(float(vph[key]), 2)
-                              for key in ("ph_gather_us", "ph_compact_us", "ph_score_us", "ph_topk_us")}
-            if truth is not None:
-                v["recall_at_k"] = round(E.recall(truth, out.cpu().numpy()[:nrec], k), 4)
-                v["score_rel_err"] = approx_score_error(X, Q, idx.center(), truth, mode)
-            variants[name] = v
-            log(f"{name}: {v}")
-        idx.set_precision(E.PREC_FP32, 0)
-    # device memory: the resident data and queries (torch), the index (centred
-    # copy, sign bits, CSR tables), then the query workspace and the BF16/FP8
-    # copies made by the calls above
-    free_end = torch.cuda.mem_get_info()[0]
-    memory = {"device_total_gb": round(total_mem / 1e9, 1),
-              "data_and_queries_gb": round((X.nbytes + Q.nbytes) / 1e9, 2),
-              "index_gb": round((free0 - free_built) / 1e9, 2),
-              "query_workspace_and_copies_gb": round((free_built - free_end) / 1e9, 2),
-              "used_at_end_gb": round((total_mem - free_end) / 1e9, 2)}
+    profiled = args.config if args.scale == 1.0 and not args.qprobes else None
+    w = Workload(args.config, args, E, synth, torch)
+    r = run_precisions(w, args, profiled, not args.no_recall)
+    sweep = None
+    if not args.no_sweep and w.truth is not None:
+        sweep = run_sweep(w, args)
+    memory = w.memory()
     log(f"memory: {memory}")
-
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args, X, Q, cfg, idx, ans32)
-
+        w.idx.set_qProbes(w.cfg["qProbes"])
+        w.call()
+        cpu = cpu_baseline(args, w, w.answer())
+    c = w.cfg
+    workload = (f"{args.config}: N={w.n} x d={w.d} "
+                + ("gaussian mixture" if c["kind"] == "mixture" else "lognormal sphere")
+                + f", Q={w.nq}, L={c['L']}, n_proj={c['n_proj']}, iProbes={c['iProbes']}, alpha={c['alpha']}, "
+                  f"bucketLimit={c['bucketLimit']}, qProbes={c['qProbes']}, k={w.k}, plain FP32 rerank; variants "
+                  f"BF16/FP8 + FP32 refine of the top {int(c.get('refine', 4 * w.k))}")
+    build_s = w.build_s
+    st = r["stats"]
+    w.release()
+    extra = None
+    if not args.no_extra and args.config != "c2" and args.scale == 1.0:
+        w2 = Workload("c2", args, E, synth, torch)
+        r2 = run_precisions(w2, args, "c2", not args.no_recall)
+        extra = {"workload": "c2: N=1000000 x d=200 gaussian mixture, Q=10000, L=100, n_proj=256, iProbes=3, "
+                             "alpha=0.005, bucketLimit=20, qProbes=4000, k=20",
+                 "value": round(r2["qps"], 1), "unit": "queries/s", "ms_per_step": round(r2["ms"], 3),
+                 "recall_at_k": r2.get("recall"), "build_s": round(w2.build_s, 3), "roofline": r2["roofline"],
+                 "variants": r2["variants"]}
+        w2.release()
     line = {
         "metric": METRIC,
-        "value": round(qps, 1),
+        "value": round(r["qps"], 1),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "ms_per_step": round(r["ms"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": (f"{args.config}: N={n} x d={d} "
-                         + ("gaussian mixture" if cfg["kind"] == "mixture" else "lognormal sphere")
-                         + f", Q={nq}, L={cfg['L']}, n_proj={cfg['n_proj']}, iProbes={cfg['iProbes']}, "
-                           f"alpha={cfg['alpha']}, bucketLimit={cfg['bucketLimit']}, qProbes={cfg['qProbes']}, k={k}, "
-                           "plain FP32 rerank"),
-            "global_batch": nq,
-            "parallelism": "single GPU",
-        },
+        "config": {"workload": workload, "global_batch": w.nq, "parallelism": "single GPU"},
         "build_s": round(build_s, 3),
-        **res,
-        "stage_ms": stages,
-        "candidates_per_query": {"c_tot": round(st["ctot"] / nq, 1), "c_uniq": round(st["cuniq"] / nq, 1)},
+        "recall_at_k": r.get("recall"),
+        "stage_ms": r["stage_ms"],
+        "candidates_per_query": {"c_tot": round(st["ctot"] / w.nq, 1), "c_uniq": round(st["cuniq"] / w.nq, 1)},
         "memory": memory,
-        "roofline": roofline,
-        "variants": variants,
+        "roofline": r["roofline"],
+        "variants": r["variants"],
+        "sweep": sweep,
         "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(line), flush=True)
 
 
-def serve_batched(args, E, synth, np, torch):
+def serve_batched(args, E, synth, torch):
     """Batched serving: one call over all queries, each with its own qProbes
-    and k, hashing and probe selection replayed from a hipGraph; the gather's
-    launch shape (workgroup size, one workgroup per query or the persistent
-    work queue) picked by the engine's autotuner on a sample of the batch.
-    Reports QPS and per-query latency."""
+    and k, captured in a hipGraph; the gather's launch shape picked by the
+    engine's autotuner on the batch.  Reports QPS and per-query latency."""
+    import numpy as np
+
     t = time.perf_counter()
     X, Q, cfg = synth.make_config(args.config, args.scale)
     n, d = X.shape
@@ -279,21 +472,18 @@ def serve_batched(args, E, synth, np, torch):
     dQ = torch.from_numpy(Q).cuda()
     torch.cuda.synchronize()
     idx = E.Index(n, d)
-    idx.setIndexParam(cfg["L"], cfg["n_proj"], cfg["bucketLimit"], cfg["alpha"], cfg["iProbes"], 16,
+    idx.setIndexParam(cfg["L"], cfg["n_proj"], cfg["bucketLimit"], cfg["alpha"], cfg["iProbes"], CPU_THREADS,
                       synth.INDEX_SEED)
     t0 = time.perf_counter()
     idx.build_device(dX.data_ptr(), d)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     log(f"build {build_s:.3f}s")
-    # launch shape autotuned on the batch itself (its own qProbes and k per
-    # query; a 4,096-query sample picked per-query launches, which the full
-    # batch runs 5% slower than the persistent launch), unless --persistent
-    # fixes the launch
-    sample = nq
+    idx.set_tuning(E.TUNE_VERBOSE, 1)
 
     def tune():
-        t = idx.autotune_batch(dQ.data_ptr(), d, sample, qps_q[:sample], ks[:sample], kmax)
+        log("autotune on the batch ...")
+        t = idx.autotune_batch(dQ.data_ptr(), d, nq, qps_q, ks, kmax)
         if args.persistent >= 0:
             idx.set_tuning(E.TUNE_PERSISTENT, args.persistent)
             t["persistent"] = args.persistent
@@ -304,7 +494,6 @@ def serve_batched(args, E, synth, np, torch):
     idx.set_tuning(E.TUNE_GRAPH, 1)
     out = torch.zeros((nq, kmax), dtype=torch.int32, device="cuda")
     sc = torch.zeros((nq, kmax), dtype=torch.float32, device="cuda")
-    idx.set_profiling(1)
 
     def call():
         idx.query_batch_device(dQ.data_ptr(), d, nq, qps_q, ks, kmax, out.data_ptr(), sc.data_ptr())
@@ -312,6 +501,7 @@ def serve_batched(args, E, synth, np, torch):
     def timed_run():
         """Warm-up, the timed steps, then one more call with per-query
         stamps; returns (QPS, ms per step, stage ms, latency)."""
+        idx.set_profiling(1)
         for _ in range(max(1, args.warmup)):
             call()
         torch.cuda.synchronize()
@@ -324,12 +514,11 @@ def serve_batched(args, E, synth, np, torch):
         dt = time.perf_counter() - t0
         stages = {key: round(float(np.mean([s[key] for s in stage])), 3)
                   for key in ("hash", "probe", "replay", "gather")}
-        # per-query latency, from the gather's per-query stamps
         idx.set_profiling(2)
         call()
         service, done = idx.query_times(nq)
         idx.set_profiling(1)
-        pre_ms = stages["hash"] + stages["probe"]
+        pre_ms = stages["hash"] + stages["probe"] + stages["replay"]
         lat = {"service_us_p50": round(float(np.percentile(service, 50)), 1),
                "service_us_p99": round(float(np.percentile(service, 99)), 1),
                "completion_ms_p50": round(pre_ms + float(np.percentile(done, 50)) / 1e3, 3),
@@ -345,15 +534,11 @@ def serve_batched(args, E, synth, np, torch):
     truth_by_k = {}
 
     def recall_by_k(ans):
-        by_k = {}
-        for kv, (sel, truth) in truth_by_k.items():
-            by_k[kv] = round(float(E.recall(truth, ans[sel, :kv], kv)), 4)
-        return by_k
+        return {kv: round(float(E.recall(truth, ans[sel, :kv], kv)), 4) for kv, (sel, truth) in truth_by_k.items()}
 
     if not args.no_recall:
         # recall@k_q against FP64 brute force, per k on a sample of at most
-        # 2000 queries each; the brute force is seeded with the answer's
-        # first k ids of those queries (all valid rows)
+        # 2000 queries each, seeded with the answer's first k ids
         ans = out.cpu().numpy()
         t = time.perf_counter()
         for kv in cfg["k_choices"]:
@@ -368,8 +553,6 @@ def serve_batched(args, E, synth, np, torch):
         log(f"recall@k by k = {res['recall_at_k_by_k']} (FP64 brute force {time.perf_counter() - t:.1f}s)")
     variants = {}
     if not args.no_variants:
-        # the same batch scored in BF16 / FP8 on MFMA, min(256, 4 k_q)
-        # survivors per query rescored exactly in FP32
         for name, mode in (("bf16_refine", E.PREC_BF16), ("fp8_refine", E.PREC_FP8)):
             idx.set_precision(mode, 0)
             vtuned = tune()
@@ -409,96 +592,6 @@ def serve_batched(args, E, synth, np, torch):
         "variants": variants,
     }
     print(json.dumps(line), flush=True)
-
-
-def _bf16(a):
-    import numpy as np
-    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
-    return (u & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
-
-
-def _e4m3(a):
-    """Round to the nearest e4m3 value (3 mantissa bits, min normal 2^-6), ties to even."""
-    import numpy as np
-    a = np.asarray(a, np.float64)
-    e = np.floor(np.log2(np.maximum(np.abs(a), 2.0 ** -6)))
-    step = 2.0 ** (e - 3)
-    return np.round(a / step) * step
-
-
-def approx_score_error(X, Q, center, truth, mode, sample=200):
-    """Relative error of the reduced-precision scores of the true top-k rows
-    against exact FP64 scores q.(x - c), on a sample of queries (host
-    emulation of the engine's rounding)."""
-    import numpy as np
-    shared = 0.0
-    if mode == 2:
-        # the engine's FP8 scale rule: one scale amax_max / 240 for all rows
-        # when every nonzero row's largest magnitude is within 2^8 of the
-        # largest, else amax_row / 240 per row
-        hi, lo = 0.0, np.inf
-        for s0 in range(0, len(X), 1 << 20):
-            am = np.abs(X[s0:s0 + (1 << 20)] - center).max(1)
-            hi = max(hi, float(am.max()))
-            nz = am[am > 0]
-            if len(nz):
-                lo = min(lo, float(nz.min()))
-        if hi > 0 and hi <= 256.0 * lo:
-            shared = hi / 240.0
-    errs = []
-    for j in range(min(sample, len(Q))):
-        rows = (X[truth[j]] - center).astype(np.float32)
-        q = Q[j].astype(np.float32)
-        exact = rows.astype(np.float64) @ q.astype(np.float64)
-        if mode == 1:
-            approx = _bf16(rows).astype(np.float64) @ _bf16(q).astype(np.float64)
-        else:
-            sr = np.full(len(rows), shared) if shared > 0 else np.maximum(np.abs(rows).max(1), 1e-30) / 240.0
-            sq = max(float(np.abs(q).max()), 1e-30) / 240.0
-            approx = (_e4m3(rows / sr[:, None]) @ _e4m3(q / sq)) * sr * sq
-        errs.append(np.abs(approx - exact) / np.maximum(np.abs(exact), 1e-12))
-    e = np.concatenate(errs)
-    return {"mean": round(float(e.mean()), 6), "p99": round(float(np.quantile(e, 0.99)), 6)}
-
-
-def cpu_baseline(args, X, Q, cfg, idx, gpu_ans):
-    import numpy as np
-
-    from falconnpp_amd import synth
-    from oracle import pyoracle as P
-
-    cores = min(16, os.cpu_count() or 1)
-    n, d = X.shape
-    center = idx.center()
-    # query: the oracle's query path on the engine's own tables
-    off, ids = idx.buckets()
-    o = P.OracleIndex(n, d)
-    o.setIndexParam(cfg["L"], cfg["n_proj"], cfg["bucketLimit"], cfg["alpha"], cfg["iProbes"], cores,
-                    synth.INDEX_SEED)
-    o.load_index(X, center, off, ids)
-    o.set_qProbes(cfg["qProbes"])
-    m = min(args.cpu_queries, len(Q))
-    o.query(Q[:8], cfg["k"])  # warm-up
-    t = time.perf_counter()
-    topk, _, _ = o.query(Q[:m], cfg["k"])
-    qdt = time.perf_counter() - t
-    agree = float(np.mean([len(set(a.tolist()) & set(b.tolist())) / cfg["k"] for a, b in zip(topk, gpu_ans[:m])]))
-    log(f"cpu query: {m} queries in {qdt:.2f}s on {cores} threads ({m / qdt:.1f} QPS), top-k agreement {agree:.4f}")
-    del o
-    # build: a sample of the tables, extrapolated over n_tables
-    lt = min(args.cpu_tables, cfg["L"])
-    ob = P.OracleIndex(n, d)
-    ob.setIndexParam(lt, cfg["n_proj"], cfg["bucketLimit"], cfg["alpha"], cfg["iProbes"], cores, synth.INDEX_SEED)
-    ob.build(X, center)
-    bs = ob.build_seconds
-    est = bs * (cfg["L"] / lt) if lt >= cores else bs * cfg["L"] / lt
-    log(f"cpu build: {lt} tables in {bs:.2f}s on {cores} threads -> {est:.1f}s for {cfg['L']} tables")
-    del ob
-    return {"value": round(m / qdt, 2), "unit": "queries/s", "cores": cores, "kind": "port",
-            "sample": (f"oracle query path on the engine's tables for {m} of {len(Q)} queries; build timed on "
-                       f"{lt} of {cfg['L']} tables and scaled by {cfg['L']}/{lt}"),
-            "build_s_est": round(est, 2), "build_s_sample": round(bs, 3), "topk_agreement": round(agree, 4)}
 
 
 if __name__ == "__main__":

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <random>
 #include <string>
@@ -65,6 +66,8 @@ struct Index {
     DevBuf<uint64_t> bucket_off;
     DevBuf<int32_t> ids;
     int64_t n_ids = 0;
+    DevBuf<uint32_t> pre_cnt;  // bucket sizes before the alpha / bucketLimit filter (built indexes)
+    bool have_pre_cnt = false;
     std::vector<uint8_t> bit1, bit2;
 
     // query scratch
@@ -106,6 +109,7 @@ struct Index {
     // hashing + probe selection of a batch captured in a hipGraph, replayed
     // while the batch shape and buffers stay the same
     bool use_graph = false;
+    bool tune_verbose = false;  // autotuner prints each timed shape
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uint64_t> graph_key;
@@ -123,12 +127,15 @@ struct Index {
     DevBuf<uint64_t> phase_ts;
 
     int64_t stat_hash_ties = 0, stat_bucket_ties = 0, stat_ctot = 0, stat_cuniq = 0, stat_qties = 0;
+    int stat_probe_path = -1;  // probe selection of the last query call (PROBE_*)
 
     // reduced-precision scoring: 0 = FP32 (reference semantics), 1 = BF16,
     // 2 = FP8-e4m3 + per-row scale; survivors rescored exactly in FP32
     int precision = 0;
     int refine = 0;  // 0: 4 * n_neighbors
     int lp_ready = 0;  // format of the current low-precision copy
+    bool fp8_per_row = false;      // FP8 rows keep their own scale even when one could be shared
+    bool lp_per_row_made = false;  // the FP8 copy was made with that setting
     int64_t ldl = 0;
     DevBuf<uint16_t> Xb;
     DevBuf<uint8_t> X8;
@@ -308,7 +315,9 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
     }
     // bucket sizes, limits and CSR offsets over all tables
     const int64_t nbk = (int64_t)L * NB;
-    DevBuf<uint32_t> cnt, ksz;
+    // pre-filter bucket sizes stay with the index (inspection)
+    DevBuf<uint32_t>& cnt = I.pre_cnt;
+    DevBuf<uint32_t> ksz;
     cnt.reserve(nbk);
     ksz.reserve(nbk);
     FPP_HIP(hipMemsetAsync(cnt.get(), 0, nbk * 4, st));
@@ -391,6 +400,7 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
     I.scratch_word.reserve(1);
     I.max_bucket = max_bucket_size(I.bucket_off.get(), (int64_t)L * NB, I.scratch_word.get(), st);
     I.stat_bucket_ties = bucket_ties;
+    I.have_pre_cnt = true;
     I.build_gen++;
     I.drop_graph();
     I.built = true;
@@ -399,7 +409,7 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
 // Low-precision copy of the centred data for the selected format, made once
 // per build.
 void ensure_lowprec(Index& I, hipStream_t st) {
-    if (I.lp_ready == I.precision) return;
+    if (I.lp_ready == I.precision && (I.precision != 2 || I.lp_per_row_made == I.fp8_per_row)) return;
     const int64_t N = I.n_points;
     // rows padded to whole 128-byte lines: a scoring load instruction reads
     // 8 rows' lines whole (FP8: 128 elements, BF16: 64 per line)
@@ -415,7 +425,8 @@ void ensure_lowprec(Index& I, hipStream_t st) {
         I.s8.reserve(N);
         I.scratch_word.reserve(2);
         I.s8_shared = launch_quant_fp8(I.Xc.get(), I.ldx, N, I.n_features, I.ldl, I.X8.get(), I.s8.get(),
-                                       I.scratch_word.get(), st);
+                                       I.scratch_word.get(), I.fp8_per_row, st);
+        I.lp_per_row_made = I.fp8_per_row;
     }
     I.lp_ready = I.precision;
 }
@@ -498,6 +509,10 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         pg.qp = I.qparams.get();
     }
     const int L = I.n_tables, PL = pl.perLayer;
+    // a pool that fits no probe-selection kernel: every query's probes come
+    // from the exact replay, which reuses the fast per-layer lists
+    const int probe_path = exact_only ? PROBE_EXACT : probe_select_path(pg);
+    I.stat_probe_path = probe_path;
     I.Qpad.reserve((size_t)nq * I.ldx);
     I.qc.reserve((size_t)nq * L * 2 * PL);
     I.qv.reserve((size_t)nq * L * 2 * PL);
@@ -511,17 +526,18 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         launch_fill_u32(reinterpret_cast<uint32_t*>(I.Qpad.get()), 0u, (int)((size_t)nq * I.ldx), st);
         FPP_HIP(hipMemcpy2DAsync(I.Qpad.get(), I.ldx * 4, dQ, ldq * 4, (size_t)I.n_features * 4, nq,
                                  hipMemcpyDeviceToDevice, st));
-        launch_fill_u32(I.qflag.get(), exact_only ? 3u : 0u, nq, st);
+        // bit 0: per-layer lists rebuilt by the replay; bit 1: probes replayed
+        launch_fill_u32(I.qflag.get(), exact_only ? 3u : (probe_path == PROBE_EXACT ? 2u : 0u), nq, st);
         if (marks) I.mark(0, st);
         if (!exact_only)
             launch_hash_query(hg, I.Qpad.get(), I.ldx, nq, PL, I.masks.get(), I.qc.get(), I.qv.get(),
                               I.qflag.get(), st);
         if (marks) I.mark(1, st);
-        if (!exact_only)
+        if (probe_path != PROBE_EXACT)
             launch_probe_select(pg, nq, I.qc.get(), I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(),
                                 I.qflag.get(), st);
     };
-    if (I.use_graph && !exact_only) {
+    if (I.use_graph && probe_path != PROBE_EXACT) {
         const std::vector<uint64_t> key = {
             (uint64_t)nq, (uint64_t)ldq, (uint64_t)(uintptr_t)dQ, (uint64_t)(uintptr_t)st, I.build_gen,
             (uint64_t)(uintptr_t)I.Qpad.get(), (uint64_t)(uintptr_t)I.qc.get(), (uint64_t)(uintptr_t)I.qv.get(),
@@ -564,27 +580,12 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     const uint32_t nl = (uint32_t)lst.size();
     const int nu = nq - (int)nl;
     I.stat_qties = (int64_t)nl;
-    // candidate segments: C_tot for the others; for a flagged query (its
-    // probes are replayed later) the qProbes largest buckets bound C_tot
-    std::vector<uint64_t> off(nq + 1);
-    off[0] = 0;
-    for (int q = 0; q < nq; ++q) {
-        const uint64_t qpq = batched ? (uint64_t)I.hqp[q].qProbes : (uint64_t)I.qProbes;
-        const uint64_t flag_cap = std::min<uint64_t>(qpq * I.max_bucket, (uint64_t)I.n_points);
-        off[q + 1] = off[q] + (flag[q] ? flag_cap : std::min<uint64_t>(ct[q], (uint64_t)I.n_points));
-    }
-    I.cand_off.reserve(nq + 1);
-    I.cand.reserve(off[nq]);
-    I.score.reserve(off[nq]);
-    I.ncand.reserve(nq);
-    FPP_HIP(hipMemcpyAsync(I.cand_off.get(), off.data(), (nq + 1) * 8, hipMemcpyHostToDevice, st));
     GatherArgs ga;
     ga.N = I.n_points;
     ga.D = I.n_features;
     ga.ldx = I.ldx;
     ga.qProbes = I.qProbes;
     ga.k = k;
-    ga.part = reinterpret_cast<int32_t*>(I.score.get());
     if (batched) {
         ga.qp = I.qparams.get();
         ga.ldo = k;
@@ -597,6 +598,24 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
                                         lowp ? lowprec_extra_words(I.refine_count(k), gthreads)
                                              : (int64_t)I.gather_threads + 256 + 2048,
                                         gthreads <= 256 ? 2 : 1);
+    // candidate segments.  One dedup window: the unique candidates, at most
+    // min(C_tot, N).  Several windows: the scatter by window holds every
+    // bucket id, duplicates included, so C_tot itself.  A flagged query (its
+    // probes are replayed later) is bounded by its qProbes largest buckets.
+    const bool one_window = I.n_points <= ga.window_bits;
+    std::vector<uint64_t> off(nq + 1);
+    off[0] = 0;
+    for (int q = 0; q < nq; ++q) {
+        const uint64_t qpq = batched ? (uint64_t)I.hqp[q].qProbes : (uint64_t)I.qProbes;
+        const uint64_t bound = flag[q] ? qpq * I.max_bucket : ct[q];
+        off[q + 1] = off[q] + (one_window ? std::min<uint64_t>(bound, (uint64_t)I.n_points) : bound);
+    }
+    I.cand_off.reserve(nq + 1);
+    I.cand.reserve(off[nq]);
+    I.score.reserve(off[nq]);
+    I.ncand.reserve(nq);
+    ga.part = reinterpret_cast<int32_t*>(I.score.get());
+    FPP_HIP(hipMemcpyAsync(I.cand_off.get(), off.data(), (nq + 1) * 8, hipMemcpyHostToDevice, st));
     const uint32_t* order = nullptr;
     if (I.order_queries && nq > 1 && !I.persistent) {
         const int bits = ilog2_exact(I.NB());
@@ -783,13 +802,21 @@ void write_device(File& out, const void* d, size_t bytes) {
         out.write(buf.data(), c);
     }
 }
-void read_device(File& in, void* d, size_t bytes) {
-    std::vector<unsigned char> buf(std::min<size_t>(bytes, size_t(64) << 20));
-    for (size_t o = 0; o < bytes; o += buf.size()) {
-        const size_t c = std::min(buf.size(), bytes - o);
-        in.read(buf.data(), c);
-        FPP_HIP(hipMemcpy(static_cast<unsigned char*>(d) + o, buf.data(), c, hipMemcpyHostToDevice));
+// check(ptr, count) sees each staged piece of T values before it is copied
+template <class T, class Check>
+void read_device(File& in, T* d, size_t count, Check&& check) {
+    const size_t per = std::max<size_t>(1, (size_t(64) << 20) / sizeof(T));
+    std::vector<T> buf(std::min(count, per));
+    for (size_t o = 0; o < count; o += buf.size()) {
+        const size_t c = std::min(buf.size(), count - o);
+        in.read(buf.data(), c * sizeof(T));
+        check(buf.data(), c);
+        FPP_HIP(hipMemcpy(d + o, buf.data(), c * sizeof(T), hipMemcpyHostToDevice));
     }
+}
+template <class T>
+void read_device(File& in, T* d, size_t count) {
+    read_device(in, d, count, [](const T*, size_t) {});
 }
 
 void save_index(const Index& I, const char* path) {
@@ -864,6 +891,8 @@ void load_index(Index& I, const char* path) {
     I.Xb.reset();
     I.X8.reset();
     I.s8.reset();
+    I.pre_cnt.reset();
+    I.have_pre_cnt = false;
     I.n_tables = T.n_tables;
     I.n_proj = T.n_proj;
     I.fhtDim = T.fhtDim;
@@ -877,21 +906,36 @@ void load_index(Index& I, const char* path) {
     in.read(I.bit1.data(), I.bit1.size());
     in.read(I.bit2.data(), I.bit2.size());
     I.masks.reserve((size_t)h.n_mask_words);
-    read_device(in, I.masks.get(), (size_t)h.n_mask_words * 4);
+    read_device(in, I.masks.get(), (size_t)h.n_mask_words);
     I.center.reserve((size_t)h.n_features);
-    read_device(in, I.center.get(), (size_t)h.n_features * 4);
+    read_device(in, I.center.get(), (size_t)h.n_features);
+    // offsets: from 0, non-decreasing, ending at the id count; the largest
+    // bucket is recomputed from them (the kernels size segments by it)
     I.bucket_off.reserve((size_t)h.n_offsets);
-    read_device(in, I.bucket_off.get(), (size_t)h.n_offsets * 8);
+    uint64_t prev = 0, largest = 0;
+    bool first = true;
+    read_device(in, I.bucket_off.get(), (size_t)h.n_offsets, [&](const uint64_t* o, size_t c) {
+        for (size_t i = 0; i < c; ++i) {
+            FPP_REQUIRE(first ? o[i] == 0 : o[i] >= prev, FPP_EINVAL,
+                        "index file: bucket offsets do not start at 0 and increase");
+            FPP_REQUIRE(o[i] <= (uint64_t)h.n_ids, FPP_EINVAL, "index file: bucket offset past the id count");
+            if (!first) largest = std::max<uint64_t>(largest, o[i] - prev);
+            prev = o[i];
+            first = false;
+        }
+    });
+    FPP_REQUIRE((int64_t)prev == h.n_ids, FPP_EINVAL, "index file: bucket offsets do not end at the id count");
+    FPP_REQUIRE(largest <= (uint64_t)h.n_points, FPP_EINVAL, "index file: a bucket holds more ids than points");
     I.ids.reserve((size_t)std::max<int64_t>(h.n_ids, 1));
-    read_device(in, I.ids.get(), (size_t)h.n_ids * 4);
+    read_device(in, I.ids.get(), (size_t)h.n_ids, [&](const int32_t* v, size_t c) {
+        for (size_t i = 0; i < c; ++i)
+            FPP_REQUIRE(v[i] >= 0 && (int64_t)v[i] < h.n_points, FPP_EINVAL, "index file: point id out of range");
+    });
     I.ldx = h.ldx;
     I.Xc.reserve((size_t)h.n_points * h.ldx);
-    read_device(in, I.Xc.get(), (size_t)h.n_points * h.ldx * 4);
-    uint64_t last = 0;
-    FPP_HIP(hipMemcpy(&last, I.bucket_off.get() + (h.n_offsets - 1), 8, hipMemcpyDeviceToHost));
-    FPP_REQUIRE((int64_t)last == h.n_ids, FPP_EINVAL, "index file: bucket offsets do not end at the id count");
+    read_device(in, I.Xc.get(), (size_t)h.n_points * h.ldx);
     I.n_ids = h.n_ids;
-    I.max_bucket = (uint32_t)h.max_bucket;
+    I.max_bucket = (uint32_t)largest;
     I.stat_hash_ties = h.hash_ties;
     I.stat_bucket_ties = h.bucket_ties;
     I.build_gen++;
@@ -929,6 +973,11 @@ void autotune(Index& I, hipStream_t st, Run run, int32_t* chosen) {
                     FPP_HIP(hipEventSynchronize(e1));
                     float ms = 0.0f;
                     FPP_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (I.tune_verbose) {
+                        fprintf(stderr, "[autotune] threads=%d order=%d persistent=%d: %.3f ms per call\n", t, o,
+                                pers, ms / 2.0f);
+                        fflush(stderr);
+                    }
                     if (best == 0.0f || ms < best) {
                         best = ms;
                         bt = t;
@@ -1037,6 +1086,8 @@ int fpp_clear(fpp_index* h) {
         I.Xb.reset();
         I.X8.reset();
         I.s8.reset();
+        I.pre_cnt.reset();
+        I.have_pre_cnt = false;
         I.lp_ready = 0;
         I.n_ids = 0;
         I.built = false;
@@ -1105,7 +1156,13 @@ int fpp_autotune_batch(fpp_index* h, const float* dQ, int64_t ldq, int nq, const
     });
 }
 
-int fpp_query(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t* out, int verbose) {
+}  // extern "C"
+
+namespace {
+// query2Layers_1D / query2Layers on host buffers; a verbose call prints the
+// reference's statistics and writes <layout>_Top_...txt.
+int query_host(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t* out, int verbose,
+               const char* layout) {
     return guard([&] {
         FPP_REQUIRE(h && (nq == 0 || (Q && out)), FPP_EINVAL, "null argument");
         Index& I = h->I;
@@ -1156,10 +1213,21 @@ int fpp_query(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t*
             printf("Querying Time: %g ms\n", ms);
             fflush(stdout);
             const std::string name = topk_file_name(k, I.n_proj, I.n_tables, I.iProbes, I.bucket_minSize,
-                                                    I.bucket_scale, I.qProbes);
+                                                    I.bucket_scale, I.qProbes, layout);
             write_topk_txt(name.c_str(), out, nq, k);
         }
     });
+}
+}  // namespace
+
+extern "C" {
+
+int fpp_query(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t* out, int verbose) {
+    return query_host(h, Q, ldq, nq, k, out, verbose, "1D");
+}
+
+int fpp_query_2d(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t* out, int verbose) {
+    return query_host(h, Q, ldq, nq, k, out, verbose, "2D");
 }
 
 int fpp_save(fpp_index* h, const char* path) {
@@ -1193,14 +1261,31 @@ int fpp_write_topk_txt(const char* path, const int32_t* ids, int nq, int k) {
     });
 }
 
-int fpp_topk_file_name(int k, int n_proj, int n_tables, int iProbes, int bucket_minSize, float bucket_scale,
-                       int qProbes, char* buf, int buflen) {
+}  // extern "C"
+
+namespace {
+int file_name(int k, int n_proj, int n_tables, int iProbes, int bucket_minSize, float bucket_scale, int qProbes,
+              char* buf, int buflen, const char* layout) {
     return guard([&] {
         FPP_REQUIRE(buf && buflen >= 1, FPP_EINVAL, "null argument");
-        const std::string s = topk_file_name(k, n_proj, n_tables, iProbes, bucket_minSize, bucket_scale, qProbes);
+        const std::string s =
+            topk_file_name(k, n_proj, n_tables, iProbes, bucket_minSize, bucket_scale, qProbes, layout);
         FPP_REQUIRE((int64_t)s.size() < (int64_t)buflen, FPP_EINVAL, "buffer too short for the file name");
         std::memcpy(buf, s.c_str(), s.size() + 1);
     });
+}
+}  // namespace
+
+extern "C" {
+
+int fpp_topk_file_name(int k, int n_proj, int n_tables, int iProbes, int bucket_minSize, float bucket_scale,
+                       int qProbes, char* buf, int buflen) {
+    return file_name(k, n_proj, n_tables, iProbes, bucket_minSize, bucket_scale, qProbes, buf, buflen, "1D");
+}
+
+int fpp_topk_file_name_2d(int k, int n_proj, int n_tables, int iProbes, int bucket_minSize, float bucket_scale,
+                          int qProbes, char* buf, int buflen) {
+    return file_name(k, n_proj, n_tables, iProbes, bucket_minSize, bucket_scale, qProbes, buf, buflen, "2D");
 }
 
 int fpp_get_shape(fpp_index* h, int64_t* out) {
@@ -1240,6 +1325,16 @@ int fpp_get_buckets(fpp_index* h, uint64_t* off, int32_t* ids) {
         const size_t nbk = (size_t)I.n_tables * I.NB();
         FPP_HIP(hipMemcpy(off, I.bucket_off.get(), (nbk + 1) * 8, hipMemcpyDeviceToHost));
         if (I.n_ids) FPP_HIP(hipMemcpy(ids, I.ids.get(), (size_t)I.n_ids * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int fpp_get_bucket_counts(fpp_index* h, uint32_t* cnt) {
+    return guard([&] {
+        FPP_REQUIRE(h && cnt, FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        FPP_REQUIRE(I.built && I.have_pre_cnt, FPP_ESTATE,
+                    "no pre-filter bucket sizes: the index was not built in this process");
+        FPP_HIP(hipMemcpy(cnt, I.pre_cnt.get(), (size_t)I.n_tables * I.NB() * 4, hipMemcpyDeviceToHost));
     });
 }
 
@@ -1296,6 +1391,7 @@ int fpp_get_stats(fpp_index* h, int64_t* out) {
         out[3] = I.stat_cuniq;
         out[4] = I.stat_qties;
         out[5] = I.last_nq;
+        out[6] = I.stat_probe_path;
     });
 }
 
@@ -1324,6 +1420,14 @@ int fpp_set_precision(fpp_index* h, int mode, int refine) {
     });
 }
 
+int fpp_get_fp8_scale(fpp_index* h, float* shared) {
+    return guard([&] {
+        FPP_REQUIRE(h && shared, FPP_EINVAL, "null argument");
+        FPP_REQUIRE(h->I.built && h->I.lp_ready == 2, FPP_ESTATE, "no FP8 copy: make an FP8 query first");
+        *shared = h->I.s8_shared;
+    });
+}
+
 int fpp_set_tuning(fpp_index* h, int which, int value) {
     return guard([&] {
         FPP_REQUIRE(h, FPP_EINVAL, "null argument");
@@ -1338,6 +1442,12 @@ int fpp_set_tuning(fpp_index* h, int which, int value) {
                 break;
             case FPP_TUNE_PERSISTENT:
                 h->I.persistent = value != 0;
+                break;
+            case FPP_TUNE_FP8_PER_ROW:
+                h->I.fp8_per_row = value != 0;
+                break;
+            case FPP_TUNE_VERBOSE:
+                h->I.tune_verbose = value != 0;
                 break;
             case FPP_TUNE_GRAPH:
                 h->I.use_graph = value != 0;

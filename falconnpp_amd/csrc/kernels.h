@@ -106,6 +106,11 @@ __device__ __forceinline__ QueryView query_view(const QueryParams* qp, int q, in
 }
 // Global top-qProbes bucket set per query; ctot[q] = sum of probed sizes;
 // qflag[q] |= 2 when a tie straddles a per-table or the global boundary.
+// probe_select_path() tells which selection a launch takes: the pool of every
+// pair, the per-table top lists (two-stage), or none because neither fits the
+// LDS (the caller then flags every query for the exact replay).
+enum { PROBE_FULL_POOL = 0, PROBE_TWO_STAGE = 1, PROBE_EXACT = 2 };
+int probe_select_path(const ProbeGeom& g);
 void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const float* qv,
                          const uint64_t* bucket_off, uint32_t* probes, uint64_t* ctot, uint32_t* qflag,
                          hipStream_t st);
@@ -185,7 +190,7 @@ int64_t lowprec_extra_words(int refine, int threads);
 void launch_quant_bf16(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint16_t* Xb, hipStream_t st);
 // returns the scale shared by every row, or 0 when rows keep their own
 float launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint8_t* X8, float* scale,
-                       uint32_t* range, hipStream_t st);
+                       uint32_t* range, bool per_row, hipStream_t st);
 void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, int nq, const uint32_t* probes,
                             const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q,
                             int64_t ldq, const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids,

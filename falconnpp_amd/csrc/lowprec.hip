@@ -426,9 +426,9 @@ void launch_quant_bf16(const float* Xc, int64_t ldx, int64_t N, int D, int64_t l
 // When every nonzero row's largest magnitude lies within 2^8 of the largest
 // one, all rows share the scale amax_max / 240: each row's largest code is then
 // at least 240 / 256, inside e4m3's normal range, and scoring needs no
-// per-row scale lookup.  Otherwise each row keeps its own.
+// per-row scale lookup.  Otherwise, or with per_row, each row keeps its own.
 float launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint8_t* X8, float* scale,
-                       uint32_t* range, hipStream_t st) {
+                       uint32_t* range, bool per_row, hipStream_t st) {
     FPP_REQUIRE(ldl % 4 == 0, FPP_EINTERNAL, "fp8 row stride must be a multiple of 4");
     launch_fill_u32(range, 0u, 1, st);
     launch_fill_u32(range + 1, 0xffffffffu, 1, st);
@@ -441,7 +441,8 @@ float launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t l
     float hi, lo;
     std::memcpy(&hi, &r[0], 4);
     std::memcpy(&lo, &r[1], 4);
-    const float uscale = (hi > 0.0f && r[1] != 0xffffffffu && hi <= 256.0f * lo) ? hi / FP8_TARGET : 0.0f;
+    const float uscale =
+        (!per_row && hi > 0.0f && r[1] != 0xffffffffu && hi <= 256.0f * lo) ? hi / FP8_TARGET : 0.0f;
     k_quant_fp8<<<(unsigned)ceil_div(N, 4), 256, 0, st>>>(Xc, ldx, N, D, ldl, uscale, X8, scale);
     FPP_HIP(hipGetLastError());
     return uscale;

@@ -998,18 +998,32 @@ void dispatch_gather_cpl(const GatherArgs& a, int nq, const uint32_t* probes, co
 
 }  // namespace
 
+namespace {
+size_t probe_fixed_lds(const ProbeGeom& g) { return (size_t)(PS_WAVES * 256 + 256 + g.L) * 4; }
+size_t probe_full_lds(const ProbeGeom& g) {
+    return ((size_t)g.L * g.PL * g.PL + (size_t)g.L * 2 * g.PL) * 4 + probe_fixed_lds(g);
+}
+size_t probe_two_stage_lds(const ProbeGeom& g) { return (size_t)2 * g.L * g.perTable * 4 + probe_fixed_lds(g); }
+}  // namespace
+
+// The pool of every pair when it fits the LDS, else the per-table top lists
+// when those fit, else none: the queries then take the exact replay.
+int probe_select_path(const ProbeGeom& g) {
+    if (probe_full_lds(g) <= 160 * 1024) return PROBE_FULL_POOL;
+    if (probe_two_stage_lds(g) <= 160 * 1024) return PROBE_TWO_STAGE;
+    return PROBE_EXACT;
+}
+
 void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const float* qv,
                          const uint64_t* bucket_off, uint32_t* probes, uint64_t* ctot, uint32_t* qflag,
                          hipStream_t st) {
-    const size_t fixed = (size_t)(PS_WAVES * 256 + 256 + g.L) * 4;
-    const size_t lds_full = ((size_t)g.L * g.PL * g.PL + (size_t)g.L * 2 * g.PL) * 4 + fixed;
-    const size_t lds = (size_t)2 * g.L * g.perTable * 4 + fixed;
-    FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "probe pool (n_tables * perTable) too large for one workgroup");
+    const int path = probe_select_path(g);
+    FPP_REQUIRE(path != PROBE_EXACT, FPP_EINTERNAL, "probe pool does not fit the LDS: replay the queries instead");
     if (nq == 0) return;
-    if (lds_full <= 160 * 1024)
-        k_probe_select<true><<<nq, PS_T, lds_full, st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
+    if (path == PROBE_FULL_POOL)
+        k_probe_select<true><<<nq, PS_T, probe_full_lds(g), st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
     else
-        k_probe_select<false><<<nq, PS_T, lds, st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
+        k_probe_select<false><<<nq, PS_T, probe_two_stage_lds(g), st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
     FPP_HIP(hipGetLastError());
 }
 

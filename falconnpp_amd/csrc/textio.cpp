@@ -98,10 +98,10 @@ void write_topk_txt(const char* path, const int32_t* ids, int64_t nq, int k) {
 }
 
 std::string topk_file_name(int k, int n_proj, int n_tables, int iProbes, int bucket_minSize, float bucket_scale,
-                           int qProbes) {
+                           int qProbes, const char* layout) {
     // the scale enters as (int)(scale * 100), a float product truncated
     const float pct = bucket_scale * 100;
-    return "1D_Top_" + std::to_string(k) + "_D_" + std::to_string(n_proj) + "_L_" + std::to_string(n_tables) +
+    return std::string(layout) + "_Top_" + std::to_string(k) + "_D_" + std::to_string(n_proj) + "_L_" + std::to_string(n_tables) +
            "_iProbe_" + std::to_string(iProbes) + "_bucketMinSize_" + std::to_string(bucket_minSize) +
            "_bucketScale_" + std::to_string((int)pct) + "_qProbe_" + std::to_string(qProbes) + ".txt";
 }

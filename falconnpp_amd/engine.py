@@ -18,15 +18,16 @@ LIB_PATH = os.path.join(HERE, "libfalconnpp_amd.so")
 # Every symbol include/falconnpp_amd.h declares.
 EXPORTS = (
     "fpp_last_error", "fpp_create", "fpp_destroy", "fpp_set_index_param", "fpp_set_qprobes", "fpp_set_threads",
-    "fpp_clear", "fpp_build", "fpp_build_device", "fpp_query", "fpp_query_device", "fpp_get_shape",
-    "fpp_get_center", "fpp_get_sign_bits", "fpp_get_buckets", "fpp_hash_points", "fpp_get_stats",
+    "fpp_clear", "fpp_build", "fpp_build_device", "fpp_query", "fpp_query_2d", "fpp_query_device", "fpp_get_shape",
+    "fpp_get_center", "fpp_get_sign_bits", "fpp_get_buckets", "fpp_get_bucket_counts", "fpp_hash_points", "fpp_get_stats",
     "fpp_get_last_probes", "fpp_get_last_candidates", "fpp_bruteforce_f64", "fpp_set_profiling",
-    "fpp_get_timings", "fpp_set_tuning", "fpp_set_precision", "fpp_autotune", "fpp_autotune_batch",
+    "fpp_get_timings", "fpp_set_tuning", "fpp_set_precision", "fpp_get_fp8_scale", "fpp_autotune", "fpp_autotune_batch",
     "fpp_query_batch_device",
-    "fpp_get_query_times", "fpp_load_txt", "fpp_write_topk_txt", "fpp_topk_file_name", "fpp_save", "fpp_load",
+    "fpp_get_query_times", "fpp_load_txt", "fpp_write_topk_txt", "fpp_topk_file_name", "fpp_topk_file_name_2d",
+    "fpp_save", "fpp_load",
 )
 PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
-TUNE_QUERY_ORDER, TUNE_GATHER_THREADS, TUNE_PERSISTENT, TUNE_GRAPH = 1, 2, 3, 4
+TUNE_QUERY_ORDER, TUNE_GATHER_THREADS, TUNE_PERSISTENT, TUNE_GRAPH, TUNE_VERBOSE, TUNE_FP8_PER_ROW = 1, 2, 3, 4, 5, 6
 
 FPP_EINVAL, FPP_ESTATE, FPP_EHIP, FPP_ENOMEM, FPP_EINTERNAL, FPP_EIO = 1, 2, 3, 4, 5, 6
 
@@ -73,11 +74,13 @@ def load():
     L.fpp_build.argtypes = [vp, vp, i64]
     L.fpp_build_device.argtypes = [vp, vp, i64, vp]
     L.fpp_query.argtypes = [vp, vp, i64, i32, i32, vp, i32]
+    L.fpp_query_2d.argtypes = [vp, vp, i64, i32, i32, vp, i32]
     L.fpp_query_device.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
     L.fpp_get_shape.argtypes = [vp, vp]
     L.fpp_get_center.argtypes = [vp, vp]
     L.fpp_get_sign_bits.argtypes = [vp, vp, vp]
     L.fpp_get_buckets.argtypes = [vp, vp, vp]
+    L.fpp_get_bucket_counts.argtypes = [vp, vp]
     L.fpp_hash_points.argtypes = [vp, vp, i64, i32, vp, vp]
     L.fpp_get_stats.argtypes = [vp, vp]
     L.fpp_get_last_probes.argtypes = [vp, vp]
@@ -85,6 +88,7 @@ def load():
     L.fpp_get_timings.argtypes = [vp, vp]
     L.fpp_set_tuning.argtypes = [vp, i32, i32]
     L.fpp_set_precision.argtypes = [vp, i32, i32]
+    L.fpp_get_fp8_scale.argtypes = [vp, vp]
     L.fpp_autotune.argtypes = [vp, vp, i64, i32, i32, vp, vp]
     L.fpp_autotune_batch.argtypes = [vp, vp, i64, i32, vp, vp, i32, vp, vp]
     L.fpp_get_query_times.argtypes = [vp, vp, vp]
@@ -97,6 +101,7 @@ def load():
     L.fpp_load.argtypes = [vp, C.c_char_p]
     L.fpp_write_topk_txt.argtypes = [C.c_char_p, vp, i32, i32]
     L.fpp_topk_file_name.argtypes = [i32, i32, i32, i32, i32, C.c_float, i32, C.c_char_p, i32]
+    L.fpp_topk_file_name_2d.argtypes = [i32, i32, i32, i32, i32, C.c_float, i32, C.c_char_p, i32]
     _lib = L
     return L
 
@@ -133,10 +138,12 @@ def write_topk_txt(path: str, ids: np.ndarray):
 
 
 def topk_file_name(k: int, n_proj: int, n_tables: int, iProbes: int, bucket_minSize: int, bucket_scale: float,
-                   qProbes: int) -> str:
-    """File name a verbose query writes (src/FalconnPP.cpp:694-700)."""
+                   qProbes: int, layout: str = "1D") -> str:
+    """File name a verbose query writes: 1D_Top_... for query
+    (src/FalconnPP.cpp:694-700), 2D_Top_... for query2D (:1400-1406)."""
     buf = C.create_string_buffer(512)
-    _check(load().fpp_topk_file_name(k, n_proj, n_tables, iProbes, bucket_minSize, bucket_scale, qProbes, buf, 512))
+    fn = load().fpp_topk_file_name if layout == "1D" else load().fpp_topk_file_name_2d
+    _check(fn(k, n_proj, n_tables, iProbes, bucket_minSize, bucket_scale, qProbes, buf, 512))
     return buf.value.decode()
 
 
@@ -195,6 +202,14 @@ class Index:
         nq = Q.shape[0]
         out = np.zeros((nq, k), np.int32)
         _check(load().fpp_query(self.h, _ptr(Q), self.d, nq, k, _ptr(out), 1 if verbose else 0))
+        return out
+
+    def query2D(self, Q_nd, k: int, verbose: bool = False) -> np.ndarray:
+        """query2Layers (src/FalconnPP.cpp:1013-1413): same ids as query()."""
+        Q = _points(Q_nd)
+        nq = Q.shape[0]
+        out = np.zeros((nq, k), np.int32)
+        _check(load().fpp_query_2d(self.h, _ptr(Q), self.d, nq, k, _ptr(out), 1 if verbose else 0))
         return out
 
     def query_batch_device(self, dQ_ptr: int, ldq: int, nq: int, qprobes, ks, kmax: int, dout_ptr: int,
@@ -258,6 +273,13 @@ class Index:
         _check(load().fpp_get_buckets(self.h, _ptr(off), _ptr(ids)))
         return off, ids[: sh["n_ids"]]
 
+    def bucket_counts(self) -> np.ndarray:
+        """Bucket sizes before the alpha / bucketLimit filter."""
+        sh = self.shape()
+        cnt = np.zeros(sh["buckets_per_table"] * sh["n_tables"], np.uint32)
+        _check(load().fpp_get_bucket_counts(self.h, _ptr(cnt)))
+        return cnt
+
     def hash_points(self, V_md):
         V = _points(V_md)
         m = V.shape[0]
@@ -267,10 +289,10 @@ class Index:
         return codes.reshape(self.L, m, self.iProbes), vals.reshape(self.L, m, self.iProbes)
 
     def stats(self):
-        s = np.zeros(6, np.int64)
+        s = np.zeros(7, np.int64)
         _check(load().fpp_get_stats(self.h, _ptr(s)))
         return dict(hash_ties=int(s[0]), bucket_ties=int(s[1]), ctot=int(s[2]), cuniq=int(s[3]),
-                    query_ties=int(s[4]), nq=int(s[5]))
+                    query_ties=int(s[4]), nq=int(s[5]), probe_path=int(s[6]))
 
     def set_profiling(self, level: int):
         """0 off, 1 HIP events per query stage, 2 also per-query phase stamps."""
@@ -287,6 +309,12 @@ class Index:
     def set_precision(self, mode: int, refine: int = 0):
         """0 FP32 (reference), 1 BF16 + FP32 refine, 2 FP8 + FP32 refine."""
         _check(load().fpp_set_precision(self.h, mode, refine))
+
+    def fp8_shared_scale(self) -> float:
+        """Scale every FP8 row shares (0.0 when rows keep their own)."""
+        s = C.c_float(0.0)
+        _check(load().fpp_get_fp8_scale(self.h, C.byref(s)))
+        return float(s.value)
 
     def set_tuning(self, which: int, value: int):
         _check(load().fpp_set_tuning(self.h, which, value))

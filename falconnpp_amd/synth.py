@@ -9,6 +9,9 @@ column-major layout (src/FalconnPP.h:30) viewed from numpy.
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 DATA_SEED = 20240601
@@ -25,16 +28,41 @@ def _normalise_rows(y: np.ndarray, rho: np.ndarray | None) -> np.ndarray:
     return y
 
 
-def lognormal_sphere(n: int, d: int, sigma: float = 0.3, seed: int = DATA_SEED, unit: bool = False,
-                     chunk: int = 1 << 20) -> np.ndarray:
-    rng = np.random.default_rng(seed)
+# Rows are generated in chunks of CHUNK; chunk c draws from its own stream
+# default_rng([seed, c]), so the bytes do not depend on how many threads run
+# the chunks.
+CHUNK = 1 << 18
+
+
+def _threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _chunked(n: int, d: int, fill) -> np.ndarray:
     out = np.empty((n, d), np.float32)
-    for s in range(0, n, chunk):
-        e = min(n, s + chunk)
-        z = rng.standard_normal((e - s, d), dtype=np.float32)
-        rho = None if unit else rng.lognormal(0.0, sigma, e - s).astype(np.float32)
-        out[s:e] = _normalise_rows(z, rho)
+    starts = list(range(0, n, CHUNK))
+
+    def run(ci):
+        s = starts[ci]
+        fill(ci, out[s:min(n, s + CHUNK)])
+
+    if len(starts) <= 1:
+        for ci in range(len(starts)):
+            run(ci)
+    else:
+        with ThreadPoolExecutor(_threads()) as ex:
+            list(ex.map(run, range(len(starts))))
     return out
+
+
+def lognormal_sphere(n: int, d: int, sigma: float = 0.3, seed: int = DATA_SEED, unit: bool = False) -> np.ndarray:
+    def fill(ci, o):
+        rng = np.random.default_rng([seed, ci])
+        z = rng.standard_normal(o.shape, dtype=np.float32)
+        rho = None if unit else rng.lognormal(0.0, sigma, len(o)).astype(np.float32)
+        o[:] = _normalise_rows(z, rho)
+
+    return _chunked(n, d, fill)
 
 
 def mixture_centres(n_centres: int, d: int, seed: int = DATA_SEED) -> np.ndarray:
@@ -43,18 +71,20 @@ def mixture_centres(n_centres: int, d: int, seed: int = DATA_SEED) -> np.ndarray
 
 
 def gaussian_mixture(n: int, d: int, n_centres: int, within: float = 0.3, norm_sigma: float = 0.25,
-                     seed: int = DATA_SEED, centres: np.ndarray | None = None, chunk: int = 1 << 20) -> np.ndarray:
+                     seed: int = DATA_SEED, centres: np.ndarray | None = None) -> np.ndarray:
     if centres is None:
         centres = mixture_centres(n_centres, d)
-    rng = np.random.default_rng(seed)
-    out = np.empty((n, d), np.float32)
-    for s in range(0, n, chunk):
-        e = min(n, s + chunk)
-        j = rng.integers(0, centres.shape[0], e - s)
-        y = centres[j] + np.float32(within) * rng.standard_normal((e - s, d), dtype=np.float32)
-        rho = rng.lognormal(0.0, norm_sigma, e - s).astype(np.float32)
-        out[s:e] = _normalise_rows(y, rho)
-    return out
+
+    def fill(ci, o):
+        rng = np.random.default_rng([seed, ci])
+        j = rng.integers(0, centres.shape[0], len(o))
+        rng.standard_normal(o.shape, dtype=np.float32, out=o)
+        o *= np.float32(within)
+        o += centres[j]
+        rho = rng.lognormal(0.0, norm_sigma, len(o)).astype(np.float32)
+        _normalise_rows(o, rho)
+
+    return _chunked(n, d, fill)
 
 
 # name -> (generator kwargs, index params, query params)

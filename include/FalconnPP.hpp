@@ -52,10 +52,12 @@ public:
     }
     // The reference's 2D-storage variants (src/FalconnPP.cpp:722-1413) keep the
     // same bucket contents and return the same ids; they are served from the
-    // same CSR tables.
+    // same CSR tables.  A verbose query2Layers writes 2D_Top_...txt.
     void build2Layers(const float* X, int64_t ld) { build2Layers_1D(X, ld); }
     std::vector<int32_t> query2Layers(const float* Q, int64_t ldq, int nq, int k, bool verbose = false) {
-        return query2Layers_1D(Q, ldq, nq, k, verbose);
+        std::vector<int32_t> out((size_t)nq * k, 0);
+        check(fpp_query_2d(h_, Q, ldq, nq, k, out.data(), verbose ? 1 : 0));
+        return out;
     }
 
     // index files (no reference counterpart)

@@ -71,6 +71,12 @@ int fpp_build_device(fpp_index* h, const float* dX, int64_t ld, void* stream);
  * (src/FalconnPP.cpp:682-704).
  * Q: host D x nq col-major (ldq >= D); out: host nq x k. */
 int fpp_query(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t* out, int verbose);
+/* query2Layers(Q, k, verbose) -- src/FalconnPP.cpp:1013-1413, bound as
+ * query2D(queries, n_neighbors, verbose=False) at python/python_wrapper.cpp:27.
+ * The 2D-storage index holds the same bucket contents as the 1D one, so this
+ * answers from the same tables with the same ids; verbose output goes to
+ * fpp_topk_file_name_2d(...) (2D_Top_..., src/FalconnPP.cpp:1400-1406). */
+int fpp_query_2d(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t* out, int verbose);
 /* Same with Q and outputs in device memory; scores (nq x k, FP32 q.(x-c)) may
  * be NULL. */
 int fpp_query_device(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_out, float* d_scores,
@@ -88,13 +94,19 @@ int fpp_get_sign_bits(fpp_index* h, uint8_t* bit1, uint8_t* bit2);
  * bucket b (global id = table*4*n_proj^2 + code) holds ids[off[b]..off[b+1])
  * in ascending id order. */
 int fpp_get_buckets(fpp_index* h, uint64_t* off, int32_t* ids);
+/* bucket sizes before the alpha / bucketLimit filter, the B of
+ * src/FalconnPP.cpp:226-242: cnt[n_tables*4*n_proj^2].  FPP_ESTATE for an
+ * index restored from a file. */
+int fpp_get_bucket_counts(fpp_index* h, uint32_t* cnt);
 /* build-side hash of m host vectors (D x m col-major, taken as already
  * centred): codes/vals[(l*m + j)*iProbes + i], i in ascending value order.
  * Needs the sign bits of a previous build. */
 int fpp_hash_points(fpp_index* h, const float* V, int64_t ld, int m, uint32_t* codes, float* vals);
 /* statistics: out[0]=build hash items replayed for ties, [1]=buckets replayed
  * for ties, [2]=sum C_tot of the last query call, [3]=sum C_uniq, [4]=queries
- * replayed for ties, [5]=queries in the last call */
+ * replayed for ties, [5]=queries in the last call, [6]=probe selection of the
+ * last call (0: pool of every pair, 1: per-table top lists, 2: exact replay of
+ * every query because neither pool fits the LDS).  out must hold 7 values. */
 int fpp_get_stats(fpp_index* h, int64_t* out);
 /* per-stage device time of the last query call when profiling is on
  * (on >= 1: HIP events on the call's stream): ms[0]=query hashing,
@@ -115,6 +127,10 @@ int fpp_get_timings(fpp_index* h, double* ms);
  * ids can differ from mode 0 when a true top-k row falls outside the refine
  * set.  The low-precision copy is made on the first query after a build. */
 int fpp_set_precision(fpp_index* h, int mode, int refine);
+/* The FP8 copy's scale (no reference counterpart): *shared receives the scale
+ * every row uses, or 0 when rows keep their own.  FPP_ESTATE before an FP8
+ * query has made the copy. */
+int fpp_get_fp8_scale(fpp_index* h, float* shared);
 /* launch-shape knobs (no reference counterpart): FPP_TUNE_QUERY_ORDER 0/1
  * (process queries grouped by their best table-0 bucket, XCD-aware),
  * FPP_TUNE_GATHER_THREADS 256/512/1024 (workgroup of the gather kernel; 256
@@ -122,8 +138,20 @@ int fpp_set_precision(fpp_index* h, int mode, int refine);
  * FPP_TUNE_PERSISTENT 0/1 (one persistent gather launch whose workgroups take
  * queries from a work queue, longest first), FPP_TUNE_GRAPH 0/1 (hashing and
  * probe selection of a batch captured once in a hipGraph and replayed while
- * the batch size and buffers stay the same).  None changes any result. */
-enum { FPP_TUNE_QUERY_ORDER = 1, FPP_TUNE_GATHER_THREADS = 2, FPP_TUNE_PERSISTENT = 3, FPP_TUNE_GRAPH = 4 };
+ * the batch size and buffers stay the same), FPP_TUNE_VERBOSE 0/1 (the
+ * autotuner prints one line per launch shape it times to stderr).  None
+ * changes any result.  FPP_TUNE_FP8_PER_ROW 0/1: the FP8 copy keeps one
+ * scale per row even where one shared scale would do (the copy is remade on
+ * the next FP8 query); it changes the approximate scores, so it can change
+ * which candidates reach the FP32 refine. */
+enum {
+    FPP_TUNE_QUERY_ORDER = 1,
+    FPP_TUNE_GATHER_THREADS = 2,
+    FPP_TUNE_PERSISTENT = 3,
+    FPP_TUNE_GRAPH = 4,
+    FPP_TUNE_VERBOSE = 5,
+    FPP_TUNE_FP8_PER_ROW = 6
+};
 int fpp_set_tuning(fpp_index* h, int which, int value);
 /* Autotuner (no reference counterpart): times the gather workgroup sizes,
  * with one workgroup per query (query order off and on) and with the
@@ -192,6 +220,9 @@ int fpp_write_topk_txt(const char* path, const int32_t* ids, int nq, int k);
  * when it is too short). */
 int fpp_topk_file_name(int k, int n_proj, int n_tables, int iProbes, int bucket_minSize, float bucket_scale,
                        int qProbes, char* buf, int buflen);
+/* The same with the 2D_Top_ prefix of query2Layers (src/FalconnPP.cpp:1400-1406). */
+int fpp_topk_file_name_2d(int k, int n_proj, int n_tables, int iProbes, int bucket_minSize, float bucket_scale,
+                          int qProbes, char* buf, int buflen);
 
 #ifdef __cplusplus
 }

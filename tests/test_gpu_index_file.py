@@ -69,7 +69,22 @@ def test_bad_index_files_are_refused(tmp_path):
     data = path.read_bytes()
     (tmp_path / "short.bin").write_bytes(data[:-4])
     (tmp_path / "magic.bin").write_bytes(b"X" + data[1:])
-    for bad in ("short.bin", "magic.bin"):
+    # sections after the 160-byte header: sign bits (twice), flip masks,
+    # centre, bucket offsets (u64), ids (i32), data
+    hdr = np.frombuffer(data[:160], np.int64, 19, 0)
+    n_ids, n_bits, n_mask, n_off = int(hdr[12]), int(hdr[16]), int(hdr[17]), int(hdr[18])
+    off_pos = 160 + 2 * n_bits + 4 * n_mask + 4 * 16
+    ids_pos = off_pos + 8 * n_off
+    off = np.frombuffer(data[off_pos:ids_pos], np.uint64)
+    assert off[-1] == n_ids and n_ids > 0
+    b = int(np.nonzero(off[1:] > off[:-1])[0][0])  # a non-empty bucket
+    dec = off.copy()
+    dec[b + 1] = n_ids if dec[b + 2] < n_ids else 0  # offsets that do not increase
+    (tmp_path / "offsets.bin").write_bytes(data[:off_pos] + dec.tobytes() + data[ids_pos:])
+    ids = np.frombuffer(data[ids_pos:ids_pos + 4 * n_ids], np.int32).copy()
+    ids[n_ids // 2] = 3000  # a point id past n_points
+    (tmp_path / "ids.bin").write_bytes(data[:ids_pos] + ids.tobytes() + data[ids_pos + 4 * n_ids:])
+    for bad in ("short.bin", "magic.bin", "offsets.bin", "ids.bin"):
         h = E.Index(3000, 16)
         with pytest.raises(E.FppError) as e:
             h.load(str(tmp_path / bad))

@@ -46,10 +46,29 @@ CASES = {
     # ... and with ties everywhere (40 per layer; 20 x 1600 lists of 256 KB,
     # read by the merge in place)
     "int_grid_pl40_exact": (lambda: _sym_int_grid(2000, 24, -2, 2, 18), 20, 64, 5, 0.3, 2, 16000, 10, 20),
-    # pool of every pair (64 x 18^2 pairs, 171 KB) above the LDS: the
-    # two-stage probe selection (top-290 per table, then the 9280 of 18,560)
-    "two_stage_l64_pl18": (lambda: synth.lognormal_sphere(20000, 64, 0.3, seed=21), 64, 64, 20, 0.01, 1, 9280, 20, 20),
+    # 64 tables x 18^2 pairs: the pool of every pair (97.5 KB of LDS with the
+    # tables' code lists) still fits and is taken
+    "full_pool_l64_pl18": (lambda: synth.lognormal_sphere(20000, 64, 0.3, seed=21), 64, 64, 20, 0.01, 1, 9280, 20, 20),
+    # the last step of the reference CLI's sweep (qProbes 10000) at 100 and 350
+    # tables: the pool of every pair fits, the per-table lists would not
+    "l100_qp10000": (lambda: synth.lognormal_sphere(4000, 32, 0.3, seed=22), 100, 32, 10, 0.1, 1, 10000, 20, 20),
+    "l350_qp10000": (lambda: synth.lognormal_sphere(3000, 16, 0.3, seed=23), 350, 16, 10, 0.1, 1, 10000, 20, 20),
+    # 4 probes per layer over 1800 tables: the pool of every pair (173 KB)
+    # does not fit, the top-10 lists per table (144 KB) do: two-stage selection
+    "two_stage_l1800_pl4": (lambda: synth.lognormal_sphere(2000, 16, 0.3, seed=24), 1800, 16, 5, 0.1, 1, 9000, 10, 10),
+    # neither pool fits (192 KB and 240 KB): every query's probes come from the
+    # exact replay, which reuses the fast per-layer lists
+    "exact_pool_l600": (lambda: synth.lognormal_sphere(2000, 16, 0.3, seed=25), 600, 16, 5, 0.1, 1, 15000, 10, 10),
+    # two dedup windows with C_tot above N (alpha = 1 keeps every id; 2000
+    # probes over 16 tables of buckets holding ~1270 ids each): the scatter by
+    # window holds every bucket id
+    "ctot_over_n_windows": (lambda: synth.gaussian_mixture(1_300_000, 16, 300, seed=26), 16, 16, 30, 1.0, 1, 2000, 20, 6),
 }
+
+# probe selection each case must take (engine stats: 0 pool of every pair,
+# 1 per-table lists, 2 exact replay of every query)
+PROBE_PATH = {"full_pool_l64_pl18": 0, "l100_qp10000": 0, "l350_qp10000": 0, "two_stage_l1800_pl4": 1,
+              "exact_pool_l600": 2, "sphere_d100_p1": 0}
 
 _cache = {}
 
@@ -160,6 +179,10 @@ def test_query_probes_candidates_topk(name):
         _check_topk(Xc, q, out[j, :m], d["topk"][:m], d["scores"][:m], k)
         assert np.all(out[j, m:] == 0)
     assert st["ctot"] == ctot
+    if name in PROBE_PATH:
+        assert st["probe_path"] == PROBE_PATH[name]
+    if name == "ctot_over_n_windows":
+        assert all(o.query_debug(q, k)["ctot"] > len(X) for q in Q[:2])
 
 
 @pytest.mark.parametrize("k", [10, 64, 65, 200])

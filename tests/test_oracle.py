@@ -255,3 +255,30 @@ def test_fewer_candidates_than_k_zero_fill():
     for i in range(3):
         nc = int(np.count_nonzero(scores[i]))
         assert np.all(topk[i, nc + 1:] == 0)
+
+
+@pytest.mark.parametrize("kind", ["gauss", "int_ties"])
+def test_bruteforce_f64_matches_numpy_topk(kind):
+    """orc_bruteforce_f64 (src/BF.cpp:17-87, in FP64) against a numpy float64
+    top-k ranked by (score desc, id asc).  Integer data makes every score an
+    exact integer in FP64, so equal scores are frequent and the tie order is
+    checked too."""
+    rng = np.random.default_rng(23)
+    if kind == "gauss":
+        X = rng.standard_normal((2500, 24)).astype(np.float32)
+        Q = rng.standard_normal((30, 24)).astype(np.float32)
+    else:
+        X = rng.integers(-2, 3, (2500, 12)).astype(np.float32)
+        Q = rng.integers(-2, 3, (30, 12)).astype(np.float32)
+    k = 15
+    ids, sc = P.bruteforce_f64(X, Q, k, threads=2)
+    S = Q.astype(np.float64) @ X.astype(np.float64).T
+    for j in range(len(Q)):
+        order = np.lexsort((np.arange(len(X)), -S[j]))[:k]
+        np.testing.assert_array_equal(sc[j], S[j, order])
+        if kind == "gauss":
+            np.testing.assert_array_equal(ids[j], order)
+        else:
+            # equal scores: the oracle's heap keeps the smaller ids (key -n)
+            assert np.all(np.diff(sc[j]) <= 0)
+            assert set(ids[j].tolist()) == set(order.tolist()), j
