@@ -120,8 +120,10 @@ class Workload:
         self.truth = None
 
     def call(self):
+        # on torch's stream: calls queue back to back, results ordered with
+        # torch's own work
         self.idx.query_device(self.dQ.data_ptr(), self.d, self.nq, self.k, self.out.data_ptr(),
-                              self.scores.data_ptr())
+                              self.scores.data_ptr(), self.torch.cuda.current_stream().cuda_stream)
 
     def timed(self, steps, warmup, events=False):
         """QPS and ms per step of `steps` calls after `warmup` untimed ones;
@@ -391,6 +393,9 @@ def main():
     from falconnpp_amd import engine as E
     from falconnpp_amd import synth
 
+    # one stream of torch's own (default priority) for every call: the
+    # engine's calls on it are asynchronous and ordered with torch's work
+    torch.cuda.set_stream(torch.cuda.Stream())
     if "qp_range" in synth.CONFIGS[args.config]:
         serve_batched(args, E, synth, torch)
         return
@@ -483,7 +488,7 @@ def serve_batched(args, E, synth, torch):
 
     def tune():
         log("autotune on the batch ...")
-        t = idx.autotune_batch(dQ.data_ptr(), d, nq, qps_q, ks, kmax)
+        t = idx.autotune_batch(dQ.data_ptr(), d, nq, qps_q, ks, kmax, torch.cuda.current_stream().cuda_stream)
         if args.persistent >= 0:
             idx.set_tuning(E.TUNE_PERSISTENT, args.persistent)
             t["persistent"] = args.persistent
@@ -496,7 +501,8 @@ def serve_batched(args, E, synth, torch):
     sc = torch.zeros((nq, kmax), dtype=torch.float32, device="cuda")
 
     def call():
-        idx.query_batch_device(dQ.data_ptr(), d, nq, qps_q, ks, kmax, out.data_ptr(), sc.data_ptr())
+        idx.query_batch_device(dQ.data_ptr(), d, nq, qps_q, ks, kmax, out.data_ptr(), sc.data_ptr(),
+                               torch.cuda.current_stream().cuda_stream)
 
     def timed_run():
         """Warm-up, the timed steps, then one more call with per-query

@@ -412,6 +412,52 @@ __global__ void k_max_bucket(const uint64_t* __restrict__ off, int64_t nbk, uint
 
 int64_t scan_tmp_elems(int64_t n) { return ceil_div(std::max<int64_t>(n, 1), SCAN_TILE) + 1; }
 
+// Bucket sizes below SIZE_BINS counted per size in LDS, then added to hist;
+// larger ones appended to big (count in *nbig, at most cap stored).
+__global__ __launch_bounds__(256) void k_size_hist(const uint64_t* __restrict__ off, int64_t nbk,
+                                                   uint32_t* __restrict__ hist, uint32_t* __restrict__ big,
+                                                   uint32_t* __restrict__ nbig, uint32_t cap) {
+    __shared__ uint32_t h[SIZE_BINS];
+    for (int i = threadIdx.x; i < SIZE_BINS; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nbk; b += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t sz = off[b + 1] - off[b];
+        if (sz < (uint64_t)SIZE_BINS) {
+            atomicAdd(&h[sz], 1u);
+        } else {
+            const uint32_t p = atomicAdd(nbig, 1u);
+            if (p < cap) big[p] = (uint32_t)(sz > 0xffffffffull ? 0xffffffffull : sz);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < SIZE_BINS; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+bool bucket_size_histogram(const uint64_t* off, int64_t nbk, std::vector<uint32_t>& hist, std::vector<uint32_t>& big,
+                           hipStream_t st) {
+    constexpr uint32_t cap = 1u << 16;
+    uint32_t* d = nullptr;
+    FPP_HIP(hipMalloc(&d, (SIZE_BINS + 1 + cap) * 4));
+    struct Free {
+        uint32_t* p;
+        ~Free() { (void)hipFree(p); }
+    } fr{d};
+    FPP_HIP(hipMemsetAsync(d, 0, (SIZE_BINS + 1) * 4, st));
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nbk, 256), 1024));
+    k_size_hist<<<grid, 256, 0, st>>>(off, nbk, d, d + SIZE_BINS + 1, d + SIZE_BINS, cap);
+    FPP_HIP(hipGetLastError());
+    hist.assign(SIZE_BINS, 0);
+    uint32_t nbig = 0;
+    FPP_HIP(hipMemcpyAsync(hist.data(), d, SIZE_BINS * 4, hipMemcpyDeviceToHost, st));
+    FPP_HIP(hipMemcpyAsync(&nbig, d + SIZE_BINS, 4, hipMemcpyDeviceToHost, st));
+    FPP_HIP(hipStreamSynchronize(st));
+    big.assign(std::min(nbig, cap), 0);
+    if (!big.empty())
+        FPP_HIP(hipMemcpy(big.data(), d + SIZE_BINS + 1, big.size() * 4, hipMemcpyDeviceToHost));
+    return nbig <= cap;
+}
+
 uint32_t max_bucket_size(const uint64_t* off, int64_t nbk, uint32_t* scratch, hipStream_t st) {
     FPP_HIP(hipMemsetAsync(scratch, 0, 4, st));
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nbk, 256), 4096));

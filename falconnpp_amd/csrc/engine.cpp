@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -46,6 +47,10 @@ struct DevBuf {
         n = std::max<size_t>(m, 1);
     }
     T* get() const { return p; }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
 };
 
 struct Index {
@@ -68,54 +73,41 @@ struct Index {
     int64_t n_ids = 0;
     DevBuf<uint32_t> pre_cnt;  // bucket sizes before the alpha / bucketLimit filter (built indexes)
     bool have_pre_cnt = false;
+    // insertable index: every point's pre-filter entries, [l][n*p + i]
+    bool insertable = false, entries_kept = false;
+    DevBuf<uint32_t> entry_codes;
+    DevBuf<float> entry_vals;
     std::vector<uint8_t> bit1, bit2;
 
-    // query scratch
-    DevBuf<float> Qpad;
-    DevBuf<int32_t> qc;
-    DevBuf<float> qv;
-    DevBuf<uint32_t> qflag;
-    DevBuf<uint32_t> probes;
-    DevBuf<uint64_t> ctot;
-    DevBuf<uint64_t> cand_off;
-    DevBuf<int32_t> cand;
-    DevBuf<float> score;
-    DevBuf<uint64_t> ncand;
-    DevBuf<uint32_t> qlist;
-    DevBuf<unsigned char> qwork;
-    DevBuf<int32_t> out_ids;
-    DevBuf<float> out_scores;
-    std::vector<uint64_t> last_cand_off;
-    int last_nq = 0;
-    int gather_threads = 512;
-    bool order_queries = true;
-    int wall_khz = 0;
-    DevBuf<uint32_t> okey, oval, okey2, oval2, ohist, order, order_u;
-    // second stream: exact replay + gather of the flagged queries, beside the
-    // main gather of the others
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    std::vector<uint32_t> flagged;
-    uint32_t max_bucket = 0;  // largest bucket of the built index
+    // bucket sizes of the built index, largest first above SIZE_BINS: they
+    // bound any query's C_tot by the sum of its qProbes largest buckets
+    uint32_t max_bucket = 0;
+    std::vector<uint32_t> size_hist, size_big;
+    bool size_big_complete = false;
     DevBuf<uint32_t> scratch_word;
-    // per-query probe counts and k (batched serving)
-    std::vector<QueryParams> hqp;
-    DevBuf<QueryParams> qparams;
-    // persistent gather: work-queue counters (main, second stream) and the
-    // longest-first order of the queries
-    bool persistent = false;
-    DevBuf<uint32_t> queue;
-    std::vector<uint32_t> h_order;
-    // hashing + probe selection of a batch captured in a hipGraph, replayed
-    // while the batch shape and buffers stay the same
+
+    // query calls: one device workspace carved per call (query copy, hash
+    // lists, flags, probes, counts, ordering, replay work, and the gather's
+    // per-workgroup candidate scratch, at most ws_budget bytes of it)
+    DevBuf<unsigned char> ws;
+    size_t ws_budget = size_t(8) << 30;
+    DevBuf<int32_t> out_ids;  // host-pointer query calls
+    std::vector<QueryParams> hqp;  // per-query probe counts and k (batched serving)
+    int gather_threads = 512;
+    int order_mode = 1;            // ORDER_* below
+    bool keep_candidates = false;  // every query's candidates kept after the call (inspection)
+    int replay_mode = 0;           // tie replay: 0 per call, 1 separate kernels, 2 inside the gather
+    DevBuf<int32_t> keep_cand, keep_part;
+    DevBuf<uint64_t> keep_off;
+    int wall_khz = 0;
+    // the whole query pipeline of a call captured in a hipGraph, replayed
+    // while the call's shape and buffers stay the same
     bool use_graph = false;
     bool tune_verbose = false;  // autotuner prints each timed shape
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uint64_t> graph_key;
     uint64_t build_gen = 0;
-    uint64_t last_nprobes = 0;
-    std::vector<uint64_t> last_ts;  // per-query phase stamps of the last stamped call
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -123,8 +115,20 @@ struct Index {
         graph = nullptr;
         graph_key.clear();
     }
-    DevBuf<uint64_t> ohist_off, oscan;
-    DevBuf<uint64_t> phase_ts;
+    // the last query call: its device arrays (statistics are summed from them
+    // when asked for) and, in keep-candidates mode, every query's segment
+    struct LastCall {
+        int nq = 0;
+        uint64_t nprobes = 0;
+        const uint32_t* probes = nullptr;
+        const uint64_t* ctot = nullptr;
+        const uint64_t* ncand = nullptr;
+        const uint32_t* qflag = nullptr;
+        hipStream_t st = nullptr;
+        bool summed = true;
+        std::vector<uint64_t> seg;  // keep-candidates: segment offsets, nq + 1
+    } last;
+    std::vector<uint64_t> last_ts;  // per-query phase stamps of the last stamped call
 
     int64_t stat_hash_ties = 0, stat_bucket_ties = 0, stat_ctot = 0, stat_cuniq = 0, stat_qties = 0;
     int stat_probe_path = -1;  // probe selection of the last query call (PROBE_*)
@@ -150,7 +154,7 @@ struct Index {
     // hash, probe select, tie replay, gather+rerank (ms); then mean per-query
     // microseconds of the gather kernel's phases: bucket gather + dedup,
     // candidate compaction, scoring, top-k
-    double stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double stage_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     void mark(int i, hipStream_t st) {
         if (!profiling) return;
         if (!ev[i]) FPP_HIP(hipEventCreate(&ev[i]));
@@ -164,9 +168,6 @@ struct Index {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         drop_graph();
-        if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_join) (void)hipEventDestroy(ev_join);
-        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -200,6 +201,18 @@ hipStream_t pick_stream(Index& I, void* s) {
 }
 
 int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// rows x D floats from host src (row stride ld floats) to dense device dst:
+// one plain copy when the rows are contiguous (a 2D copy of many short rows
+// from pageable memory goes row by row)
+void copy_rows_h2d(float* dst, const float* src, int64_t ld, int64_t rows, int D, hipStream_t st) {
+    if (rows <= 0) return;
+    if (ld == D)
+        FPP_HIP(hipMemcpyAsync(dst, src, (size_t)rows * D * 4, hipMemcpyHostToDevice, st));
+    else
+        FPP_HIP(hipMemcpy2DAsync(dst, (size_t)D * 4, src, (size_t)ld * 4, (size_t)D * 4, rows, hipMemcpyHostToDevice,
+                                 st));
+}
 
 // src/FalconnPP.h:52-75 with libstdc++'s own engine and distribution.
 void make_sign_bits(Index& I) {
@@ -242,35 +255,23 @@ void check_params(const Index& I) {
     FPP_REQUIRE((int64_t)I.n_tables * I.NB() < (int64_t)INT32_MAX, FPP_EINVAL, "n_tables * 4 * n_proj^2 must be < 2^31");
 }
 
-void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
-    check_params(I);
-    const int64_t N = I.n_points;
-    const int D = I.n_features, L = I.n_tables, p = I.iProbes, NB = I.NB();
-    FPP_REQUIRE(ld >= D, FPP_EINVAL, "leading dimension of X must be >= n_features");
-    I.built = false;
-    I.lp_ready = 0;
-    make_sign_bits(I);
-    // centre (deterministic FP64 chunk sums) and the centred copy
-    I.ldx = round_up(D, 4);
-    const int64_t chunk = 4096;
-    const int nchunks = (int)ceil_div(N, chunk);
-    {
-        DevBuf<double> partial;
-        partial.reserve((size_t)nchunks * D);
-        I.center.reserve(D);
-        launch_center_partial(dX, ld, N, D, partial.get(), chunk, nchunks, st);
-        launch_center_final(partial.get(), nchunks, D, N, I.center.get(), st);
-        I.Xc.reserve((size_t)N * I.ldx);
-        launch_center_apply(dX, ld, N, D, I.center.get(), I.Xc.get(), I.ldx, st);
-        FPP_HIP(hipStreamSynchronize(st));
-    }
+// Bucket sizes of a built or loaded index: the largest, and the histogram
+// that bounds C_tot per query.
+void index_sizes(Index& I, hipStream_t st) {
+    const int64_t nbk = (int64_t)I.n_tables * I.NB();
+    I.scratch_word.reserve(1);
+    I.max_bucket = max_bucket_size(I.bucket_off.get(), nbk, I.scratch_word.get(), st);
+    I.size_big_complete = bucket_size_histogram(I.bucket_off.get(), nbk, I.size_hist, I.size_big, st);
+    std::sort(I.size_big.begin(), I.size_big.end(), std::greater<uint32_t>());
+}
+
+// Build-side hash of the N centred rows at Xc (row stride I.ldx) into
+// codes/vals[l*N*p + n*p + i], the items whose result depends on an exact
+// float tie replayed with the reference's heaps; returns how many were.
+int64_t hash_rows(Index& I, const float* Xc, int64_t N, uint32_t* codes, float* vals, hipStream_t st) {
     const HashGeom g = I.hgeom();
-    const int64_t per_table = N * p;
-    const int64_t total = per_table * L;
-    DevBuf<uint32_t> codes;
-    DevBuf<float> vals;
-    codes.reserve(total);
-    vals.reserve(total);
+    const int p = I.iProbes, L = I.n_tables;
+    int64_t ties = 0;
     // hashing + exact replay of tie-dependent items
     {
         const uint32_t cap = (uint32_t)std::min<int64_t>(N * L, std::max<int64_t>(1 << 16, N * L / 16));
@@ -279,14 +280,14 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
         flags.reserve(cap);
         fcount.reserve(1);
         FPP_HIP(hipMemsetAsync(fcount.get(), 0, 4, st));
-        launch_hash_build(g, I.Xc.get(), I.ldx, N, p, I.masks.get(), codes.get(), vals.get(), flags.get(),
+        launch_hash_build(g, Xc, I.ldx, N, p, I.masks.get(), codes, vals, flags.get(),
                           fcount.get(), cap, st);
         uint32_t nflag = 0;
         FPP_HIP(hipMemcpyAsync(&nflag, fcount.get(), 4, hipMemcpyDeviceToHost, st));
         FPP_HIP(hipStreamSynchronize(st));
         const bool all = nflag > cap;
         const uint64_t count = all ? (uint64_t)N * L : nflag;
-        I.stat_hash_ties = (int64_t)nflag;
+        ties = (int64_t)nflag;
         if (count) {
             // bounded workspace: replay in slices
             const int64_t per_item = 2 * (int64_t)I.fhtDim + 8 * p + 8;
@@ -302,17 +303,29 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
                     DevBuf<uint64_t> dl;
                     dl.reserve(c);
                     FPP_HIP(hipMemcpyAsync(dl.get(), lst.data(), c * 8, hipMemcpyHostToDevice, st));
-                    launch_hash_build_exact(g, I.Xc.get(), I.ldx, N, p, I.masks.get(), codes.get(), vals.get(),
+                    launch_hash_build_exact(g, Xc, I.ldx, N, p, I.masks.get(), codes, vals,
                                             dl.get(), c, 0, work.get(), st);
                     FPP_HIP(hipStreamSynchronize(st));
                 } else {
-                    launch_hash_build_exact(g, I.Xc.get(), I.ldx, N, p, I.masks.get(), codes.get(), vals.get(),
+                    launch_hash_build_exact(g, Xc, I.ldx, N, p, I.masks.get(), codes, vals,
                                             flags.get() + s, c, 0, work.get(), st);
                 }
             }
             FPP_HIP(hipStreamSynchronize(st));
         }
     }
+    return ties;
+}
+
+// Bucket fill and filter over the pre-filter entries of N points, laid out
+// as hash_rows writes them (entry n*p + i of table l belongs to point n, so
+// a stable sort by bucket keeps the reference's insertion order): bucket
+// sizes, the alpha / bucketLimit rule, CSR offsets, then per table the copy
+// of whole buckets, the selection of the iLimit largest values and the exact
+// replay of buckets whose cut falls between equal values.
+void fill_tables(Index& I, const uint32_t* codes, const float* vals, int64_t N, hipStream_t st) {
+    const int L = I.n_tables, p = I.iProbes, NB = I.NB();
+    const int64_t per_table = N * p;
     // bucket sizes, limits and CSR offsets over all tables
     const int64_t nbk = (int64_t)L * NB;
     // pre-filter bucket sizes stay with the index (inspection)
@@ -321,7 +334,7 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
     cnt.reserve(nbk);
     ksz.reserve(nbk);
     FPP_HIP(hipMemsetAsync(cnt.get(), 0, nbk * 4, st));
-    launch_bucket_count(codes.get(), per_table, L, NB, cnt.get(), st);
+    launch_bucket_count(codes, per_table, L, NB, cnt.get(), st);
     launch_bucket_limit(cnt.get(), nbk, I.bucket_scale, p, I.bucket_minSize, ksz.get(), st);
     const int64_t radix_hist = radix_hist_elems(per_table);
     DevBuf<uint64_t> scan_tmp;
@@ -353,8 +366,8 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
         const uint32_t* cnt_l = cnt.get() + (size_t)l * NB;
         const uint32_t* ksz_l = ksz.get() + (size_t)l * NB;
         const uint64_t* off_l = I.bucket_off.get() + (size_t)l * NB;
-        const float* vals_l = vals.get() + (size_t)l * per_table;
-        FPP_HIP(hipMemcpyAsync(k0.get(), codes.get() + (size_t)l * per_table, per_table * 4, hipMemcpyDeviceToDevice,
+        const float* vals_l = vals + (size_t)l * per_table;
+        FPP_HIP(hipMemcpyAsync(k0.get(), codes + (size_t)l * per_table, per_table * 4, hipMemcpyDeviceToDevice,
                                st));
         uint32_t *sk = nullptr, *sv = nullptr;
         radix_sort_pairs(k0.get(), v0.get(), k1.get(), v1.get(), per_table, key_bits, true, hist.get(),
@@ -397,10 +410,106 @@ void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
         }
     }
     FPP_HIP(hipStreamSynchronize(st));
-    I.scratch_word.reserve(1);
-    I.max_bucket = max_bucket_size(I.bucket_off.get(), (int64_t)L * NB, I.scratch_word.get(), st);
+    index_sizes(I, st);
     I.stat_bucket_ties = bucket_ties;
     I.have_pre_cnt = true;
+}
+
+void do_build(Index& I, const float* dX, int64_t ld, hipStream_t st) {
+    check_params(I);
+    const int64_t N = I.n_points;
+    const int D = I.n_features, L = I.n_tables, p = I.iProbes, NB = I.NB();
+    FPP_REQUIRE(ld >= D, FPP_EINVAL, "leading dimension of X must be >= n_features");
+    I.built = false;
+    I.lp_ready = 0;
+    make_sign_bits(I);
+    // centre (deterministic FP64 chunk sums) and the centred copy
+    I.ldx = round_up(D, 4);
+    const int64_t chunk = 4096;
+    const int nchunks = (int)ceil_div(N, chunk);
+    {
+        DevBuf<double> partial;
+        partial.reserve((size_t)nchunks * D);
+        I.center.reserve(D);
+        launch_center_partial(dX, ld, N, D, partial.get(), chunk, nchunks, st);
+        launch_center_final(partial.get(), nchunks, D, N, I.center.get(), st);
+        I.Xc.reserve((size_t)N * I.ldx);
+        launch_center_apply(dX, ld, N, D, I.center.get(), I.Xc.get(), I.ldx, st);
+        FPP_HIP(hipStreamSynchronize(st));
+    }
+    const int64_t total = N * p * L;
+    DevBuf<uint32_t> codes;
+    DevBuf<float> vals;
+    codes.reserve(total);
+    vals.reserve(total);
+    I.stat_hash_ties = hash_rows(I, I.Xc.get(), N, codes.get(), vals.get(), st);
+    fill_tables(I, codes.get(), vals.get(), N, st);
+    // an insertable index keeps every point's pre-filter entries
+    I.entry_codes.reset();
+    I.entry_vals.reset();
+    if (I.insertable) {
+        I.entry_codes.swap(codes);
+        I.entry_vals.swap(vals);
+    }
+    I.entries_kept = I.insertable;
+    I.build_gen++;
+    I.drop_graph();
+    I.built = true;
+}
+
+// Appends m points (dX: D x m col-major, leading dimension ld) as ids
+// n_points .. n_points+m-1: centred with the index's centre, hashed with its
+// sign bits, their entries placed after the old points' in every table, and
+// every bucket filtered again over old and new entries.  The index then
+// equals a build of all points with that centre.
+void do_insert(Index& I, const float* dX, int64_t ld, int m, hipStream_t st) {
+    FPP_REQUIRE(I.built, FPP_ESTATE, "insert called before build (or after clear)");
+    FPP_REQUIRE(I.entries_kept, FPP_ESTATE,
+                "the index keeps no pre-filter entries: build it after fpp_set_insertable(h, 1)");
+    FPP_REQUIRE(m >= 1, FPP_EINVAL, "number of inserted points must be >= 1");
+    FPP_REQUIRE(ld >= I.n_features, FPP_EINVAL, "leading dimension of X must be >= n_features");
+    const int64_t N0 = I.n_points, N1 = N0 + m;
+    const int D = I.n_features, L = I.n_tables, p = I.iProbes;
+    FPP_REQUIRE(N1 <= INT32_MAX && N1 * p < (int64_t)UINT32_MAX, FPP_EINVAL, "n_points * iProbes must be < 2^32");
+    // centred rows: the old ones, then the new ones with the same centre
+    {
+        DevBuf<float> Xn;
+        Xn.reserve((size_t)N1 * I.ldx);
+        FPP_HIP(hipMemcpyAsync(Xn.get(), I.Xc.get(), (size_t)N0 * I.ldx * 4, hipMemcpyDeviceToDevice, st));
+        launch_center_apply(dX, ld, m, D, I.center.get(), Xn.get() + (size_t)N0 * I.ldx, I.ldx, st);
+        FPP_HIP(hipStreamSynchronize(st));
+        I.Xc.swap(Xn);
+    }
+    DevBuf<uint32_t> nc;
+    DevBuf<float> nv;
+    nc.reserve((size_t)m * p * L);
+    nv.reserve((size_t)m * p * L);
+    const int64_t ties = hash_rows(I, I.Xc.get() + (size_t)N0 * I.ldx, m, nc.get(), nv.get(), st);
+    DevBuf<uint32_t> codes;
+    DevBuf<float> vals;
+    codes.reserve((size_t)N1 * p * L);
+    vals.reserve((size_t)N1 * p * L);
+    const size_t o0 = (size_t)N0 * p, o1 = (size_t)N1 * p, om = (size_t)m * p;
+    for (int l = 0; l < L; ++l) {
+        FPP_HIP(hipMemcpyAsync(codes.get() + l * o1, I.entry_codes.get() + l * o0, o0 * 4, hipMemcpyDeviceToDevice, st));
+        FPP_HIP(hipMemcpyAsync(codes.get() + l * o1 + o0, nc.get() + l * om, om * 4, hipMemcpyDeviceToDevice, st));
+        FPP_HIP(hipMemcpyAsync(vals.get() + l * o1, I.entry_vals.get() + l * o0, o0 * 4, hipMemcpyDeviceToDevice, st));
+        FPP_HIP(hipMemcpyAsync(vals.get() + l * o1 + o0, nv.get() + l * om, om * 4, hipMemcpyDeviceToDevice, st));
+    }
+    nc.reset();
+    nv.reset();
+    I.entry_codes.reset();
+    I.entry_vals.reset();
+    I.built = false;
+    I.n_points = (int)N1;
+    fill_tables(I, codes.get(), vals.get(), N1, st);
+    I.entry_codes.swap(codes);
+    I.entry_vals.swap(vals);
+    I.stat_hash_ties = ties;
+    I.lp_ready = 0;
+    I.Xb.reset();
+    I.X8.reset();
+    I.s8.reset();
     I.build_gen++;
     I.drop_graph();
     I.built = true;
@@ -455,11 +564,87 @@ void check_plan(const Index& I, const QueryPlan& pl, bool batched) {
                 "qProbes too large for a batched call: more than 31 probes per layer");
 }
 
+// Order in which the persistent gather takes the queries.
+enum { ORDER_IDENTITY = 0, ORDER_BUCKET = 1, ORDER_LONGEST = 2 };
+
+// Upper bound of C_tot over queries with at most qp probes: the sum of the qp
+// largest buckets of the index (a query's probes are distinct buckets).
+uint64_t ctot_bound(const Index& I, uint64_t qp) {
+    if (!I.size_big_complete) return qp * (uint64_t)I.max_bucket;
+    uint64_t s = 0, left = qp;
+    for (uint32_t v : I.size_big) {
+        if (!left) return s;
+        s += v;
+        --left;
+    }
+    for (int b = (int)I.size_hist.size() - 1; b > 0 && left; --b) {
+        const uint64_t c = std::min<uint64_t>(left, I.size_hist[b]);
+        s += c * (uint64_t)b;
+        left -= c;
+    }
+    return s;
+}
+
+// Byte offsets of one call's arrays in its device workspace.
+struct QueryLayout {
+    size_t Qpad, qc, qv, qflag, probes, ctot, ncand, qparams, okey, oval, okey2, oval2, ohist, ohist_off, oscan, qlist,
+        counters, rwork, phase_ts, cand, part, total;
+};
+constexpr int N_COUNTERS = 16;  // [0, 8) queue heads, [8] replay claim, [9] flagged queries
+
+QueryLayout query_layout(int nq, int L, int PL, uint64_t nprobes, bool batched, unsigned rareas, int64_t wpq,
+                         bool stamps, unsigned slots, uint64_t slot_cand, uint64_t slot_part) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        off = (off + 255) & ~size_t(255);
+        const size_t o = off;
+        off += std::max<size_t>(bytes, 1);
+        return o;
+    };
+    QueryLayout w{};
+    const size_t n = (size_t)nq;
+    const int64_t rh = radix_hist_elems(nq);
+    w.Qpad = 0;  // filled in by the caller's ldx
+    w.qc = take(n * L * 2 * PL * 4);
+    w.qv = take(n * L * 2 * PL * 4);
+    w.qflag = take(n * 4);
+    w.probes = take(nprobes * 4);
+    w.ctot = take(n * 8);
+    w.ncand = take(n * 8);
+    w.qparams = take(batched ? n * sizeof(QueryParams) : 0);
+    w.okey = take(n * 4);
+    w.oval = take(n * 4);
+    w.okey2 = take(n * 4);
+    w.oval2 = take(n * 4);
+    w.ohist = take((size_t)rh * 4);
+    w.ohist_off = take(((size_t)rh + 1) * 8);
+    w.oscan = take((size_t)scan_tmp_elems(rh) * 8);
+    w.qlist = take(n * 4);
+    w.counters = take(N_COUNTERS * 4);
+    w.rwork = take((size_t)rareas * (size_t)wpq);
+    w.phase_ts = take(stamps ? n * 5 * 8 : 0);
+    w.cand = take((size_t)slots * slot_cand * 4);
+    w.part = take((size_t)slots * slot_part * 4);
+    w.total = (off + 255) & ~size_t(255);
+    return w;
+}
+
 // One batch of queries.  k is the number of neighbours, or with per-query
 // counts (h_qprobes, h_k: host arrays of nq) the largest k and the row stride
-// of the outputs.
+// of the outputs.  Everything runs on st in one sequence with no host read:
+//  1. queries copied (padded), hashed (k_hash_query), probes selected
+//     (k_probe_select), the tie-flagged queries listed on the device and
+//     their probes replayed exactly (k_probe_replay), so C_tot is exact;
+//  2. the queries ordered on the device (by their best table-0 bucket, in 8
+//     runs for the 8 XCDs, or longest first) for the persistent gather, whose
+//     workgroups each own a scratch slot bounded by the C_tot bound above;
+//  3. one persistent gather (dedup, scoring, top-k).
+// With the graph on, the whole sequence is captured once and replayed.
+// fixed_ws (no allocation at all): the workspace is fixed_bytes there.
+// sync_end: wait for the results before returning.
 void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_out, float* d_scores,
-              hipStream_t st, const int32_t* h_qprobes = nullptr, const int32_t* h_k = nullptr) {
+              hipStream_t st, const int32_t* h_qprobes = nullptr, const int32_t* h_k = nullptr,
+              unsigned char* fixed_ws = nullptr, size_t fixed_bytes = 0, bool sync_end = false) {
     FPP_REQUIRE(I.built, FPP_ESTATE, "query called before build (or after clear)");
     FPP_REQUIRE(nq >= 0, FPP_EINVAL, "number of queries must be >= 0");
     FPP_REQUIRE(k >= 1 && k <= 1024, FPP_EINVAL, "n_neighbors must be in [1, 1024]");
@@ -468,6 +653,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     FPP_REQUIRE(ldq >= I.n_features, FPP_EINVAL, "leading dimension of Q must be >= n_features");
     FPP_REQUIRE((h_qprobes == nullptr) == (h_k == nullptr), FPP_EINVAL, "per-query qProbes and k go together");
     const bool batched = h_qprobes != nullptr;
+    const bool fixed = fixed_ws != nullptr;
     QueryPlan pl;
     int qp_max = I.qProbes;
     uint64_t nprobes = 0;
@@ -495,167 +681,43 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     }
     // more probes per layer than the fast kernels take: every query's probes
     // come from the exact replay (its per-table lists are rebuilt from the
-    // rotated query, qflag bit 0), on the replay stream
+    // rotated query, qflag bit 0)
     const bool exact_only = pl.perLayer > FAST_MAX_PL;
-    I.stat_ctot = I.stat_cuniq = I.stat_qties = 0;
-    I.last_nq = nq;
-    if (nq == 0) return;
-    const HashGeom hg = I.hgeom();
-    ProbeGeom pg{I.n_tables, I.NB(), pl.perLayer, pl.perTable, qp_max, 2 * I.n_proj};
-    if (batched) {
-        I.qparams.reserve(nq);
-        FPP_HIP(hipMemcpyAsync(I.qparams.get(), I.hqp.data(), (size_t)nq * sizeof(QueryParams),
-                               hipMemcpyHostToDevice, st));
-        pg.qp = I.qparams.get();
-    }
     const int L = I.n_tables, PL = pl.perLayer;
+    const HashGeom hg = I.hgeom();
+    ProbeGeom pg{L, I.NB(), PL, pl.perTable, qp_max, 2 * I.n_proj};
     // a pool that fits no probe-selection kernel: every query's probes come
     // from the exact replay, which reuses the fast per-layer lists
     const int probe_path = exact_only ? PROBE_EXACT : probe_select_path(pg);
     I.stat_probe_path = probe_path;
-    I.Qpad.reserve((size_t)nq * I.ldx);
-    I.qc.reserve((size_t)nq * L * 2 * PL);
-    I.qv.reserve((size_t)nq * L * 2 * PL);
-    I.qflag.reserve(nq);
-    I.probes.reserve(nprobes);
-    I.ctot.reserve(nq);
-    I.last_nprobes = nprobes;
-    // hashing and probe selection: queries padded to the scored row stride,
-    // flags cleared by fill kernels (no memset nodes in a captured graph)
-    auto phase1 = [&](bool marks) {
-        launch_fill_u32(reinterpret_cast<uint32_t*>(I.Qpad.get()), 0u, (int)((size_t)nq * I.ldx), st);
-        FPP_HIP(hipMemcpy2DAsync(I.Qpad.get(), I.ldx * 4, dQ, ldq * 4, (size_t)I.n_features * 4, nq,
-                                 hipMemcpyDeviceToDevice, st));
-        // bit 0: per-layer lists rebuilt by the replay; bit 1: probes replayed
-        launch_fill_u32(I.qflag.get(), exact_only ? 3u : (probe_path == PROBE_EXACT ? 2u : 0u), nq, st);
-        if (marks) I.mark(0, st);
-        if (!exact_only)
-            launch_hash_query(hg, I.Qpad.get(), I.ldx, nq, PL, I.masks.get(), I.qc.get(), I.qv.get(),
-                              I.qflag.get(), st);
-        if (marks) I.mark(1, st);
-        if (probe_path != PROBE_EXACT)
-            launch_probe_select(pg, nq, I.qc.get(), I.qv.get(), I.bucket_off.get(), I.probes.get(), I.ctot.get(),
-                                I.qflag.get(), st);
-    };
-    if (I.use_graph && probe_path != PROBE_EXACT) {
-        const std::vector<uint64_t> key = {
-            (uint64_t)nq, (uint64_t)ldq, (uint64_t)(uintptr_t)dQ, (uint64_t)(uintptr_t)st, I.build_gen,
-            (uint64_t)(uintptr_t)I.Qpad.get(), (uint64_t)(uintptr_t)I.qc.get(), (uint64_t)(uintptr_t)I.qv.get(),
-            (uint64_t)(uintptr_t)I.qflag.get(), (uint64_t)(uintptr_t)I.probes.get(),
-            (uint64_t)(uintptr_t)I.ctot.get(), (uint64_t)(uintptr_t)pg.qp, (uint64_t)pg.PL,
-            (uint64_t)pg.perTable, (uint64_t)pg.qProbes};
-        if (!I.graph_exec || key != I.graph_key) {
-            I.drop_graph();
-            FPP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            try {
-                phase1(false);
-            } catch (...) {
-                hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(st, &g);
-                if (g) (void)hipGraphDestroy(g);
-                throw;
-            }
-            FPP_HIP(hipStreamEndCapture(st, &I.graph));
-            FPP_HIP(hipGraphInstantiate(&I.graph_exec, I.graph, nullptr, nullptr, 0));
-            I.graph_key = key;
-        }
-        // hashing is timed together with probe selection here
-        I.mark(0, st);
-        I.mark(1, st);
-        FPP_HIP(hipGraphLaunch(I.graph_exec, st));
-    } else {
-        phase1(true);
+    I.last = Index::LastCall{};
+    I.last.nq = nq;
+    I.last.nprobes = nprobes;
+    if (nq == 0) return;
+
+    // gather configuration
+    const bool lowp = I.precision != 0;
+    if (lowp) {
+        if (fixed)
+            FPP_REQUIRE(I.lp_ready == I.precision && (I.precision != 2 || I.lp_per_row_made == I.fp8_per_row),
+                        FPP_ESTATE, "the reduced-precision copy is made by a first query through fpp_query_device");
+        else
+            ensure_lowprec(I, st);
     }
-    I.mark(2, st);
-    // flags and candidate totals of the fast path
-    std::vector<uint32_t> flag(nq);
-    std::vector<uint64_t> ct(nq);
-    FPP_HIP(hipMemcpyAsync(flag.data(), I.qflag.get(), (size_t)nq * 4, hipMemcpyDeviceToHost, st));
-    FPP_HIP(hipMemcpyAsync(ct.data(), I.ctot.get(), (size_t)nq * 8, hipMemcpyDeviceToHost, st));
-    FPP_HIP(hipStreamSynchronize(st));
-    std::vector<uint32_t>& lst = I.flagged;
-    lst.clear();
-    for (int q = 0; q < nq; ++q)
-        if (flag[q]) lst.push_back((uint32_t)q);
-    const uint32_t nl = (uint32_t)lst.size();
-    const int nu = nq - (int)nl;
-    I.stat_qties = (int64_t)nl;
+    // 256-thread workgroups run two per CU (dedup window sized for half the
+    // LDS), larger ones one per CU
+    const int gthreads = lowp ? (I.gather_threads <= 256 ? 256 : 512) : I.gather_threads;
     GatherArgs ga;
     ga.N = I.n_points;
     ga.D = I.n_features;
     ga.ldx = I.ldx;
     ga.qProbes = I.qProbes;
     ga.k = k;
-    if (batched) {
-        ga.qp = I.qparams.get();
-        ga.ldo = k;
-    }
-    const bool lowp = I.precision != 0;
-    // 256-thread workgroups run two per CU (dedup window sized for half the
-    // LDS), larger ones one per CU
-    const int gthreads = lowp ? (I.gather_threads <= 256 ? 256 : 512) : I.gather_threads;
+    if (batched) ga.ldo = k;
     ga.window_bits = gather_window_bits(I.n_points,
                                         lowp ? lowprec_extra_words(I.refine_count(k), gthreads)
                                              : (int64_t)I.gather_threads + 256 + 2048,
                                         gthreads <= 256 ? 2 : 1);
-    // candidate segments.  One dedup window: the unique candidates, at most
-    // min(C_tot, N).  Several windows: the scatter by window holds every
-    // bucket id, duplicates included, so C_tot itself.  A flagged query (its
-    // probes are replayed later) is bounded by its qProbes largest buckets.
-    const bool one_window = I.n_points <= ga.window_bits;
-    std::vector<uint64_t> off(nq + 1);
-    off[0] = 0;
-    for (int q = 0; q < nq; ++q) {
-        const uint64_t qpq = batched ? (uint64_t)I.hqp[q].qProbes : (uint64_t)I.qProbes;
-        const uint64_t bound = flag[q] ? qpq * I.max_bucket : ct[q];
-        off[q + 1] = off[q] + (one_window ? std::min<uint64_t>(bound, (uint64_t)I.n_points) : bound);
-    }
-    I.cand_off.reserve(nq + 1);
-    I.cand.reserve(off[nq]);
-    I.score.reserve(off[nq]);
-    I.ncand.reserve(nq);
-    ga.part = reinterpret_cast<int32_t*>(I.score.get());
-    FPP_HIP(hipMemcpyAsync(I.cand_off.get(), off.data(), (nq + 1) * 8, hipMemcpyHostToDevice, st));
-    const uint32_t* order = nullptr;
-    if (I.order_queries && nq > 1 && !I.persistent) {
-        const int bits = ilog2_exact(I.NB());
-        const int64_t rh = radix_hist_elems(nq);
-        I.okey.reserve(nq);
-        I.oval.reserve(nq);
-        I.okey2.reserve(nq);
-        I.oval2.reserve(nq);
-        I.ohist.reserve(rh);
-        I.ohist_off.reserve(rh + 1);
-        I.oscan.reserve(scan_tmp_elems(rh));
-        I.order.reserve(nq);
-        launch_order_keys(I.qc.get(), nq, L, PL, 2 * I.n_proj, I.okey.get(), st);
-        uint32_t *sk = nullptr, *sv = nullptr;
-        radix_sort_pairs(I.okey.get(), I.oval.get(), I.okey2.get(), I.oval2.get(), nq, bits, true, I.ohist.get(),
-                         I.ohist_off.get(), I.oscan.get(), &sk, &sv, st);
-        launch_xcd_order(sv, nq, 8, I.order.get(), st);
-        order = I.order.get();
-    }
-    if (I.persistent) {
-        // work queue, longest first: the unflagged queries by C_tot
-        I.h_order.clear();
-        for (int q = 0; q < nq; ++q)
-            if (!flag[q]) I.h_order.push_back((uint32_t)q);
-        std::stable_sort(I.h_order.begin(), I.h_order.end(), [&](uint32_t a, uint32_t b) { return ct[a] > ct[b]; });
-        I.order_u.reserve(nq);
-        if (nu > 0)
-            FPP_HIP(hipMemcpyAsync(I.order_u.get(), I.h_order.data(), (size_t)nu * 4, hipMemcpyHostToDevice, st));
-        order = I.order_u.get();
-        I.queue.reserve(2);
-    } else if (nl > 0) {
-        I.order_u.reserve(nq);
-        launch_split_order(order, nq, I.qflag.get(), I.order_u.get(), st);
-        order = I.order_u.get();
-    }
-    if (I.phase_stamps) {
-        I.phase_ts.reserve((size_t)nq * 5);
-        ga.phase_ts = I.phase_ts.get();
-    }
-    if (lowp) ensure_lowprec(I, st);
     LowPrecArgs lp;
     if (lowp) {
         lp.format = I.precision;
@@ -668,68 +730,233 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         lp.uscale = I.precision == 2 ? I.s8_shared : 0.0f;
         lp.threads = gthreads;
     }
-    auto gather = [&](const GatherArgs& a, int n, hipStream_t s2) {
-        if (lowp)
-            launch_gather_score_lp(a, lp, n, I.probes.get(), I.bucket_off.get(), I.ids.get(), I.Xc.get(),
-                                   I.Qpad.get(), I.ldx, I.cand_off.get(), I.cand.get(), I.ncand.get(), d_out,
-                                   d_scores, s2);
-        else
-            launch_gather_score_f32(a, n, I.probes.get(), I.bucket_off.get(), I.ids.get(), I.Xc.get(), I.Qpad.get(),
-                                    I.ldx, I.cand_off.get(), I.cand.get(), I.score.get(), I.ncand.get(), d_out,
-                                    d_scores, I.gather_threads, s2);
+    const bool keep = I.keep_candidates && !fixed;
+    // exact replay of the tie-flagged queries: inside the gather (the
+    // workgroup that claims such a query replays it first, in a work area of
+    // its own; those queries head the work queue), or, when every query's
+    // C_tot must be known before the gather (kept candidates) or a table's
+    // replay does not fit a workgroup's LDS, by separate kernels before it
+    ReplayArgs rp;
+    rp.hg = hg;
+    rp.pg = pg;
+    const int64_t wpq = probe_replay_work_bytes(pg);
+    const size_t rlds = replay_lds_bytes(rp, gthreads);
+    const bool in_gather = !keep && rlds > 0 && I.replay_mode != 1;
+    const size_t glds =
+        std::max(lowp ? gather_lp_lds(ga, lp) : gather_f32_lds(ga, gthreads), in_gather ? rlds : size_t(0));
+    unsigned grid = persistent_grid(nq, glds, gthreads);
+    // per-workgroup scratch: the unique candidates (at most min(C_tot, N)) and
+    // the window scatter (C_tot with several dedup windows) or score area
+    const uint64_t bound = std::max<uint64_t>(1, ctot_bound(I, (uint64_t)qp_max));
+    const bool one_window = I.n_points <= ga.window_bits;
+    const uint64_t slot_cand = std::min<uint64_t>(bound, (uint64_t)I.n_points);
+    const uint64_t slot_part = one_window ? slot_cand : bound;
+    // separate replay kernels: one work area for each of the first rcap
+    // flagged queries, and rover persistent workgroups for any beyond
+    unsigned rcap = in_gather ? 0u : (unsigned)std::min(nq, 512);
+    unsigned rover = !in_gather && nq > (int)rcap ? (unsigned)std::min(nq - (int)rcap, 256) : 0u;
+    const bool stamps = I.phase_stamps && !fixed;
+    const size_t qbytes = (size_t)nq * I.ldx * 4;
+    auto layout = [&](unsigned slots, unsigned rareas) {
+        QueryLayout w = query_layout(nq, L, PL, nprobes, batched, rareas, wpq, stamps, keep ? 0 : slots,
+                                     keep ? 0 : slot_cand, keep ? 0 : slot_part);
+        // the padded queries go first
+        const size_t shift = (qbytes + 255) & ~size_t(255);
+        for (size_t* f : {&w.qc, &w.qv, &w.qflag, &w.probes, &w.ctot, &w.ncand, &w.qparams, &w.okey, &w.oval,
+                          &w.okey2, &w.oval2, &w.ohist, &w.ohist_off, &w.oscan, &w.qlist, &w.counters, &w.rwork,
+                          &w.phase_ts, &w.cand, &w.part})
+            *f += shift;
+        w.total += shift;
+        return w;
     };
-    if (nl > 0) {
-        // flagged queries on the second stream: exact replay of their probes,
-        // then their gather; overlapped with the main gather below
-        if (!I.side) FPP_HIP(hipStreamCreateWithFlags(&I.side, hipStreamNonBlocking));
-        if (!I.ev_fork) FPP_HIP(hipEventCreateWithFlags(&I.ev_fork, hipEventDisableTiming));
-        if (!I.ev_join) FPP_HIP(hipEventCreateWithFlags(&I.ev_join, hipEventDisableTiming));
-        FPP_HIP(hipEventRecord(I.ev_fork, st));
-        FPP_HIP(hipStreamWaitEvent(I.side, I.ev_fork, 0));
-        const int64_t wpq = probe_exact_work_bytes(hg, pg);
-        // the per-table lists of at most 1 GiB of queries at a time
-        const uint32_t slice = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(nl, (int64_t(1) << 30) / wpq));
-        I.qlist.reserve(nl);
-        I.qwork.reserve((size_t)wpq * slice);
-        FPP_HIP(hipMemcpyAsync(I.qlist.get(), lst.data(), (size_t)nl * 4, hipMemcpyHostToDevice, I.side));
-        I.mark(3, I.side);
-        for (uint32_t s0 = 0; s0 < nl; s0 += slice)
-            launch_probe_exact(hg, pg, I.Qpad.get(), I.ldx, I.masks.get(), I.qlist.get() + s0,
-                               std::min(slice, nl - s0), I.qflag.get(), I.qc.get(), I.qv.get(), I.bucket_off.get(),
-                               I.probes.get(), I.ctot.get(), I.qwork.get(), wpq, I.side);
-        I.mark(4, I.side);
-        GatherArgs gf = ga;
-        gf.order = I.qlist.get();
-        if (I.persistent) {
-            gf.queue = I.queue.get() + 1;
-            gf.nitems = nl;
+    // replay work areas: one per gather workgroup, or those of the kernels
+    auto areas = [&](unsigned g) { return in_gather ? g : rcap + rover; };
+    QueryLayout lo = layout(grid, areas(grid));
+    const size_t per_slot = (size_t)(slot_cand + slot_part) * 4 + (in_gather ? (size_t)wpq : 0);
+    if (fixed) {
+        // fit the fixed workspace: fewer replay work areas, then fewer
+        // gather workgroups (each step halves one of them, so the loop ends)
+        while (!in_gather && lo.total > fixed_bytes && (rover > 1 || rcap > 16)) {
+            if (rover > 1)
+                rover /= 2;
+            else
+                rcap = std::max(16u, rcap / 2);
+            if (nq > (int)rcap) rover = std::max(rover, 1u);
+            lo = layout(grid, areas(grid));
         }
-        gather(gf, (int)nl, I.side);
-        FPP_HIP(hipEventRecord(I.ev_join, I.side));
+        if (lo.total > fixed_bytes) {
+            const size_t base = layout(0, areas(0)).total;
+            grid = base + per_slot <= fixed_bytes ? (unsigned)std::min<size_t>(grid, (fixed_bytes - base) / per_slot) : 0;
+            FPP_REQUIRE(grid >= 1, FPP_EINVAL,
+                        "query batch too large for the fixed workspace of " + std::to_string(fixed_bytes >> 20) +
+                            " MiB (fewer queries or probes per call)");
+            lo = layout(grid, areas(grid));
+        }
+    } else if (!keep && (size_t)grid * per_slot > I.ws_budget) {
+        grid = (unsigned)std::max<size_t>(1, I.ws_budget / per_slot);
+        lo = layout(grid, areas(grid));
+    }
+    unsigned char* W;
+    if (fixed) {
+        W = fixed_ws;
     } else {
-        I.mark(3, st);
-        I.mark(4, st);
+        I.ws.reserve(lo.total);
+        W = I.ws.get();
     }
-    I.mark(5, st);
-    ga.order = order;
-    if (I.persistent) {
-        ga.queue = I.queue.get();
-        ga.nitems = (uint32_t)nu;
-        // every CU: the second stream's replay gets CUs only as persistent
-        // workgroups retire whether or not some are left out of the grid (16
-        // left out made the batch 3-5% slower and the replay no sooner)
-        ga.spare_cus = 0;
+    float* Qpad = reinterpret_cast<float*>(W);
+    int32_t* qc = reinterpret_cast<int32_t*>(W + lo.qc);
+    float* qv = reinterpret_cast<float*>(W + lo.qv);
+    uint32_t* qflag = reinterpret_cast<uint32_t*>(W + lo.qflag);
+    uint32_t* probes = reinterpret_cast<uint32_t*>(W + lo.probes);
+    uint64_t* ctot = reinterpret_cast<uint64_t*>(W + lo.ctot);
+    uint64_t* ncand = reinterpret_cast<uint64_t*>(W + lo.ncand);
+    QueryParams* qparams = reinterpret_cast<QueryParams*>(W + lo.qparams);
+    uint32_t* counters = reinterpret_cast<uint32_t*>(W + lo.counters);
+    uint32_t* qlist = reinterpret_cast<uint32_t*>(W + lo.qlist);
+    if (batched) {
+        pg.qp = qparams;
+        rp.pg.qp = qparams;
+        ga.qp = qparams;
+        FPP_HIP(hipMemcpyAsync(qparams, I.hqp.data(), (size_t)nq * sizeof(QueryParams), hipMemcpyHostToDevice, st));
     }
-    gather(ga, nu, st);
-    I.mark(6, st);
-    if (nl > 0) FPP_HIP(hipStreamWaitEvent(st, I.ev_join, 0));
-    std::vector<uint64_t> nc(nq);
-    FPP_HIP(hipMemcpyAsync(nc.data(), I.ncand.get(), (size_t)nq * 8, hipMemcpyDeviceToHost, st));
-    if (nl > 0) FPP_HIP(hipMemcpyAsync(ct.data(), I.ctot.get(), (size_t)nq * 8, hipMemcpyDeviceToHost, st));
-    FPP_HIP(hipStreamSynchronize(st));
-    for (int q = 0; q < nq; ++q) I.stat_ctot += (int64_t)ct[q];
-    if (I.profiling) {
+    ga.phase_ts = stamps ? reinterpret_cast<uint64_t*>(W + lo.phase_ts) : nullptr;
+    ga.heads = counters;
+    ga.ncand = ncand;
+    ga.nitems = (uint32_t)nq;
+    ga.cand = reinterpret_cast<int32_t*>(W + lo.cand);
+    ga.part = reinterpret_cast<int32_t*>(W + lo.part);
+    ga.slot_cand = slot_cand;
+    ga.slot_part = slot_part;
+    rp.Q = Qpad;
+    rp.ldq = I.ldx;
+    rp.masks = I.masks.get();
+    rp.qflag = in_gather ? qflag : nullptr;
+    rp.qc = qc;
+    rp.qv = qv;
+    rp.bucket_off = I.bucket_off.get();
+    rp.probes = probes;
+    rp.ctot = ctot;
+    rp.work = W + lo.rwork;
+    rp.wpq = wpq;
+    const int order_mode = exact_only ? ORDER_LONGEST : I.order_mode;
+    const bool marks = I.profiling && !fixed;
+    // 1. hashing, probe selection, exact replay of the flagged queries
+    auto front = [&](bool ev) {
+        launch_pad_rows(dQ, ldq, nq, I.n_features, Qpad, I.ldx, st);
+        // bit 0: per-layer lists rebuilt by the replay; bit 1: probes replayed
+        launch_fill_u32(qflag, exact_only ? 3u : (probe_path == PROBE_EXACT ? 2u : 0u), nq, st);
+        launch_fill_u32(counters, 0u, N_COUNTERS, st);
+        if (ev) I.mark(0, st);
+        if (!exact_only)
+            launch_hash_query(hg, Qpad, I.ldx, nq, PL, I.masks.get(), qc, qv, qflag, st);
+        if (ev) I.mark(1, st);
+        if (probe_path != PROBE_EXACT)
+            launch_probe_select(pg, nq, qc, qv, I.bucket_off.get(), probes, ctot, qflag, st);
+        if (ev) I.mark(2, st);
+        if (ev) I.mark(3, st);
+        if (!in_gather) {
+            launch_compact_flagged(nq, qflag, qlist, counters + 9, st);
+            launch_probe_replay(hg, pg, Qpad, I.ldx, I.masks.get(), qlist, counters + 9, counters + 8, qflag, qc, qv,
+                                I.bucket_off.get(), probes, ctot, W + lo.rwork, wpq, rcap, rover, st);
+        }
+        if (ev) I.mark(4, st);
+    };
+    // 2-3. order, gather
+    auto back = [&](bool ev) {
+        const uint32_t* order = nullptr;
+        int nqueues = 1;
+        if (order_mode != ORDER_IDENTITY && nq > 1) {
+            uint32_t* okey = reinterpret_cast<uint32_t*>(W + lo.okey);
+            int bits = 32;
+            if (order_mode == ORDER_BUCKET) {
+                launch_order_keys(qc, qflag, nq, L, PL, 2 * I.n_proj, okey, st);
+                bits = ilog2_exact(I.NB()) + 1;
+                nqueues = nq >= 64 ? 8 : 1;
+            } else {
+                launch_ctot_keys(ctot, qflag, nq, okey, st);
+            }
+            uint32_t *sk = nullptr, *sv = nullptr;
+            radix_sort_pairs(okey, reinterpret_cast<uint32_t*>(W + lo.oval), reinterpret_cast<uint32_t*>(W + lo.okey2),
+                             reinterpret_cast<uint32_t*>(W + lo.oval2), nq, bits, true,
+                             reinterpret_cast<uint32_t*>(W + lo.ohist), reinterpret_cast<uint64_t*>(W + lo.ohist_off),
+                             reinterpret_cast<uint64_t*>(W + lo.oscan), &sk, &sv, st);
+            order = sv;
+        }
+        GatherArgs a = ga;
+        a.order = order;
+        a.nqueues = nqueues;
+        if (ev) I.mark(5, st);
+        if (lowp)
+            launch_gather_score_lp(a, lp, rp, glds, grid, probes, I.bucket_off.get(), I.ids.get(), I.Xc.get(), Qpad,
+                                   I.ldx, d_out, d_scores, st);
+        else
+            launch_gather_score_f32(a, rp, glds, grid, probes, I.bucket_off.get(), I.ids.get(), I.Xc.get(), Qpad,
+                                    I.ldx, d_out, d_scores, gthreads, st);
+        if (ev) I.mark(6, st);
+    };
+    if (keep) {
+        // every query's candidates stay in a segment of its own, sized by
+        // its C_tot (read back after the replay)
+        front(marks);
+        std::vector<uint64_t> ct(nq);
+        FPP_HIP(hipMemcpyAsync(ct.data(), ctot, (size_t)nq * 8, hipMemcpyDeviceToHost, st));
+        FPP_HIP(hipStreamSynchronize(st));
+        std::vector<uint64_t>& seg = I.last.seg;
+        seg.assign(nq + 1, 0);
+        for (int q = 0; q < nq; ++q)
+            seg[q + 1] = seg[q] + std::max<uint64_t>(1, one_window ? std::min<uint64_t>(ct[q], I.n_points) : ct[q]);
+        I.keep_cand.reserve(seg[nq]);
+        I.keep_part.reserve(seg[nq]);
+        I.keep_off.reserve(nq + 1);
+        FPP_HIP(hipMemcpyAsync(I.keep_off.get(), seg.data(), (size_t)(nq + 1) * 8, hipMemcpyHostToDevice, st));
+        ga.cand = I.keep_cand.get();
+        ga.part = I.keep_part.get();
+        ga.seg_off = I.keep_off.get();
+        back(marks);
+    } else if (I.use_graph && !fixed) {
+        const std::vector<uint64_t> key = {
+            (uint64_t)nq, (uint64_t)ldq, (uint64_t)(uintptr_t)dQ, (uint64_t)(uintptr_t)st, I.build_gen,
+            (uint64_t)(uintptr_t)W, lo.total, (uint64_t)(uintptr_t)d_out, (uint64_t)(uintptr_t)d_scores,
+            (uint64_t)k, (uint64_t)pg.PL, (uint64_t)pg.perTable, (uint64_t)pg.qProbes, (uint64_t)batched,
+            (uint64_t)I.precision, (uint64_t)lp.refine, (uint64_t)lp.refine_per_k,
+            (uint64_t)(uintptr_t)(lowp ? (I.precision == 1 ? (const void*)lp.xb : (const void*)lp.x8) : nullptr),
+            (uint64_t)__builtin_bit_cast(uint32_t, lp.uscale), (uint64_t)gthreads, (uint64_t)order_mode, grid,
+            slot_cand, slot_part, (uint64_t)stamps, (uint64_t)probe_path, (uint64_t)ga.window_bits, rcap, rover,
+            (uint64_t)in_gather, glds};
+        if (!I.graph_exec || key != I.graph_key) {
+            I.drop_graph();
+            FPP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                front(false);
+                back(false);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            FPP_HIP(hipStreamEndCapture(st, &I.graph));
+            FPP_HIP(hipGraphInstantiate(&I.graph_exec, I.graph, nullptr, nullptr, 0));
+            I.graph_key = key;
+        }
+        // the whole call is timed as one stage
+        for (int i = 0; i < 6; ++i)
+            if (marks) I.mark(i, st);
+        FPP_HIP(hipGraphLaunch(I.graph_exec, st));
+        if (marks) I.mark(6, st);
+    } else {
+        front(marks);
+        back(marks);
+    }
+    I.last.probes = probes;
+    I.last.ctot = ctot;
+    I.last.ncand = ncand;
+    I.last.qflag = qflag;
+    I.last.st = st;
+    I.last.summed = false;
+    if (marks) {
         float a = 0, b = 0, c = 0, d = 0;
+        FPP_HIP(hipEventSynchronize(I.ev[6]));
         FPP_HIP(hipEventElapsedTime(&a, I.ev[0], I.ev[1]));
         FPP_HIP(hipEventElapsedTime(&b, I.ev[1], I.ev[2]));
         FPP_HIP(hipEventElapsedTime(&c, I.ev[3], I.ev[4]));
@@ -739,7 +966,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         I.stage_ms[2] = c;
         I.stage_ms[3] = d;
     }
-    if (I.phase_stamps) {
+    if (stamps) {
         if (!I.wall_khz) {
             int dev = 0;
             FPP_HIP(hipGetDevice(&dev));
@@ -747,14 +974,46 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         }
         std::vector<uint64_t>& ts = I.last_ts;
         ts.resize((size_t)nq * 5);
-        FPP_HIP(hipMemcpy(ts.data(), I.phase_ts.get(), ts.size() * 8, hipMemcpyDeviceToHost));
+        FPP_HIP(hipMemcpyAsync(ts.data(), ga.phase_ts, ts.size() * 8, hipMemcpyDeviceToHost, st));
+        FPP_HIP(hipStreamSynchronize(st));
         double acc[4] = {0, 0, 0, 0};
-        for (int q = 0; q < nq; ++q)
-            for (int i = 0; i < 4; ++i) acc[i] += (double)(ts[(size_t)q * 5 + i + 1] - ts[(size_t)q * 5 + i]);
-        for (int i = 0; i < 4; ++i) I.stage_ms[4 + i] = acc[i] / nq * 1000.0 / std::max(1, I.wall_khz);
+        uint64_t longest = 0, longest_flagged = 0;
+        std::vector<uint32_t> fl;
+        fl.resize(nq);
+        FPP_HIP(hipMemcpy(fl.data(), qflag, (size_t)nq * 4, hipMemcpyDeviceToHost));
+        for (int q = 0; q < nq; ++q) {
+            const uint64_t* t = &ts[(size_t)q * 5];
+            for (int i = 0; i < 4; ++i) acc[i] += (double)(t[i + 1] - t[i]);
+            longest = std::max(longest, t[4] - t[0]);
+            if (fl[q]) longest_flagged = std::max(longest_flagged, t[1] - t[0]);
+        }
+        const double us = 1000.0 / std::max(1, I.wall_khz);
+        for (int i = 0; i < 4; ++i) I.stage_ms[4 + i] = acc[i] / nq * us;
+        I.stage_ms[8] = (double)longest * us;
+        I.stage_ms[9] = (double)longest_flagged * us;
     }
-    for (int q = 0; q < nq; ++q) I.stat_cuniq += (int64_t)nc[q];
-    I.last_cand_off = off;
+    if (sync_end) FPP_HIP(hipStreamSynchronize(st));
+}
+
+// Statistics of the last query call, summed from its device arrays on the
+// first request (this waits for the call).
+void sum_last_call(Index& I) {
+    Index::LastCall& c = I.last;
+    if (c.summed) return;
+    c.summed = true;
+    I.stat_ctot = I.stat_cuniq = I.stat_qties = 0;
+    if (c.nq == 0 || !c.ctot) return;
+    std::vector<uint64_t> ct(c.nq), nc(c.nq);
+    std::vector<uint32_t> fl(c.nq);
+    FPP_HIP(hipStreamSynchronize(c.st));
+    FPP_HIP(hipMemcpy(ct.data(), c.ctot, (size_t)c.nq * 8, hipMemcpyDeviceToHost));
+    FPP_HIP(hipMemcpy(nc.data(), c.ncand, (size_t)c.nq * 8, hipMemcpyDeviceToHost));
+    FPP_HIP(hipMemcpy(fl.data(), c.qflag, (size_t)c.nq * 4, hipMemcpyDeviceToHost));
+    for (int q = 0; q < c.nq; ++q) {
+        I.stat_ctot += (int64_t)ct[q];
+        I.stat_cuniq += (int64_t)nc[q];
+        I.stat_qties += fl[q] ? 1 : 0;
+    }
 }
 
 // ---- index files ------------------------------------------------------------
@@ -893,6 +1152,9 @@ void load_index(Index& I, const char* path) {
     I.s8.reset();
     I.pre_cnt.reset();
     I.have_pre_cnt = false;
+    I.entry_codes.reset();
+    I.entry_vals.reset();
+    I.entries_kept = false;
     I.n_tables = T.n_tables;
     I.n_proj = T.n_proj;
     I.fhtDim = T.fhtDim;
@@ -935,7 +1197,8 @@ void load_index(Index& I, const char* path) {
     I.Xc.reserve((size_t)h.n_points * h.ldx);
     read_device(in, I.Xc.get(), (size_t)h.n_points * h.ldx);
     I.n_ids = h.n_ids;
-    I.max_bucket = (uint32_t)largest;
+    index_sizes(I, nullptr);
+    FPP_REQUIRE(I.max_bucket == (uint32_t)largest, FPP_EINTERNAL, "largest bucket differs from the offsets read");
     I.stat_hash_ties = h.hash_ties;
     I.stat_bucket_ties = h.bucket_ties;
     I.build_gen++;
@@ -944,9 +1207,9 @@ void load_index(Index& I, const char* path) {
 }
 
 // Times the gather launch shapes with run(st) (one query call on stream st)
-// and keeps the fastest: workgroup size, query order (per-query launches
-// only; the persistent launch takes its own longest-first order) and
-// persistent launch.  chosen receives {threads, order, persistent, us}.
+// and keeps the fastest: workgroup size and the order the persistent gather
+// takes the queries in (by best table-0 bucket in per-XCD runs, or longest
+// first).  chosen receives {threads, order, longest first, microseconds}.
 template <class Run>
 void autotune(Index& I, hipStream_t st, Run run, int32_t* chosen) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -955,36 +1218,34 @@ void autotune(Index& I, hipStream_t st, Run run, int32_t* chosen) {
     const bool profiling = I.profiling, stamps = I.phase_stamps;
     I.profiling = I.phase_stamps = false;
     // workgroup size applies to the FP32 gather; the reduced-precision
-    // gather has one shape
-    const std::vector<int> threads = I.precision == 0 ? std::vector<int>{256, 512, 1024} : std::vector<int>{512};
+    // gather runs 256 or 512
+    const std::vector<int> threads =
+        I.precision == 0 ? std::vector<int>{256, 512, 1024} : std::vector<int>{256, 512};
     float best = 0.0f;
-    int bt = I.gather_threads, bo = I.order_queries ? 1 : 0, bp = I.persistent ? 1 : 0;
+    int bt = I.gather_threads, bo = I.order_mode;
     try {
         for (int t : threads)
-            for (int pers = 0; pers < 2; ++pers)
-                for (int o = 0; o < (pers ? 1 : 2); ++o) {
-                    I.gather_threads = t;
-                    I.order_queries = o != 0;
-                    I.persistent = pers != 0;
-                    run(st);
-                    FPP_HIP(hipEventRecord(e0, st));
-                    for (int r = 0; r < 2; ++r) run(st);
-                    FPP_HIP(hipEventRecord(e1, st));
-                    FPP_HIP(hipEventSynchronize(e1));
-                    float ms = 0.0f;
-                    FPP_HIP(hipEventElapsedTime(&ms, e0, e1));
-                    if (I.tune_verbose) {
-                        fprintf(stderr, "[autotune] threads=%d order=%d persistent=%d: %.3f ms per call\n", t, o,
-                                pers, ms / 2.0f);
-                        fflush(stderr);
-                    }
-                    if (best == 0.0f || ms < best) {
-                        best = ms;
-                        bt = t;
-                        bo = o;
-                        bp = pers;
-                    }
+            for (int o : {ORDER_BUCKET, ORDER_LONGEST}) {
+                I.gather_threads = t;
+                I.order_mode = o;
+                run(st);
+                FPP_HIP(hipEventRecord(e0, st));
+                for (int r = 0; r < 2; ++r) run(st);
+                FPP_HIP(hipEventRecord(e1, st));
+                FPP_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                FPP_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (I.tune_verbose) {
+                    fprintf(stderr, "[autotune] threads=%d order=%s: %.3f ms per call\n", t,
+                            o == ORDER_BUCKET ? "bucket" : "longest-first", ms / 2.0f);
+                    fflush(stderr);
                 }
+                if (best == 0.0f || ms < best) {
+                    best = ms;
+                    bt = t;
+                    bo = o;
+                }
+            }
     } catch (...) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -997,12 +1258,11 @@ void autotune(Index& I, hipStream_t st, Run run, int32_t* chosen) {
     I.profiling = profiling;
     I.phase_stamps = stamps;
     I.gather_threads = bt;
-    I.order_queries = bo != 0;
-    I.persistent = bp != 0;
+    I.order_mode = bo;
     if (chosen) {
         chosen[0] = bt;
         chosen[1] = bo;
-        chosen[2] = bp;
+        chosen[2] = bo == ORDER_LONGEST ? 1 : 0;
         chosen[3] = (int32_t)(best / 2.0f * 1000.0f);  // microseconds per batch
     }
 }
@@ -1088,6 +1348,9 @@ int fpp_clear(fpp_index* h) {
         I.s8.reset();
         I.pre_cnt.reset();
         I.have_pre_cnt = false;
+        I.entry_codes.reset();
+        I.entry_vals.reset();
+        I.entries_kept = false;
         I.lp_ready = 0;
         I.n_ids = 0;
         I.built = false;
@@ -1101,6 +1364,33 @@ int fpp_build_device(fpp_index* h, const float* dX, int64_t ld, void* stream) {
     });
 }
 
+int fpp_set_insertable(fpp_index* h, int on) {
+    return guard([&] {
+        FPP_REQUIRE(h, FPP_EINVAL, "null index");
+        h->I.insertable = on != 0;
+    });
+}
+
+int fpp_insert_device(fpp_index* h, const float* dX, int64_t ld, int m, void* stream) {
+    return guard([&] {
+        FPP_REQUIRE(h && dX, FPP_EINVAL, "null argument");
+        do_insert(h->I, dX, ld, m, pick_stream(h->I, stream));
+    });
+}
+
+int fpp_insert(fpp_index* h, const float* X, int64_t ld, int m) {
+    return guard([&] {
+        FPP_REQUIRE(h && X, FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        FPP_REQUIRE(m >= 1 && ld >= I.n_features, FPP_EINVAL, "bad shape of the inserted points");
+        hipStream_t st = pick_stream(I, nullptr);
+        DevBuf<float> dX;
+        dX.reserve((size_t)m * I.n_features);
+        copy_rows_h2d(dX.get(), X, ld, m, I.n_features, st);
+        do_insert(I, dX.get(), I.n_features, m, st);
+    });
+}
+
 int fpp_build(fpp_index* h, const float* X, int64_t ld) {
     return guard([&] {
         FPP_REQUIRE(h && X, FPP_EINVAL, "null argument");
@@ -1109,8 +1399,7 @@ int fpp_build(fpp_index* h, const float* X, int64_t ld) {
         hipStream_t st = pick_stream(I, nullptr);
         DevBuf<float> dX;
         dX.reserve((size_t)I.n_points * I.n_features);
-        FPP_HIP(hipMemcpy2DAsync(dX.get(), (size_t)I.n_features * 4, X, (size_t)ld * 4, (size_t)I.n_features * 4,
-                                 I.n_points, hipMemcpyHostToDevice, st));
+        copy_rows_h2d(dX.get(), X, ld, I.n_points, I.n_features, st);
         do_build(I, dX.get(), I.n_features, st);
     });
 }
@@ -1119,7 +1408,10 @@ int fpp_query_device(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, 
                      void* stream) {
     return guard([&] {
         FPP_REQUIRE(h && (nq == 0 || (dQ && d_out)), FPP_EINVAL, "null argument");
-        do_query(h->I, dQ, ldq, nq, k, d_out, d_scores, pick_stream(h->I, stream));
+        // on the library's own stream the call returns with the results
+        // written; on the caller's stream it is asynchronous
+        do_query(h->I, dQ, ldq, nq, k, d_out, d_scores, pick_stream(h->I, stream), nullptr, nullptr, nullptr, 0,
+                 stream == nullptr);
     });
 }
 
@@ -1127,7 +1419,8 @@ int fpp_query_batch_device(fpp_index* h, const float* dQ, int64_t ldq, int nq, c
                            const int32_t* ks, int kmax, int32_t* d_out, float* d_scores, void* stream) {
     return guard([&] {
         FPP_REQUIRE(h && (nq == 0 || (dQ && d_out && qprobes && ks)), FPP_EINVAL, "null argument");
-        do_query(h->I, dQ, ldq, nq, kmax, d_out, d_scores, pick_stream(h->I, stream), qprobes, ks);
+        do_query(h->I, dQ, ldq, nq, kmax, d_out, d_scores, pick_stream(h->I, stream), qprobes, ks, nullptr, 0,
+                 stream == nullptr);
     });
 }
 
@@ -1184,8 +1477,7 @@ int query_host(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t
         DevBuf<float> dQ;
         dQ.reserve((size_t)std::max(nq, 1) * I.n_features);
         if (nq > 0)
-            FPP_HIP(hipMemcpy2DAsync(dQ.get(), (size_t)I.n_features * 4, Q, (size_t)ldq * 4,
-                                     (size_t)I.n_features * 4, nq, hipMemcpyHostToDevice, st));
+            copy_rows_h2d(dQ.get(), Q, ldq, nq, I.n_features, st);
         I.out_ids.reserve((size_t)std::max(nq, 1) * k);
         do_query(I, dQ.get(), I.n_features, nq, k, I.out_ids.get(), nullptr, st);
         if (nq > 0)
@@ -1198,7 +1490,8 @@ int query_host(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32_t
             // lookup with the distances: its time is split by the mean
             // per-query phase times (walk + dedup vs scoring + top-k).
             I.scratch_word.reserve(2);
-            const uint64_t empty = count_empty_probes(I.probes.get(), (int64_t)I.last_nprobes, I.bucket_off.get(),
+            sum_last_call(I);
+            const uint64_t empty = count_empty_probes(I.last.probes, (int64_t)I.last.nprobes, I.bucket_off.get(),
                                                       I.scratch_word.get(), st);
             const double ph_look = I.stage_ms[4] + I.stage_ms[5], ph_dist = I.stage_ms[6] + I.stage_ms[7];
             const double share = ph_look + ph_dist > 0 ? ph_look / (ph_look + ph_dist) : 0.0;
@@ -1297,6 +1590,7 @@ int fpp_get_shape(fpp_index* h, int64_t* out) {
         out[2] = I.n_tables;
         out[3] = I.n_ids;
         out[4] = I.ldx;
+        out[5] = I.n_points;
     });
 }
 
@@ -1385,12 +1679,13 @@ int fpp_get_stats(fpp_index* h, int64_t* out) {
     return guard([&] {
         FPP_REQUIRE(h && out, FPP_EINVAL, "null argument");
         Index& I = h->I;
+        sum_last_call(I);
         out[0] = I.stat_hash_ties;
         out[1] = I.stat_bucket_ties;
         out[2] = I.stat_ctot;
         out[3] = I.stat_cuniq;
         out[4] = I.stat_qties;
-        out[5] = I.last_nq;
+        out[5] = I.last.nq;
         out[6] = I.stat_probe_path;
     });
 }
@@ -1406,7 +1701,7 @@ int fpp_set_profiling(fpp_index* h, int on) {
 int fpp_get_timings(fpp_index* h, double* ms) {
     return guard([&] {
         FPP_REQUIRE(h && ms, FPP_EINVAL, "null argument");
-        for (int i = 0; i < 8; ++i) ms[i] = h->I.stage_ms[i];
+        for (int i = 0; i < 10; ++i) ms[i] = h->I.stage_ms[i];
     });
 }
 
@@ -1433,7 +1728,8 @@ int fpp_set_tuning(fpp_index* h, int which, int value) {
         FPP_REQUIRE(h, FPP_EINVAL, "null argument");
         switch (which) {
             case FPP_TUNE_QUERY_ORDER:
-                h->I.order_queries = value != 0;
+                FPP_REQUIRE(value >= 0 && value <= 2, FPP_EINVAL, "query order must be 0, 1 or 2");
+                h->I.order_mode = value;
                 break;
             case FPP_TUNE_GATHER_THREADS:
                 FPP_REQUIRE(value == 256 || value == 512 || value == 1024, FPP_EINVAL,
@@ -1441,7 +1737,15 @@ int fpp_set_tuning(fpp_index* h, int which, int value) {
                 h->I.gather_threads = value;
                 break;
             case FPP_TUNE_PERSISTENT:
-                h->I.persistent = value != 0;
+                // the gather is always persistent; 1 asks for longest first
+                h->I.order_mode = value ? ORDER_LONGEST : ORDER_BUCKET;
+                break;
+            case FPP_TUNE_KEEP_CANDIDATES:
+                h->I.keep_candidates = value != 0;
+                break;
+            case FPP_TUNE_REPLAY:
+                FPP_REQUIRE(value >= 0 && value <= 2, FPP_EINVAL, "replay placement must be 0, 1 or 2");
+                h->I.replay_mode = value;
                 break;
             case FPP_TUNE_FP8_PER_ROW:
                 h->I.fp8_per_row = value != 0;
@@ -1464,7 +1768,7 @@ int fpp_get_query_times(fpp_index* h, double* service_us, double* done_us) {
         FPP_REQUIRE(h && service_us && done_us, FPP_EINVAL, "null argument");
         Index& I = h->I;
         const size_t nq = I.last_ts.size() / 5;
-        FPP_REQUIRE(nq > 0 && nq == (size_t)I.last_nq, FPP_ESTATE,
+        FPP_REQUIRE(nq > 0 && nq == (size_t)I.last.nq, FPP_ESTATE,
                     "no per-query stamps: make the last query call with profiling level 2");
         uint64_t t0 = ~0ull;
         for (size_t q = 0; q < nq; ++q) t0 = std::min(t0, I.last_ts[q * 5]);
@@ -1480,8 +1784,10 @@ int fpp_get_last_probes(fpp_index* h, uint32_t* out) {
     return guard([&] {
         FPP_REQUIRE(h && out, FPP_EINVAL, "null argument");
         Index& I = h->I;
-        if (I.last_nq)
-            FPP_HIP(hipMemcpy(out, I.probes.get(), (size_t)I.last_nprobes * 4, hipMemcpyDeviceToHost));
+        if (I.last.nq && I.last.probes) {
+            FPP_HIP(hipStreamSynchronize(I.last.st));
+            FPP_HIP(hipMemcpy(out, I.last.probes, (size_t)I.last.nprobes * 4, hipMemcpyDeviceToHost));
+        }
     });
 }
 
@@ -1490,16 +1796,19 @@ int64_t fpp_get_last_candidates(fpp_index* h, int j, int32_t* out, int64_t cap) 
     const int rc = guard([&] {
         FPP_REQUIRE(h, FPP_EINVAL, "null argument");
         Index& I = h->I;
-        FPP_REQUIRE(j >= 0 && j < I.last_nq, FPP_EINVAL, "query index out of range");
+        FPP_REQUIRE(j >= 0 && j < I.last.nq, FPP_EINVAL, "query index out of range");
+        FPP_REQUIRE(!I.last.seg.empty(), FPP_ESTATE,
+                    "candidates are kept only by calls made with FPP_TUNE_KEEP_CANDIDATES on");
+        FPP_HIP(hipStreamSynchronize(I.last.st));
         uint64_t nc = 0;
-        FPP_HIP(hipMemcpy(&nc, I.ncand.get() + j, 8, hipMemcpyDeviceToHost));
+        FPP_HIP(hipMemcpy(&nc, I.last.ncand + j, 8, hipMemcpyDeviceToHost));
         n = (int64_t)nc;
         const int64_t c = std::min<int64_t>(cap, n);
         if (c > 0 && out) {
-            // above one dedup window the device list is unordered inside each
+            // above one dedup window the list is unordered inside each
             // window; report it ascending
             std::vector<int32_t> all((size_t)n);
-            FPP_HIP(hipMemcpy(all.data(), I.cand.get() + I.last_cand_off[j], (size_t)n * 4, hipMemcpyDeviceToHost));
+            FPP_HIP(hipMemcpy(all.data(), I.keep_cand.get() + I.last.seg[j], (size_t)n * 4, hipMemcpyDeviceToHost));
             std::sort(all.begin(), all.end());
             std::copy(all.begin(), all.begin() + c, out);
         }

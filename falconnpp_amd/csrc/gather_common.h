@@ -206,6 +206,58 @@ __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ 
     while (!w.done()) w.step(f);
 }
 
+// Next query of a persistent gather workgroup, or -1 when every run of the
+// work queue is drained.  order[0 .. nitems) (identity when null) is split
+// into nqueues runs of consecutive positions; workgroup b takes from run
+// b % nqueues first (with 8 runs, the XCD the workgroup sits on), then from
+// the others.  Every successful claim consumes an item, so each workgroup
+// leaves the loop once the items are gone.  Thread 0 claims; s_q is a
+// __shared__ word.
+__device__ __forceinline__ int claim_query(const GatherArgs& a, int* s_q) {
+    if (threadIdx.x == 0) {
+        int q = -1;
+        const uint32_t R = (uint32_t)a.nqueues;
+        const uint32_t home = blockIdx.x % R;
+        for (uint32_t j = 0; j < R && q < 0; ++j) {
+            const uint32_t r = (home + j) % R;
+            const uint32_t lo = (uint32_t)((uint64_t)a.nitems * r / R);
+            const uint32_t hi = (uint32_t)((uint64_t)a.nitems * (r + 1) / R);
+            if (lo + __hip_atomic_load(&a.heads[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= hi) continue;
+            const uint32_t i = atomicAdd(&a.heads[r], 1u);
+            if (lo + i < hi) q = a.order ? (int)a.order[lo + i] : (int)(lo + i);
+        }
+        *s_q = q;
+    }
+    __syncthreads();
+    const int q = __builtin_amdgcn_readfirstlane(*s_q);
+    __syncthreads();
+    return q;
+}
+
+// Scratch of the query a workgroup serves: its candidate list and its
+// window / score area, either the workgroup's own slot or, when the caller
+// keeps every query's candidates, the query's segment.
+struct QueryScratch {
+    int32_t* cand;
+    int32_t* part;
+    uint64_t ccap, pcap;
+};
+__device__ __forceinline__ QueryScratch query_scratch(const GatherArgs& a, int q) {
+    QueryScratch s;
+    if (a.seg_off) {
+        const uint64_t b = a.seg_off[q], e = a.seg_off[q + 1];
+        s.cand = a.cand + b;
+        s.part = a.part + b;
+        s.ccap = s.pcap = e - b;
+    } else {
+        s.cand = a.cand + (uint64_t)blockIdx.x * a.slot_cand;
+        s.part = a.part + (uint64_t)blockIdx.x * a.slot_part;
+        s.ccap = a.slot_cand;
+        s.pcap = a.slot_part;
+    }
+    return s;
+}
+
 // Rank of this lane among the set lanes of m.
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -272,14 +324,15 @@ __device__ __forceinline__ bool bitset_insert(uint32_t* bits, int64_t rel) {
     return (atomicOr(&bits[rel >> 5], m) & m) == 0u;
 }
 
-// Collect the unique candidates of one query into cq (capacity ccap, the
-// query's total bucket size); returns their count.  Bucket ids go through an
+// Collect the unique candidates of one query into cq (capacity ccap, at
+// least min(C_tot, N)); returns their count.  Bucket ids go through an
 // LDS bitset window of W ids (the reference's boost::dynamic_bitset,
 // src/FalconnPP.cpp:552-611).
 //  * N fits one window: ids are ORed in and the set bits compacted in id
 //    order, so cq is ascending.
 //  * Otherwise (W is then a power of two): two passes over the bucket ids
-//    count and then scatter them by window into part (same capacity); each
+//    count and then scatter them by window into part (capacity pcap, at
+//    least C_tot: duplicates included); each
 //    window's segment goes through a test-and-set and first occurrences are
 //    appended, so cq is ordered by window and unordered inside one.
 // Callers rank by (score, id), which does not depend on the order of cq.
@@ -288,7 +341,7 @@ template <int TB>
 __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32_t* __restrict__ pr,
                                        const uint64_t* __restrict__ bucket_off, const int32_t* __restrict__ ids,
                                        uint32_t* bits, uint32_t* wc, uint32_t* sh, int32_t* __restrict__ cq,
-                                       int32_t* __restrict__ part, uint64_t ccap, uint64_t* ts) {
+                                       int32_t* __restrict__ part, uint64_t ccap, uint64_t pcap, uint64_t* ts) {
     const int64_t W = a.window_bits;
     if (a.N <= W) {
         const int words = (int)(W / 32);
@@ -337,7 +390,7 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         if (ok) {
             const int w = v >> shift;
             const uint32_t pos = woff[w] + atomicAdd(&wfill[w], 1u);
-            if (pos < ccap) part[pos] = v;
+            if (pos < pcap) part[pos] = v;
         }
     });
     uint32_t* count = sh;  // appended ids
@@ -358,7 +411,7 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
 #pragma unroll
             for (int r = 0; r < UNR; ++r) {
                 const uint32_t i = i0 + r * TB + threadIdx.x;
-                v[r] = (i < cnt && base + i < ccap) ? part[base + i] : -1;
+                v[r] = (i < cnt && base + i < pcap) ? part[base + i] : -1;
             }
 #pragma unroll
             for (int r = 0; r < UNR; ++r)

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 namespace fpp {
 
@@ -114,15 +115,51 @@ int probe_select_path(const ProbeGeom& g);
 void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const float* qv,
                          const uint64_t* bucket_off, uint32_t* probes, uint64_t* ctot, uint32_t* qflag,
                          hipStream_t st);
-// Exact replay of the probing sequence for flagged queries: per (query,
-// table) lists in parallel, the global cut by radix select, and the global
-// heap merge only for queries whose cut has equal values on both sides.
-void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, int64_t ldq,
-                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, uint32_t* qflag,
-                        const int32_t* qc, const float* qv, const uint64_t* bucket_off, uint32_t* probes,
-                        uint64_t* ctot, void* work, int64_t work_per_query, hipStream_t st);
-int64_t probe_exact_work_bytes(const HashGeom& hg, const ProbeGeom& g);
+// The queries with a nonzero qflag, in query order, into list; *count =
+// their number (device side, so no host read is needed).
+void launch_compact_flagged(int nq, const uint32_t* qflag, uint32_t* list, uint32_t* count, hipStream_t st);
+// Exact replay of the probing sequence of the *nflag queries in qlist, as
+// separate kernels (used when every query's C_tot is needed before the
+// gather): per-table lists, the global cut by radix select, and the
+// reference's heap merge only where equal values straddle the cut.  Writes
+// their probes and ctot.  The first `cap` flagged queries get one wave per
+// (query, table) and a work area each; the rest go to `over` persistent
+// workgroups (claim zeroed beforehand) with an area each.  work holds
+// (cap + over) * wpq bytes, wpq = probe_replay_work_bytes(g).
+void launch_probe_replay(const HashGeom& hg, const ProbeGeom& g, const float* Q, int64_t ldq, const uint32_t* masks,
+                         const uint32_t* qlist, const uint32_t* nflag, uint32_t* claim, const uint32_t* qflag,
+                         const int32_t* qc, const float* qv, const uint64_t* bucket_off, uint32_t* probes,
+                         uint64_t* ctot, void* work, int64_t wpq, unsigned cap, unsigned over, hipStream_t st);
 
+// Exact replay of a flagged query's probing sequence done by the gather
+// workgroup that claims it, before its gather: the workgroup's waves rebuild
+// the per-table lists into work + blockIdx.x * wpq, then the cut or merge.
+// qflag null: no query is replayed there.
+struct ReplayArgs {
+    HashGeom hg;
+    ProbeGeom pg;
+    const float* Q = nullptr;  // padded queries, row stride ldq
+    int64_t ldq = 0;
+    const uint32_t* masks = nullptr;
+    const uint32_t* qflag = nullptr;
+    const int32_t* qc = nullptr;
+    const float* qv = nullptr;
+    const uint64_t* bucket_off = nullptr;
+    uint32_t* probes = nullptr;
+    uint64_t* ctot = nullptr;
+    unsigned char* work = nullptr;
+    int64_t wpq = 0;
+    int table_lds = 0;     // LDS bytes of one table's wave
+    int twaves = 0;        // waves that rebuild tables
+    int merge_staged = 0;  // lists staged in LDS for the merge
+};
+// LDS the replay needs inside a gather workgroup of `threads` (0 when a
+// workgroup of that size cannot hold one table's wave); fills r's LDS fields.
+size_t replay_lds_bytes(ReplayArgs& r, int threads);
+int64_t probe_replay_work_bytes(const ProbeGeom& g);
+
+// One persistent gather launch: every workgroup takes queries from a work
+// queue (claim_query) until it is drained.
 struct GatherArgs {
     int64_t N;
     int D;
@@ -130,28 +167,47 @@ struct GatherArgs {
     int qProbes;
     int k;
     int64_t window_bits;  // LDS bitset window
-    const uint32_t* order = nullptr;  // block -> query (null: identity)
     uint64_t* phase_ts = nullptr;     // per query 5 wall-clock stamps (null: off)
-    int32_t* part = nullptr;          // dedup-window scratch, per-query segments at cand_off
     const QueryParams* qp = nullptr;  // per-query probe counts and k (null: qProbes, k above)
     int ldo = 0;                      // row stride of the outputs (0: k)
-    // persistent launch: workgroups take order[item] for item = atomicAdd(queue)
-    // until nitems (queue zeroed beforehand); null: one workgroup per query
-    uint32_t* queue = nullptr;
+    // work queue: order[0 .. nitems) (identity when null) in nqueues runs;
+    // heads[nqueues] zeroed beforehand
+    const uint32_t* order = nullptr;
     uint32_t nitems = 0;
-    int spare_cus = 0;  // persistent launch: CUs left free for concurrent work
+    int nqueues = 1;
+    uint32_t* heads = nullptr;
+    // scratch: workgroup b uses cand[b*slot_cand ...] (the unique candidates,
+    // at least min(C_tot, N) per query) and part[b*slot_part ...] (the scatter
+    // by dedup window, at least C_tot with several windows; FP32 scores or
+    // approximate score keys).  With seg_off, query q uses both at seg_off[q]
+    // instead (its candidates stay there after the call).
+    int32_t* cand = nullptr;
+    int32_t* part = nullptr;
+    uint64_t slot_cand = 0, slot_part = 0;
+    const uint64_t* seg_off = nullptr;
+    uint64_t* ncand = nullptr;  // unique candidates per query
 };
-// Query order for cache reuse: sort key per query, then the XCD-aware map.
-void launch_order_keys(const int32_t* qc, int nq, int L, int PL, int two_np, uint32_t* key, hipStream_t st);
-void launch_xcd_order(const uint32_t* sorted, int nq, int nxcd, uint32_t* order, hipStream_t st);
-// The queries of order (identity when null) whose qflag is clear, in order,
-// into order_u.
-void launch_split_order(const uint32_t* order, int nq, const uint32_t* qflag, uint32_t* order_u, hipStream_t st);
+// Query order sort keys: the best bucket of table 0 (queries with equal keys
+// share most candidates), or C_tot descending (longest first).
+// Queries whose qflag is set (replayed inside the gather) get key 0, ahead of
+// the others.
+void launch_order_keys(const int32_t* qc, const uint32_t* qflag, int nq, int L, int PL, int two_np, uint32_t* key,
+                       hipStream_t st);
+void launch_ctot_keys(const uint64_t* ctot, const uint32_t* qflag, int nq, uint32_t* key, hipStream_t st);
+// dst[r*ldd + c] = src[r*lds + c] for c < D, 0 up to ldd (queries padded to
+// the scored row stride).
+void launch_pad_rows(const float* src, int64_t lds, int64_t rows, int D, float* dst, int64_t ldd, hipStream_t st);
 // p[0 .. n) = v (a kernel, so it can be captured in a graph).
 void launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
-// Workgroups of a persistent launch over nitems: every CU but spare_cus holds
-// as many as fit in its LDS, no more than there are items.
-unsigned persistent_grid(int nitems, size_t lds_bytes, int spare_cus, int threads);
+// Workgroups of a persistent launch over nitems: every CU holds as many as
+// fit in its LDS (and two waves per SIMD), no more than there are items.
+unsigned persistent_grid(int nitems, size_t lds_bytes, int threads);
+// Sizes of the buckets of a built index (synchronises st): hist[s] = number
+// of buckets holding s < SIZE_BINS ids; big = the sizes of the others.
+// Returns false when there were more than 65536 of those (big incomplete).
+constexpr int SIZE_BINS = 4096;
+bool bucket_size_histogram(const uint64_t* off, int64_t nbk, std::vector<uint32_t>& hist, std::vector<uint32_t>& big,
+                           hipStream_t st);
 // Largest bucket of a built index (synchronises st); scratch: one word.
 uint32_t max_bucket_size(const uint64_t* off, int64_t nbk, uint32_t* scratch, hipStream_t st);
 // Dedup windows over the ids [0, N) and the LDS words of their counters
@@ -167,12 +223,14 @@ __host__ __device__ inline int64_t window_counter_words(int64_t N, int64_t W) {
 uint64_t count_empty_probes(const uint32_t* probes, int64_t n, const uint64_t* bucket_off, uint32_t* scratch2,
                             hipStream_t st);
 int64_t gather_window_bits(int64_t N, int64_t extra_words, int per_cu);
-// Dedup (LDS bitset windows), FP32 rerank and top-k of each query.
-// cand/score: per-query segments at cand_off[q]; ncand[q] = unique count.
-void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,
-                             const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
-                             const uint64_t* cand_off, int32_t* cand, float* score, uint64_t* ncand,
-                             int32_t* out_ids, float* out_scores, int threads, hipStream_t st);
+// Dedup (LDS bitset windows), FP32 rerank and top-k of each query, one
+// persistent launch of `grid` workgroups (persistent_grid with the LDS of
+// gather_f32_lds); a.ncand[q] = unique candidates.
+size_t gather_f32_lds(const GatherArgs& a, int threads);
+void launch_gather_score_f32(const GatherArgs& a, const ReplayArgs& rp, size_t lds, unsigned grid,
+                             const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids, const float* Xc,
+                             const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores, int threads,
+                             hipStream_t st);
 
 // ---- reduced-precision scoring + FP32 refine (lowprec.hip) ----------------
 struct LowPrecArgs {
@@ -191,10 +249,11 @@ void launch_quant_bf16(const float* Xc, int64_t ldx, int64_t N, int D, int64_t l
 // returns the scale shared by every row, or 0 when rows keep their own
 float launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint8_t* X8, float* scale,
                        uint32_t* range, bool per_row, hipStream_t st);
-void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, int nq, const uint32_t* probes,
-                            const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q,
-                            int64_t ldq, const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids,
-                            float* out_scores, hipStream_t st);
+size_t gather_lp_lds(const GatherArgs& a, const LowPrecArgs& lp);
+void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, const ReplayArgs& rp, size_t lds,
+                            unsigned grid, const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids,
+                            const float* Xc, const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores,
+                            hipStream_t st);
 
 // ---- FP64 brute force (bf.hip), verification only -------------------------
 void launch_bf64(const float* X, int64_t ldx, int64_t N, int D, const float* Q, int64_t ldq, int nq, int k,

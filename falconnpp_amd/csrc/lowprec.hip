@@ -17,6 +17,7 @@
 #include "common.h"
 #include "gather_common.h"
 #include "kernels.h"
+#include "replay_common.h"
 
 namespace fpp {
 namespace {
@@ -111,14 +112,14 @@ constexpr int lp_tiles(int fmt, int kt) {
 }
 
 template <int TB, int FMT, int KT>
-__device__ __forceinline__ void k_gather_score_lp_one(GatherArgs a, LowPrecArgs lp,
+__device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const LowPrecArgs& lp,
+                                                        const ReplayArgs& rp,
                                                         const uint32_t* __restrict__ probes,
                                                         const uint64_t* __restrict__ bucket_off,
                                                         const int32_t* __restrict__ ids, const float* __restrict__ Xc,
                                                         const float* __restrict__ Q, int64_t ldq,
-                                                        const uint64_t* __restrict__ cand_off,
-                                                        int32_t* __restrict__ cand, uint64_t* __restrict__ ncand,
-                                                        int32_t* __restrict__ out_ids, float* __restrict__ out_scores, int q) {
+                                                        int32_t* __restrict__ out_ids, float* __restrict__ out_scores,
+                                                        int q) {
     constexpr int NW = TB / WAVE;
     extern __shared__ uint32_t lds[];
     const int64_t W = a.window_bits;
@@ -136,15 +137,22 @@ __device__ __forceinline__ void k_gather_score_lp_one(GatherArgs a, LowPrecArgs 
     __shared__ uint32_t s_below;
     uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * 5 : nullptr;
     if (ts && threadIdx.x == 0) ts[0] = wall_clock64();
+    // a tie-flagged query's probes are replayed exactly first (the LDS is
+    // free until the dedup below)
+    if (rp.qflag && rp.qflag[q]) {
+        replay_query<TB>(rp, (uint32_t)q, reinterpret_cast<unsigned char*>(lds),
+                         reinterpret_cast<IFPair*>(rp.work + (int64_t)blockIdx.x * rp.wpq));
+        __syncthreads();
+    }
     const QueryView qv_ = query_view(a.qp, q, a.qProbes, 0, 0, a.k);
     const int kq = qv_.k;
     const int Rq = lp.refine_per_k ? min(R, 4 * kq) : R;
     const uint32_t* pr = probes + qv_.probe_off;
-    const uint64_t cbase = cand_off[q];
-    const uint64_t ccap = cand_off[q + 1] - cbase;
-    int32_t* cq = cand + cbase;
-    const uint32_t nc = collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, a.part + cbase, ccap, ts);
-    if (threadIdx.x == 0) ncand[q] = nc;
+    const QueryScratch qs = query_scratch(a, q);
+    int32_t* cq = qs.cand;
+    const uint32_t nc =
+        collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, qs.part, qs.ccap, qs.pcap, ts);
+    if (threadIdx.x == 0) a.ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
 
     const int lane = threadIdx.x & (WAVE - 1);
@@ -218,7 +226,7 @@ __device__ __forceinline__ void k_gather_score_lp_one(GatherArgs a, LowPrecArgs 
     // position: in LDS over the dedup bitset when they fit, else in the
     // query's scratch segment
     const uint32_t lds_cap = (uint32_t)Wwords;
-    uint32_t* keys = nc <= lds_cap ? bits : reinterpret_cast<uint32_t*>(a.part + cbase);
+    uint32_t* keys = nc <= lds_cap ? bits : reinterpret_cast<uint32_t*>(qs.part);
     constexpr int NT = lp_tiles(FMT, KT);
     constexpr uint32_t SPAN = 8 * NT;
     const uint32_t stride = (uint32_t)NW * SPAN;
@@ -287,7 +295,7 @@ __device__ __forceinline__ void k_gather_score_lp_one(GatherArgs a, LowPrecArgs 
     // the R best approximate scores (ties: smaller id)
     const uint32_t rr = nc <= lds_cap
                             ? select_smallest<TB>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid)
-                            : select_smallest<TB>(reinterpret_cast<const uint32_t*>(a.part + cbase), cq, nc,
+                            : select_smallest<TB>(reinterpret_cast<const uint32_t*>(qs.part), cq, nc,
                                                   (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid);
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
     // exact FP32 rescoring of the survivors, 8 lanes per row
@@ -334,58 +342,32 @@ __device__ __forceinline__ void k_gather_score_lp_one(GatherArgs a, LowPrecArgs 
     if (ts && threadIdx.x == 0) ts[4] = wall_clock64();
 }
 
-template <int TB, int FMT, int KT, bool PERSIST>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gather_score_lp(GatherArgs a, LowPrecArgs lp,
+// Persistent: each workgroup takes queries from the work queue until it is
+// drained.
+template <int TB, int FMT, int KT>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gather_score_lp(GatherArgs a, LowPrecArgs lp, ReplayArgs rp,
                                                         const uint32_t* __restrict__ probes,
                                                         const uint64_t* __restrict__ bucket_off,
                                                         const int32_t* __restrict__ ids, const float* __restrict__ Xc,
                                                         const float* __restrict__ Q, int64_t ldq,
-                                                        const uint64_t* __restrict__ cand_off,
-                                                        int32_t* __restrict__ cand, uint64_t* __restrict__ ncand,
                                                         int32_t* __restrict__ out_ids, float* __restrict__ out_scores) {
-    if constexpr (!PERSIST) {
-        const int q = a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
-        k_gather_score_lp_one<TB, FMT, KT>(a, lp, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand,
-                                           out_ids, out_scores, q);
-    } else {
-        // persistent: take queries from the work queue until it is empty
-        __shared__ uint32_t s_item;
-        for (;;) {
-            if (threadIdx.x == 0) s_item = atomicAdd(a.queue, 1u);
-            __syncthreads();
-            const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
-            __syncthreads();
-            if (item >= a.nitems) break;
-            const int q = __builtin_amdgcn_readfirstlane((int)a.order[item]);
-            k_gather_score_lp_one<TB, FMT, KT>(a, lp, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand,
-                                               out_ids, out_scores, q);
-            __syncthreads();
-        }
+    __shared__ int s_q;
+    for (;;) {
+        const int q = claim_query(a, &s_q);
+        if (q < 0) break;
+        k_gather_score_lp_one<TB, FMT, KT>(a, lp, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, q);
+        __syncthreads();
     }
 }
 
-template <int TB, int FMT, int KT>
-void dispatch_lp_launch(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t lds, const uint32_t* probes,
-                        const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
-                        const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids,
-                        float* out_scores, hipStream_t st) {
-    if (a.queue)
-        k_gather_score_lp<TB, FMT, KT, true><<<persistent_grid(nq, lds, a.spare_cus, TB), TB, lds, st>>>(
-            a, lp, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids, out_scores);
-    else
-        k_gather_score_lp<TB, FMT, KT, false><<<nq, TB, lds, st>>>(a, lp, probes, bucket_off, ids, Xc, Q, ldq,
-                                                                      cand_off, cand, ncand, out_ids, out_scores);
-}
-
 template <int TB, int FMT>
-void dispatch_lp_kt(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t lds, const uint32_t* probes,
-                    const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
-                    const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids, float* out_scores,
-                    hipStream_t st) {
+void dispatch_lp_kt(const GatherArgs& a, const LowPrecArgs& lp, const ReplayArgs& rp, unsigned grid, size_t lds,
+                    const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids, const float* Xc,
+                    const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores, hipStream_t st) {
     const int64_t kt = lp.ldl / 32;
-#define FPP_LP(KT)                                                                                                  \
-    dispatch_lp_launch<TB, FMT, KT>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, \
-                                    out_ids, out_scores, st)
+#define FPP_LP(KT)                                                                                            \
+    k_gather_score_lp<TB, FMT, KT><<<grid, TB, lds, st>>>(a, lp, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, \
+                                                          out_scores)
     if (kt <= 4)
         FPP_LP(4);
     else if (kt <= 8)
@@ -398,16 +380,13 @@ void dispatch_lp_kt(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t l
 }
 
 template <int TB>
-void dispatch_lp_fmt(const GatherArgs& a, const LowPrecArgs& lp, int nq, size_t lds, const uint32_t* probes,
-                     const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
-                     const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids, float* out_scores,
-                     hipStream_t st) {
+void dispatch_lp_fmt(const GatherArgs& a, const LowPrecArgs& lp, const ReplayArgs& rp, unsigned grid, size_t lds,
+                     const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids, const float* Xc,
+                     const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores, hipStream_t st) {
     if (lp.format == BF16)
-        dispatch_lp_kt<TB, BF16>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
-                                 out_scores, st);
+        dispatch_lp_kt<TB, BF16>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
     else
-        dispatch_lp_kt<TB, FP8>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
-                                out_scores, st);
+        dispatch_lp_kt<TB, FP8>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
 }
 
 }  // namespace
@@ -448,27 +427,29 @@ float launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t l
     return uscale;
 }
 
-void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, int nq, const uint32_t* probes,
-                            const uint64_t* bucket_off, const int32_t* ids, const float* Xc, const float* Q,
-                            int64_t ldq, const uint64_t* cand_off, int32_t* cand, uint64_t* ncand, int32_t* out_ids,
-                            float* out_scores, hipStream_t st) {
+size_t gather_lp_lds(const GatherArgs& a, const LowPrecArgs& lp) {
+    const int64_t wwords = (a.window_bits / 32 + 1) & ~1;
+    return (size_t)(wwords + window_counter_words(a.N, a.window_bits) + lowprec_extra_words(lp.refine, lp.threads)) *
+           4;
+}
+
+void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, const ReplayArgs& rp, size_t lds,
+                            unsigned grid, const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids,
+                            const float* Xc, const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores,
+                            hipStream_t st) {
     FPP_REQUIRE(lp.format == BF16 || lp.format == FP8, FPP_EINVAL, "unknown scoring format");
     FPP_REQUIRE(lp.refine >= 1 && lp.refine <= 4 * WAVE, FPP_EINVAL, "refine count must be in [1, 256]");
     FPP_REQUIRE(a.k >= 1 && a.k <= lp.refine, FPP_EINVAL, "n_neighbors must not exceed the refine count");
     FPP_REQUIRE(lp.ldl % 32 == 0 && lp.ldl / 32 <= 32, FPP_EINVAL, "n_features must be <= 1024");
-    if (nq == 0) return;
-    const int64_t wwords = (a.window_bits / 32 + 1) & ~1;
     FPP_REQUIRE(lp.threads == 256 || lp.threads == 512, FPP_EINVAL, "low-precision workgroups are 256 or 512 threads");
-    const size_t lds = (size_t)(wwords + window_counter_words(a.N, a.window_bits) +
-                                lowprec_extra_words(lp.refine, lp.threads)) * 4;
-    FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "LDS budget exceeded (refine count or qProbes too large)");
-    if (a.queue) launch_fill_u32(a.queue, 0u, 1, st);
+    FPP_REQUIRE(a.heads && a.ncand && a.cand && a.part && a.nqueues >= 1, FPP_EINTERNAL, "gather scratch missing");
+    FPP_REQUIRE(lds >= gather_lp_lds(a, lp) && lds <= 160 * 1024, FPP_EINVAL,
+                "LDS budget exceeded (refine count or qProbes too large)");
+    if (a.nitems == 0 || grid == 0) return;
     if (lp.threads == 256)
-        dispatch_lp_fmt<256>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
-                             out_scores, st);
+        dispatch_lp_fmt<256>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
     else
-        dispatch_lp_fmt<512>(a, lp, nq, lds, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, ncand, out_ids,
-                             out_scores, st);
+        dispatch_lp_fmt<512>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
     FPP_HIP(hipGetLastError());
 }
 

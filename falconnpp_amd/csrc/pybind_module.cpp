@@ -88,6 +88,30 @@ PYBIND11_MODULE(FalconnPP, m) {
                      rethrow(e);
                  }
              })
+        // streaming insert (no reference counterpart)
+        .def(
+            "set_insertable",
+            [](FalconnPP& f, bool on) {
+                try {
+                    f.setInsertable(on);
+                } catch (const falconnpp::Error& e) {
+                    rethrow(e);
+                }
+            },
+            py::arg("on") = true)
+        .def(
+            "insert",
+            [](FalconnPP& f, const py::handle& pts) {
+                FArray a = as_matrix(pts, "points");
+                if (a.shape(0) != f.n_features()) throw py::value_error("points must be n_features x m");
+                try {
+                    py::gil_scoped_release nogil;
+                    f.insert(a.data(), a.shape(0), (int)a.shape(1));
+                } catch (const falconnpp::Error& e) {
+                    rethrow(e);
+                }
+            },
+            py::arg("points"))
         // index files (no reference counterpart)
         .def(
             "save",

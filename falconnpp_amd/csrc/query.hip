@@ -13,6 +13,7 @@
 #include "heap.h"
 #include "gather_common.h"
 #include "kernels.h"
+#include "replay_common.h"
 
 namespace fpp {
 namespace {
@@ -371,271 +372,127 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     }
 }
 
-// ---- exact replay of a flagged query's probing sequence -------------------
-// One wave per (flagged query, table).  Rotation of the query for one layer
-// into x (F floats of LDS): three rounds of sign flips and the unnormalised
-// FWHT with the stages in ascending stride (src/FalconnPP.cpp:392-411 via
-// FFHT), butterflies of a stage spread over the lanes (bit-identical: the
-// butterflies of a stage are independent).
-__device__ void rotate_wave(const float* src, int D, int F, const uint32_t* masks_layer, float* x) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int MW = (F + 31) / 32;
-    for (int d = lane; d < F; d += WAVE) x[d] = d < D ? src[d] : 0.0f;
-    for (int r = 0; r < 3; ++r) {
-        const uint32_t* mb = masks_layer + r * MW;
-        __syncthreads();
-        for (int d = lane; d < F; d += WAVE)
-            x[d] = __uint_as_float(__float_as_uint(x[d]) ^ (((mb[d >> 5] >> (d & 31)) & 1u) << 31));
-        for (int h = 1; h < F; h <<= 1) {
-            __syncthreads();
-            for (int b = lane; b < F / 2; b += WAVE) {
-                const int j = (b / h) * 2 * h + (b % h);
-                const float u = x[j], v = x[j + h];
-                x[j] = u + v;
-                x[j + h] = u - v;
-            }
-        }
-    }
-    __syncthreads();
-}
-
-// The per-layer heap of src/FalconnPP.cpp:413-440 (min-heap of the p largest
-// |projections|, then popped smallest first into vec[p-1 .. 0]).
-__device__ void layer_heap(const float* x, int n_proj, int log2_np, int p, IFPair* hs, IFPair* vec) {
-    MinHeap q;
-    q.init(hs);
-    for (int r = 0; r < n_proj; ++r) {
-        const int s = x[r] >= 0.0f ? 1 : -1;
-        const float v = s * x[r];
-        int code = r;
-        if (s < 0) code |= 1 << log2_np;
-        if (q.size() < p)
-            q.push(IFPair{code, v});
-        else if (v > q.top().val) {
-            q.pop();
-            q.push(IFPair{code, v});
-        }
-    }
-    for (int i = p - 1; i >= 0; --i) {
-        vec[i] = q.top();
-        q.pop();
+// Exact replay of the flagged queries' probing sequences in three kernels,
+// all sized by the device's flag count, not the host's:
+//  k_replay_tables   one wave per (flagged query f < cap, table): the table
+//                    lists into work area f;
+//  k_replay_select   one workgroup per flagged query f < cap: cut, set or
+//                    merge, ctot;
+//  k_probe_replay    the flagged queries from cap on (beyond the work areas
+//                    of the first two), one persistent workgroup per query
+//                    doing all three steps in its own area.
+__global__ __launch_bounds__(WAVE) void k_replay_tables(HashGeom hg, ProbeGeom g, const float* __restrict__ Q,
+                                                        int64_t ldq, const uint32_t* __restrict__ masks,
+                                                        const uint32_t* __restrict__ qlist,
+                                                        const uint32_t* __restrict__ nflag, uint32_t cap,
+                                                        const uint32_t* __restrict__ qflag,
+                                                        const int32_t* __restrict__ qc,
+                                                        const float* __restrict__ qvals,
+                                                        unsigned char* __restrict__ work, int64_t wpq) {
+    extern __shared__ __align__(16) unsigned char tl[];
+    const uint64_t items = (uint64_t)min(*nflag, cap) * (uint64_t)g.L;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t f = (uint32_t)(it / (uint64_t)g.L);
+        const int l = (int)(it - (uint64_t)f * g.L);
+        const uint32_t q = qlist[f];
+        const QueryView qv_ = query_view(g.qp, (int)q, g.qProbes, g.perTable, g.PL, 0);
+        IFPair* pool = reinterpret_cast<IFPair*>(work + (int64_t)f * wpq);
+        replay_table(hg, g, qv_, Q, ldq, masks, q, l, qflag[q], qc, qvals, tl, pool + (int64_t)l * qv_.perTable);
     }
 }
 
-// LDS of one (query, table) wave: two rotated layers, two layer heaps and
-// their lists, the table's pair heap.
-__host__ __device__ inline int64_t exact_table_lds_bytes(int F, int PL, int PT) {
-    return (int64_t)2 * F * 4 + (int64_t)(4 * PL + PT) * 8;
-}
-// Work area of one flagged query in global memory: the per-table probe lists.
-__host__ __device__ inline int64_t exact_work_bytes(int L, int PT) { return ((int64_t)L * PT * 8 + 15) / 16 * 16; }
-
-// Per (flagged query, table), one wave: the table's probe list,
-// src/FalconnPP.cpp:358-531 replayed with the emulated heaps.  When no
-// per-layer tie was seen (qflag bit 0 clear), the per-layer top lists of the
-// fast kernel are exact (distinct values, largest first, as the reference
-// pops them) and are reused; otherwise the table is rotated again and its
-// layer heaps replayed.
-__global__ __launch_bounds__(WAVE) void k_probe_exact_tables(HashGeom hg, ProbeGeom g, const float* __restrict__ Q,
-                                                             int64_t ldq, const uint32_t* __restrict__ masks,
-                                                             const uint32_t* __restrict__ qlist,
-                                                             const uint32_t* __restrict__ qflag,
-                                                             const int32_t* __restrict__ qc,
-                                                             const float* __restrict__ qvals,
-                                                             unsigned char* __restrict__ work, int64_t wpq) {
-    extern __shared__ float xs[];
-    const uint32_t f = blockIdx.x / g.L;
-    const int l = (int)(blockIdx.x % g.L);
-    const uint32_t q = qlist[f];
-    const QueryView qv_ = query_view(g.qp, (int)q, g.qProbes, g.perTable, g.PL, 0);
-    const int F = hg.F, PL = qv_.PL, PT = qv_.perTable;
-    const int lane = threadIdx.x;
-    float* x1 = xs;
-    float* x2 = x1 + F;
-    IFPair* hs1 = reinterpret_cast<IFPair*>(x2 + F);
-    IFPair* hs2 = hs1 + PL;
-    IFPair* v1 = hs2 + PL;
-    IFPair* v2 = v1 + PL;
-    IFPair* tq = v2 + PL;
-    if (qflag[q] & 1u) {
-        const int MW = (F + 31) / 32;
-        const float* qv = Q + (int64_t)q * ldq;
-        rotate_wave(qv, hg.D, F, masks + (size_t)(l * 2 + 0) * 3 * MW, x1);
-        rotate_wave(qv, hg.D, F, masks + (size_t)(l * 2 + 1) * 3 * MW, x2);
-        if (lane == 0) layer_heap(x1, hg.n_proj, hg.log2_nproj, PL, hs1, v1);
-        if (lane == 1) layer_heap(x2, hg.n_proj, hg.log2_nproj, PL, hs2, v2);
-    } else {
-        const size_t base = ((size_t)q * g.L + l) * 2 * g.PL;
-        for (int i = lane; i < PL; i += WAVE) {
-            v1[i] = IFPair{qc[base + i], qvals[base + i]};
-            v2[i] = IFPair{qc[base + g.PL + i], qvals[base + g.PL + i]};
-        }
-    }
-    __syncthreads();
-    if (lane != 0) return;
-    IFPair* list = reinterpret_cast<IFPair*>(work + (int64_t)f * wpq);
-    MinHeap th;
-    th.init(tq);
-    for (int i = 0; i < PL; ++i)
-        for (int j = 0; j < PL; ++j) {
-            const int b = v1[i].idx * g.two_np + v2[j].idx;
-            const float s = v1[i].val + v2[j].val;
-            if (th.size() < PT)
-                th.push(IFPair{b, s});
-            else if (s > th.top().val) {
-                th.pop();
-                th.push(IFPair{b, s});
-            }
-        }
-    int idx = PT - 1;
-    while (!th.empty()) {
-        IFPair pr = th.top();
-        th.pop();
-        pr.idx = l * g.NB + pr.idx;
-        list[(int64_t)l * PT + idx] = pr;
-        idx--;
-    }
-}
-
-// Per flagged query, after the exact per-table lists: the reference's global
-// merge (src/FalconnPP.cpp:540-582) pops values in non-increasing order, so
-// its probe set is the top-qProbes of the pool by value, and libstdc++'s
-// heap order matters only when equal values straddle the cut.  A radix
-// select over the pool finds the cut; when it is unambiguous the set is
-// written here and the query marked resolved (qflag bit 2), else the merge is
-// replayed by k_probe_exact_merge.
-constexpr int MG_T = 256;
-constexpr uint32_t RESOLVED = 4u;
-__global__ __launch_bounds__(MG_T) void k_probe_exact_select(ProbeGeom g, const uint32_t* __restrict__ qlist,
-                                                             const uint64_t* __restrict__ bucket_off,
-                                                             uint32_t* __restrict__ probes,
-                                                             uint64_t* __restrict__ ctot, uint32_t* __restrict__ qflag,
-                                                             const unsigned char* __restrict__ work, int64_t wpq) {
-    __shared__ uint32_t hist[256];
-    __shared__ int s_digit;
-    __shared__ uint32_t s_below, s_eqc, s_cnt;
-    __shared__ unsigned long long s_ct;
-    const uint32_t f = blockIdx.x;
-    const uint32_t q = qlist[f];
-    const QueryView qv_ = query_view(g.qp, (int)q, g.qProbes, g.perTable, g.PL, 0);
-    const int n = g.L * qv_.perTable;
-    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-    const IFPair* pool = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
-    // descending order: smaller key = larger value
-    uint32_t prefix = 0, pmask = 0, k = (uint32_t)qv_.qProbes, eqc = 0;
-    for (int shift = 24; shift >= 0; shift -= 8) {
-        hist[threadIdx.x] = 0;
-        __syncthreads();
-        for (int i = threadIdx.x; i < n; i += MG_T) {
-            const uint32_t key = ~heap_key(pool[i].val);
-            if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        if (w == 0) {
-            int d;
-            uint32_t below, e;
-            wave_find_digit(hist, k, d, below, e);
-            if (lane == 0) {
-                s_digit = d;
-                s_below = below;
-                s_eqc = e;
-            }
-        }
-        __syncthreads();
-        prefix |= (uint32_t)s_digit << shift;
-        pmask |= 255u << shift;
-        k -= s_below;
-        eqc = s_eqc;
-    }
-    if (eqc != k) return;  // equal values straddle the cut: replay the merge
-    const uint32_t T = prefix;
-    if (threadIdx.x == 0) {
-        s_cnt = 0;
-        s_ct = 0;
-    }
-    __syncthreads();
-    uint32_t* out = probes + qv_.probe_off;
-    unsigned long long ct = 0;
-    for (int c0 = 0; c0 < n; c0 += MG_T) {
-        const int i = c0 + threadIdx.x;
-        IFPair e{0, 0.0f};
-        bool take = false;
-        if (i < n) {
-            e = pool[i];
-            take = ~heap_key(e.val) <= T;
-        }
-        const uint64_t m = __ballot(take);
-        if (m == 0ull) continue;
-        const int leader = (int)__ffsll((long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&s_cnt, (uint32_t)__popcll(m));
-        base = __shfl(base, leader, WAVE);
-        const uint32_t pos = base + lane_rank(m);
-        if (take && pos < (uint32_t)qv_.qProbes) {
-            const uint32_t b = (uint32_t)e.idx;
-            out[pos] = b;
-            ct += bucket_off[b + 1] - bucket_off[b];
-        }
-    }
-    atomicAdd(&s_ct, ct);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        ctot[q] = s_ct;
-        qflag[q] |= RESOLVED;
-    }
-}
-
-// Per flagged query whose cut is ambiguous: the global merge replayed.  The
-// per-table lists are staged in LDS by the workgroup; one lane replays the
-// max-heap merge out of LDS (heap and cursors in LDS too).
-__global__ __launch_bounds__(MG_T) void k_probe_exact_merge(ProbeGeom g, const uint32_t* __restrict__ qlist,
+constexpr int RS_SEL_T = 256;
+__global__ __launch_bounds__(RS_SEL_T) void k_replay_select(ProbeGeom g, const uint32_t* __restrict__ qlist,
+                                                            const uint32_t* __restrict__ nflag, uint32_t cap,
                                                             const uint64_t* __restrict__ bucket_off,
-                                                            uint32_t* __restrict__ probes, uint64_t* __restrict__ ctot,
-                                                            const uint32_t* __restrict__ qflag,
+                                                            uint32_t* __restrict__ probes,
+                                                            uint64_t* __restrict__ ctot,
                                                             const unsigned char* __restrict__ work, int64_t wpq,
-                                                            int staged) {
-    extern __shared__ IFPair mlds[];
+                                                            int merge_staged) {
+    extern __shared__ __align__(16) unsigned char sl[];
     const uint32_t f = blockIdx.x;
+    if (f >= min(*nflag, cap)) return;
     const uint32_t q = qlist[f];
-    if (qflag[q] & RESOLVED) return;
     const QueryView qv_ = query_view(g.qp, (int)q, g.qProbes, g.perTable, g.PL, 0);
-    const int L = g.L, PT = qv_.perTable, QP = qv_.qProbes;
-    const IFPair* list_g = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
-    // the per-table lists are staged in LDS when they fit, else read in place
-    IFPair* mq = staged ? mlds + (int64_t)L * PT : mlds;
-    int* track = reinterpret_cast<int*>(mq + L);
-    if (staged)
-        for (int64_t i = threadIdx.x; i < (int64_t)L * PT; i += MG_T) mlds[i] = list_g[i];
-    const IFPair* list = staged ? mlds : list_g;
-    for (int l = threadIdx.x; l < L; l += MG_T) track[l] = 0;
-    __syncthreads();
-    uint32_t* out = probes + qv_.probe_off;
-    if (threadIdx.x == 0) {
-        MaxHeap mh;
-        mh.init(mq);
-        for (int l = 0; l < L; ++l) mh.push(list[(int64_t)l * PT]);
-        for (int c = 0; c < QP; ++c) {
-            const IFPair pr = mh.top();
-            mh.pop();
-            out[c] = (uint32_t)pr.idx;
-            const int tb = pr.idx / g.NB;
-            const int tk = ++track[tb];
-            if (tk < PT) mh.push(list[(int64_t)tb * PT + tk]);
+    const IFPair* pool = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
+    replay_select<RS_SEL_T>(g, qv_, q, pool, bucket_off, probes, ctot, sl, merge_staged);
+}
+
+constexpr int RX_T = 512;
+constexpr int RX_W = RX_T / WAVE;
+__global__ __launch_bounds__(RX_T) void k_probe_replay(HashGeom hg, ProbeGeom g, const float* __restrict__ Q,
+                                                       int64_t ldq, const uint32_t* __restrict__ masks,
+                                                       const uint32_t* __restrict__ qlist,
+                                                       const uint32_t* __restrict__ nflag, uint32_t first,
+                                                       uint32_t* claim, const uint32_t* __restrict__ qflag,
+                                                       const int32_t* __restrict__ qc,
+                                                       const float* __restrict__ qvals,
+                                                       const uint64_t* __restrict__ bucket_off,
+                                                       uint32_t* __restrict__ probes, uint64_t* __restrict__ ctot,
+                                                       unsigned char* __restrict__ work, int64_t wpq, int table_lds,
+                                                       int twaves, int merge_staged) {
+    extern __shared__ __align__(16) unsigned char rlds[];
+    __shared__ int s_f;
+    const int w = threadIdx.x / WAVE;
+    IFPair* pool = reinterpret_cast<IFPair*>(work + (int64_t)blockIdx.x * wpq);
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const uint32_t f = first + atomicAdd(claim, 1u);
+            s_f = f < *nflag ? (int)f : -1;
         }
+        __syncthreads();
+        const int f = s_f;
+        __syncthreads();
+        if (f < 0) break;
+        const uint32_t q = qlist[f];
+        const QueryView qv_ = query_view(g.qp, (int)q, g.qProbes, g.perTable, g.PL, 0);
+        const uint32_t flag = qflag[q];
+        if (w < twaves)
+            for (int l = w; l < g.L; l += twaves)
+                replay_table(hg, g, qv_, Q, ldq, masks, q, l, flag, qc, qvals, rlds + (int64_t)w * table_lds,
+                             pool + (int64_t)l * qv_.perTable);
+        __syncthreads();
+        replay_select<RX_T>(g, qv_, q, pool, bucket_off, probes, ctot, rlds, merge_staged);
     }
+}
+
+// The queries whose qflag is nonzero, in query order, into list; *count =
+// their number.  One workgroup.
+constexpr int CF_T = 1024;
+__global__ __launch_bounds__(CF_T) void k_compact_flagged(int nq, const uint32_t* __restrict__ qflag,
+                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    __shared__ uint32_t wsum[CF_T / WAVE];
+    __shared__ uint32_t s_base;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    if (threadIdx.x == 0) s_base = 0;
     __syncthreads();
-    // C_tot of the probed set
-    __shared__ unsigned long long s_ct;
-    if (threadIdx.x == 0) s_ct = 0;
-    __syncthreads();
-    unsigned long long ct = 0;
-    for (int c = threadIdx.x; c < QP; c += MG_T) {
-        const uint32_t b = out[c];
-        ct += bucket_off[b + 1] - bucket_off[b];
+    for (int c0 = 0; c0 < nq; c0 += CF_T) {
+        const int i = c0 + threadIdx.x;
+        const bool keep = i < nq && qflag[i] != 0u;
+        const uint64_t m = __ballot(keep);
+        if (lane == 0) wsum[wv] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = s_base, tot = 0;
+        for (int v = 0; v < CF_T / WAVE; ++v) {
+            before += v < wv ? wsum[v] : 0u;
+            tot += wsum[v];
+        }
+        if (keep) list[before + lane_rank(m)] = (uint32_t)i;
+        __syncthreads();
+        if (threadIdx.x == 0) s_base += tot;
+        __syncthreads();
     }
-    atomicAdd(&s_ct, ct);
-    __syncthreads();
-    if (threadIdx.x == 0) ctot[q] = s_ct;
+    if (threadIdx.x == 0) *count = s_base;
+}
+
+// Sort key for longest-first order: C_tot descending, tie-flagged queries
+// (replayed in the gather, C_tot not known yet) first.
+__global__ void k_ctot_keys(const uint64_t* __restrict__ ctot, const uint32_t* __restrict__ qflag, int nq,
+                            uint32_t* __restrict__ key) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const uint64_t c = ctot[q];
+    key[q] = qflag[q] ? 0u : 0xffffffffu - (uint32_t)(c >= 0xffffffffull ? 0xfffffffeull : c);
 }
 
 // Probes of a call whose bucket is empty (the reference's iNumEmptyBucket,
@@ -652,19 +509,15 @@ __global__ __launch_bounds__(256) void k_count_empty(const uint32_t* __restrict_
     if ((threadIdx.x & (WAVE - 1)) == 0 && c) atomicAdd(count, c);
 }
 
-size_t merge_lds_bytes(const ProbeGeom& g) {
-    return (size_t)g.L * g.perTable * sizeof(IFPair) + (size_t)g.L * (sizeof(IFPair) + sizeof(int));
-}
 
 // ---- gather + dedup + FP32 rerank + top-k ---------------------------------
 template <int TB, int CPL>
-__device__ __forceinline__ void k_gather_score_f32_one(GatherArgs a, const uint32_t* __restrict__ probes,
+__device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, const ReplayArgs& rp,
+                                                          const uint32_t* __restrict__ probes,
                                                           const uint64_t* __restrict__ bucket_off,
                                                           const int32_t* __restrict__ ids,
                                                           const float* __restrict__ Xc, const float* __restrict__ Q,
-                                                          int64_t ldq, const uint64_t* __restrict__ cand_off,
-                                                          int32_t* __restrict__ cand, float* __restrict__ score,
-                                                          uint64_t* __restrict__ ncand, int32_t* __restrict__ out_ids,
+                                                          int64_t ldq, int32_t* __restrict__ out_ids,
                                                           float* __restrict__ out_scores, int q) {
     extern __shared__ uint32_t lds[];
     const int64_t W = a.window_bits;
@@ -680,15 +533,24 @@ __device__ __forceinline__ void k_gather_score_f32_one(GatherArgs a, const uint3
     __shared__ uint32_t s_below;
     uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * 5 : nullptr;
     if (ts && threadIdx.x == 0) ts[0] = wall_clock64();
+    // a tie-flagged query's probes are replayed exactly first (the LDS is
+    // free until the dedup below)
+    if (rp.qflag && rp.qflag[q]) {
+        replay_query<TB>(rp, (uint32_t)q, reinterpret_cast<unsigned char*>(lds),
+                         reinterpret_cast<IFPair*>(rp.work + (int64_t)blockIdx.x * rp.wpq));
+        __syncthreads();
+    }
     const QueryView qv_ = query_view(a.qp, q, a.qProbes, 0, 0, a.k);
     const int kq = qv_.k;
     const uint32_t* pr = probes + qv_.probe_off;
-    const uint64_t cbase = cand_off[q];
-    const uint64_t ccap = cand_off[q + 1] - cbase;
-    int32_t* cq = cand + cbase;
-    float* sq = score + cbase;
-    const uint32_t nc = collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, a.part + cbase, ccap, ts);
-    if (threadIdx.x == 0) ncand[q] = nc;
+    // candidates in the query's scratch; FP32 scores (k > 64) over the
+    // window scatter, which is dead once the candidates are collected
+    const QueryScratch qs = query_scratch(a, q);
+    int32_t* cq = qs.cand;
+    float* sq = reinterpret_cast<float*>(qs.part);
+    const uint32_t nc =
+        collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, qs.part, qs.ccap, qs.pcap, ts);
+    if (threadIdx.x == 0) a.ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
     // FP32 scores: groups of 8 lanes per candidate row, CPL float4 chunks per
     // lane, U rows per group in flight, candidate ids fetched one step ahead.
@@ -870,88 +732,42 @@ __device__ __forceinline__ void k_gather_score_f32_one(GatherArgs a, const uint3
     if (ts && threadIdx.x == 0) ts[4] = wall_clock64();
 }
 
-template <int TB, int CPL, bool PERSIST>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB <= 512 ? 2 : TB / 256, TB <= 512 ? 2 : TB / 256))) void k_gather_score_f32(GatherArgs a, const uint32_t* __restrict__ probes,
+// Persistent: each workgroup takes queries from the work queue until it is
+// drained.
+template <int TB, int CPL>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB <= 512 ? 2 : TB / 256, TB <= 512 ? 2 : TB / 256))) void k_gather_score_f32(GatherArgs a, ReplayArgs rp, const uint32_t* __restrict__ probes,
                                                           const uint64_t* __restrict__ bucket_off,
                                                           const int32_t* __restrict__ ids,
                                                           const float* __restrict__ Xc, const float* __restrict__ Q,
-                                                          int64_t ldq, const uint64_t* __restrict__ cand_off,
-                                                          int32_t* __restrict__ cand, float* __restrict__ score,
-                                                          uint64_t* __restrict__ ncand, int32_t* __restrict__ out_ids,
+                                                          int64_t ldq, int32_t* __restrict__ out_ids,
                                                           float* __restrict__ out_scores) {
-    if constexpr (!PERSIST) {
-        const int q = a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
-        k_gather_score_f32_one<TB, CPL>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand,
-                                        out_ids, out_scores, q);
-    } else {
-        // persistent: take queries from the work queue until it is empty
-        __shared__ uint32_t s_item;
-        for (;;) {
-            if (threadIdx.x == 0) s_item = atomicAdd(a.queue, 1u);
-            __syncthreads();
-            const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
-            __syncthreads();
-            if (item >= a.nitems) break;
-            const int q = __builtin_amdgcn_readfirstlane((int)a.order[item]);
-            k_gather_score_f32_one<TB, CPL>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand,
-                                            out_ids, out_scores, q);
-            __syncthreads();
-        }
+    __shared__ int s_q;
+    for (;;) {
+        const int q = claim_query(a, &s_q);
+        if (q < 0) break;
+        k_gather_score_f32_one<TB, CPL>(a, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, q);
+        __syncthreads();
     }
 }
 
 // Sort key of a query: its best bucket of table 0 (layer-1 and layer-2 top
 // codes).  Queries with equal keys share most candidates.
-__global__ void k_order_keys(const int32_t* __restrict__ qc, int nq, int L, int PL, int two_np,
-                             uint32_t* __restrict__ key) {
+__global__ void k_order_keys(const int32_t* __restrict__ qc, const uint32_t* __restrict__ qflag, int nq, int L,
+                             int PL, int two_np, uint32_t* __restrict__ key) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     const size_t base = (size_t)q * L * 2 * PL;
-    key[q] = (uint32_t)(qc[base] * two_np + qc[base + PL]);
+    key[q] = qflag[q] ? 0u : 1u + (uint32_t)(qc[base] * two_np + qc[base + PL]);
 }
 
-// Block -> query map: XCD x (blocks x, x+8, ...) walks a contiguous run of the
-// sorted queries, so similar queries share that XCD's L2.
-__global__ void k_xcd_order(const uint32_t* __restrict__ sorted, int nq, int nxcd, uint32_t* __restrict__ order) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nq) return;
-    const int x = b % nxcd, j = b / nxcd;
-    int start = 0;
-    for (int y = 0; y < x; ++y) start += (nq - y + nxcd - 1) / nxcd;
-    order[b] = sorted[start + j];
-}
-
-// Stable compaction of the unflagged queries of order (identity when null),
-// one workgroup.
-constexpr int SO_T = 1024;
-__global__ __launch_bounds__(SO_T) void k_split_order(const uint32_t* __restrict__ order, int nq,
-                                                      const uint32_t* __restrict__ qflag,
-                                                      uint32_t* __restrict__ order_u) {
-    __shared__ uint32_t wsum[SO_T / WAVE];
-    __shared__ uint32_t s_base;
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    if (threadIdx.x == 0) s_base = 0;
-    __syncthreads();
-    for (int c0 = 0; c0 < nq; c0 += SO_T) {
-        const int i = c0 + threadIdx.x;
-        uint32_t q = 0;
-        bool keep = false;
-        if (i < nq) {
-            q = order ? order[i] : (uint32_t)i;
-            keep = qflag[q] == 0u;
-        }
-        const uint64_t m = __ballot(keep);
-        if (lane == 0) wsum[wv] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t before = s_base, tot = 0;
-        for (int v = 0; v < SO_T / WAVE; ++v) {
-            before += v < wv ? wsum[v] : 0u;
-            tot += wsum[v];
-        }
-        if (keep) order_u[before + lane_rank(m)] = q;
-        __syncthreads();
-        if (threadIdx.x == 0) s_base += tot;
-        __syncthreads();
+// dst row r = src row r (D floats) followed by zeros up to ldd.
+__global__ void k_pad_rows(const float* __restrict__ src, int64_t lds, int64_t rows, int D, float* __restrict__ dst,
+                           int64_t ldd) {
+    const int64_t total = rows * ldd;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / ldd;
+        const int c = (int)(i - r * ldd);
+        dst[i] = c < D ? src[r * lds + c] : 0.0f;
     }
 }
 
@@ -966,20 +782,13 @@ size_t gather_lds_bytes(const GatherArgs& a, int TB) {
 }
 
 template <int TB>
-void dispatch_gather_cpl(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,
-                         const int32_t* ids, const float* Xc, const float* Q, int64_t ldq, const uint64_t* cand_off,
-                         int32_t* cand, float* score, uint64_t* ncand, int32_t* out_ids, float* out_scores,
-                         hipStream_t st) {
+void dispatch_gather_cpl(const GatherArgs& a, const ReplayArgs& rp, size_t lds, unsigned grid,
+                         const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids, const float* Xc,
+                         const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores, hipStream_t st) {
     const int64_t nchunk = a.ldx / 4;
     const int cpl = (int)ceil_div(nchunk, 8);
-    const size_t lds = gather_lds_bytes(a, TB);
 #define FPP_G(C) \
-    if (a.queue)                                                                                                    \
-        k_gather_score_f32<TB, C, true><<<persistent_grid(nq, lds, a.spare_cus, TB), TB, lds, st>>>(                                   \
-            a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids, out_scores);               \
-    else                                                                                                            \
-        k_gather_score_f32<TB, C, false><<<nq, TB, lds, st>>>(a, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, \
-                                                             score, ncand, out_ids, out_scores)
+    k_gather_score_f32<TB, C><<<grid, TB, lds, st>>>(a, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores)
     if (cpl <= 1)
         FPP_G(1);
     else if (cpl <= 2)
@@ -1027,9 +836,17 @@ void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const fl
     FPP_HIP(hipGetLastError());
 }
 
-void launch_order_keys(const int32_t* qc, int nq, int L, int PL, int two_np, uint32_t* key, hipStream_t st) {
+void launch_order_keys(const int32_t* qc, const uint32_t* qflag, int nq, int L, int PL, int two_np, uint32_t* key,
+                       hipStream_t st) {
     if (nq == 0) return;
-    k_order_keys<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(qc, nq, L, PL, two_np, key);
+    k_order_keys<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(qc, qflag, nq, L, PL, two_np, key);
+    FPP_HIP(hipGetLastError());
+}
+
+void launch_pad_rows(const float* src, int64_t lds, int64_t rows, int D, float* dst, int64_t ldd, hipStream_t st) {
+    if (rows <= 0) return;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(rows * ldd, 256), 4096));
+    k_pad_rows<<<grid, 256, 0, st>>>(src, lds, rows, D, dst, ldd);
     FPP_HIP(hipGetLastError());
 }
 
@@ -1039,7 +856,7 @@ void launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st) {
     FPP_HIP(hipGetLastError());
 }
 
-unsigned persistent_grid(int nitems, size_t lds_bytes, int spare_cus, int threads) {
+unsigned persistent_grid(int nitems, size_t lds_bytes, int threads) {
     static int cus = 0;
     if (!cus) {
         int dev = 0;
@@ -1048,20 +865,7 @@ unsigned persistent_grid(int nitems, size_t lds_bytes, int spare_cus, int thread
     }
     // resident workgroups per CU: LDS, and two waves per SIMD
     const int per_cu = std::max<int>(1, std::min<int>(threads <= 256 ? 2 : 1, (int)(160 * 1024 / (lds_bytes + 1024))));
-    const int use = std::max(1, cus - std::max(0, spare_cus));
-    return (unsigned)std::max(1, std::min(nitems, use * per_cu));
-}
-
-void launch_split_order(const uint32_t* order, int nq, const uint32_t* qflag, uint32_t* order_u, hipStream_t st) {
-    if (nq == 0) return;
-    k_split_order<<<1, SO_T, 0, st>>>(order, nq, qflag, order_u);
-    FPP_HIP(hipGetLastError());
-}
-
-void launch_xcd_order(const uint32_t* sorted, int nq, int nxcd, uint32_t* order, hipStream_t st) {
-    if (nq == 0) return;
-    k_xcd_order<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(sorted, nq, nxcd, order);
-    FPP_HIP(hipGetLastError());
+    return (unsigned)std::max(1, std::min(nitems, cus * per_cu));
 }
 
 uint64_t count_empty_probes(const uint32_t* probes, int64_t n, const uint64_t* bucket_off, uint32_t* scratch2,
@@ -1077,30 +881,62 @@ uint64_t count_empty_probes(const uint32_t* probes, int64_t n, const uint64_t* b
     return c;
 }
 
-int64_t probe_exact_work_bytes(const HashGeom& hg, const ProbeGeom& g) {
-    return (exact_work_bytes(g.L, g.perTable) + 255) / 256 * 256;
+size_t replay_lds_bytes(ReplayArgs& r, int threads) {
+    constexpr size_t budget = 150 * 1024;
+    const int64_t tlds = exact_table_lds_bytes(r.hg.F, r.pg.PL, r.pg.perTable);
+    if (tlds > (int64_t)budget) return 0;
+    const size_t staged = (size_t)merge_lds_bytes(r.pg.L, r.pg.perTable, true);
+    const size_t in_place = (size_t)merge_lds_bytes(r.pg.L, r.pg.perTable, false);
+    if (in_place > budget) return 0;
+    r.table_lds = (int)tlds;
+    r.twaves = (int)std::min<int64_t>(threads / WAVE, (int64_t)budget / tlds);
+    r.merge_staged = staged <= budget ? 1 : 0;
+    return std::max((size_t)r.twaves * (size_t)tlds, r.merge_staged ? staged : in_place);
 }
 
-void launch_probe_exact(const HashGeom& hg, const ProbeGeom& g, const float* Q, int64_t ldq,
-                        const uint32_t* masks, const uint32_t* qlist, uint32_t nlist, uint32_t* qflag,
-                        const int32_t* qc, const float* qv, const uint64_t* bucket_off, uint32_t* probes,
-                        uint64_t* ctot, void* work, int64_t work_per_query, hipStream_t st) {
-    if (nlist == 0) return;
+int64_t probe_replay_work_bytes(const ProbeGeom& g) { return ((int64_t)g.L * g.perTable * 8 + 255) / 256 * 256; }
+
+void launch_probe_replay(const HashGeom& hg, const ProbeGeom& g, const float* Q, int64_t ldq, const uint32_t* masks,
+                         const uint32_t* qlist, const uint32_t* nflag, uint32_t* claim, const uint32_t* qflag,
+                         const int32_t* qc, const float* qv, const uint64_t* bucket_off, uint32_t* probes,
+                         uint64_t* ctot, void* work, int64_t wpq, unsigned cap, unsigned over, hipStream_t st) {
+    constexpr size_t budget = 150 * 1024;  // dynamic LDS; the kernels' own words stay below 2 KB
+    const int64_t tlds = exact_table_lds_bytes(hg.F, g.PL, g.perTable);
+    FPP_REQUIRE(tlds <= (int64_t)budget, FPP_EINVAL, "perTable too large for the exact probe replay");
+    const size_t staged = (size_t)merge_lds_bytes(g.L, g.perTable, true);
+    const size_t in_place = (size_t)merge_lds_bytes(g.L, g.perTable, false);
+    FPP_REQUIRE(in_place <= budget, FPP_EINVAL, "n_tables too large for the exact probe merge");
+    const bool stage = staged <= budget;
+    const size_t mlds = stage ? staged : in_place;
     unsigned char* w = reinterpret_cast<unsigned char*>(work);
-    const size_t tlds = (size_t)exact_table_lds_bytes(hg.F, g.PL, g.perTable);
-    FPP_REQUIRE(tlds <= 160 * 1024, FPP_EINVAL, "perTable too large for the exact probe replay");
-    k_probe_exact_tables<<<(unsigned)((int64_t)nlist * g.L), WAVE, tlds, st>>>(hg, g, Q, ldq, masks, qlist, qflag,
-                                                                            qc, qv, w, work_per_query);
+    if (cap > 0) {
+        const unsigned tgrid = (unsigned)std::min<int64_t>((int64_t)cap * g.L, 4096);
+        k_replay_tables<<<tgrid, WAVE, (size_t)tlds, st>>>(hg, g, Q, ldq, masks, qlist, nflag, cap, qflag, qc, qv, w,
+                                                          wpq);
+        FPP_HIP(hipGetLastError());
+        k_replay_select<<<cap, RS_SEL_T, mlds, st>>>(g, qlist, nflag, cap, bucket_off, probes, ctot, w, wpq,
+                                                     stage ? 1 : 0);
+        FPP_HIP(hipGetLastError());
+    }
+    if (over > 0) {
+        const int twaves = (int)std::min<int64_t>(RX_W, (int64_t)budget / tlds);
+        const size_t lds = std::max((size_t)twaves * (size_t)tlds, mlds);
+        k_probe_replay<<<over, RX_T, lds, st>>>(hg, g, Q, ldq, masks, qlist, nflag, cap, claim, qflag, qc, qv,
+                                                bucket_off, probes, ctot, w + (int64_t)cap * wpq, wpq, (int)tlds,
+                                                twaves, stage ? 1 : 0);
+        FPP_HIP(hipGetLastError());
+    }
+}
+
+void launch_compact_flagged(int nq, const uint32_t* qflag, uint32_t* list, uint32_t* count, hipStream_t st) {
+    if (nq <= 0) return;
+    k_compact_flagged<<<1, CF_T, 0, st>>>(nq, qflag, list, count);
     FPP_HIP(hipGetLastError());
-    k_probe_exact_select<<<nlist, MG_T, 0, st>>>(g, qlist, bucket_off, probes, ctot, qflag, w,
-                                                   work_per_query);
-    FPP_HIP(hipGetLastError());
-    const size_t full = merge_lds_bytes(g);
-    const bool staged = full <= 160 * 1024;
-    const size_t lds = staged ? full : (size_t)g.L * (sizeof(IFPair) + sizeof(int));
-    FPP_REQUIRE(lds <= 160 * 1024, FPP_EINVAL, "n_tables too large for the exact probe merge");
-    k_probe_exact_merge<<<nlist, MG_T, lds, st>>>(g, qlist, bucket_off, probes, ctot, qflag, w, work_per_query,
-                                                  staged ? 1 : 0);
+}
+
+void launch_ctot_keys(const uint64_t* ctot, const uint32_t* qflag, int nq, uint32_t* key, hipStream_t st) {
+    if (nq <= 0) return;
+    k_ctot_keys<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(ctot, qflag, nq, key);
     FPP_HIP(hipGetLastError());
 }
 
@@ -1116,24 +952,26 @@ int64_t gather_window_bits(int64_t N, int64_t extra_words, int per_cu) {
     return 0;
 }
 
-void launch_gather_score_f32(const GatherArgs& a, int nq, const uint32_t* probes, const uint64_t* bucket_off,
-                             const int32_t* ids, const float* Xc, const float* Q, int64_t ldq,
-                             const uint64_t* cand_off, int32_t* cand, float* score, uint64_t* ncand,
-                             int32_t* out_ids, float* out_scores, int threads, hipStream_t st) {
+size_t gather_f32_lds(const GatherArgs& a, int threads) { return gather_lds_bytes(a, threads); }
+
+void launch_gather_score_f32(const GatherArgs& a, const ReplayArgs& rp, size_t lds, unsigned grid,
+                             const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids, const float* Xc,
+                             const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores, int threads,
+                             hipStream_t st) {
     FPP_REQUIRE(a.k >= 1 && a.k <= 1024, FPP_EINVAL, "n_neighbors must be in [1, 1024]");
     FPP_REQUIRE(a.ldx % 4 == 0, FPP_EINTERNAL, "row stride must be a multiple of 4");
     FPP_REQUIRE(a.ldx <= 1024, FPP_EINVAL, "n_features must be <= 1024");
-    if (nq == 0) return;
-    if (a.queue) launch_fill_u32(a.queue, 0u, 1, st);
+    FPP_REQUIRE(a.heads && a.ncand && a.cand && a.part && a.nqueues >= 1, FPP_EINTERNAL, "gather scratch missing");
+    FPP_REQUIRE(lds >= gather_lds_bytes(a, threads <= 256 ? 256 : (threads <= 512 ? 512 : 1024)) &&
+                    lds <= 160 * 1024,
+                FPP_EINTERNAL, "gather LDS size");
+    if (a.nitems == 0 || grid == 0) return;
     if (threads <= 256)
-        dispatch_gather_cpl<256>(a, nq, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids,
-                                 out_scores, st);
+        dispatch_gather_cpl<256>(a, rp, lds, grid, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
     else if (threads <= 512)
-        dispatch_gather_cpl<512>(a, nq, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand, out_ids,
-                                 out_scores, st);
+        dispatch_gather_cpl<512>(a, rp, lds, grid, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
     else
-        dispatch_gather_cpl<1024>(a, nq, probes, bucket_off, ids, Xc, Q, ldq, cand_off, cand, score, ncand,
-                                  out_ids, out_scores, st);
+        dispatch_gather_cpl<1024>(a, rp, lds, grid, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
 }
 
 }  // namespace fpp

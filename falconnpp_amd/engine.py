@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "libfalconnpp_amd.so")
 # Every symbol include/falconnpp_amd.h declares.
 EXPORTS = (
     "fpp_last_error", "fpp_create", "fpp_destroy", "fpp_set_index_param", "fpp_set_qprobes", "fpp_set_threads",
-    "fpp_clear", "fpp_build", "fpp_build_device", "fpp_query", "fpp_query_2d", "fpp_query_device", "fpp_get_shape",
+    "fpp_clear", "fpp_build", "fpp_build_device", "fpp_set_insertable", "fpp_insert", "fpp_insert_device", "fpp_query", "fpp_query_2d", "fpp_query_device", "fpp_get_shape",
     "fpp_get_center", "fpp_get_sign_bits", "fpp_get_buckets", "fpp_get_bucket_counts", "fpp_hash_points", "fpp_get_stats",
     "fpp_get_last_probes", "fpp_get_last_candidates", "fpp_bruteforce_f64", "fpp_set_profiling",
     "fpp_get_timings", "fpp_set_tuning", "fpp_set_precision", "fpp_get_fp8_scale", "fpp_autotune", "fpp_autotune_batch",
@@ -28,6 +28,8 @@ EXPORTS = (
 )
 PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
 TUNE_QUERY_ORDER, TUNE_GATHER_THREADS, TUNE_PERSISTENT, TUNE_GRAPH, TUNE_VERBOSE, TUNE_FP8_PER_ROW = 1, 2, 3, 4, 5, 6
+TUNE_KEEP_CANDIDATES = 7
+TUNE_REPLAY = 8  # 0 per call, 1 separate kernels, 2 inside the gather
 
 FPP_EINVAL, FPP_ESTATE, FPP_EHIP, FPP_ENOMEM, FPP_EINTERNAL, FPP_EIO = 1, 2, 3, 4, 5, 6
 
@@ -73,6 +75,9 @@ def load():
     L.fpp_clear.argtypes = [vp]
     L.fpp_build.argtypes = [vp, vp, i64]
     L.fpp_build_device.argtypes = [vp, vp, i64, vp]
+    L.fpp_set_insertable.argtypes = [vp, i32]
+    L.fpp_insert.argtypes = [vp, vp, i64, i32]
+    L.fpp_insert_device.argtypes = [vp, vp, i64, i32, vp]
     L.fpp_query.argtypes = [vp, vp, i64, i32, i32, vp, i32]
     L.fpp_query_2d.argtypes = [vp, vp, i64, i32, i32, vp, i32]
     L.fpp_query_device.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
@@ -197,6 +202,17 @@ class Index:
     def build_device(self, dX_ptr: int, ld: int, stream: int = 0):
         _check(load().fpp_build_device(self.h, C.c_void_p(dX_ptr), ld, C.c_void_p(stream or None)))
 
+    def set_insertable(self, on: bool = True):
+        """Keep the pre-filter bucket entries at the next build, for insert()."""
+        _check(load().fpp_set_insertable(self.h, 1 if on else 0))
+
+    def insert(self, X_md):
+        """Append m points (rows) as ids n .. n+m-1; buckets are filtered again."""
+        X = _points(X_md)
+        assert X.shape[1] == self.d
+        _check(load().fpp_insert(self.h, _ptr(X), self.d, X.shape[0]))
+        self.n += X.shape[0]
+
     def query(self, Q_nd, k: int, verbose: bool = False) -> np.ndarray:
         Q = _points(Q_nd)
         nq = Q.shape[0]
@@ -247,10 +263,10 @@ class Index:
 
     # ---- inspection ----
     def shape(self):
-        s = np.zeros(5, np.int64)
+        s = np.zeros(6, np.int64)
         _check(load().fpp_get_shape(self.h, _ptr(s)))
         return dict(fht_dim=int(s[0]), buckets_per_table=int(s[1]), n_tables=int(s[2]), n_ids=int(s[3]),
-                    ldx=int(s[4]))
+                    ldx=int(s[4]), n_points=int(s[5]))
 
     def center(self) -> np.ndarray:
         c = np.zeros(self.d, np.float32)
@@ -299,12 +315,14 @@ class Index:
         _check(load().fpp_set_profiling(self.h, int(level)))
 
     def timings(self):
-        """Device ms of the last query (hash, probe select, tie replay, gather+rerank)
-        and mean per-query us of the gather kernel's phases."""
-        t = np.zeros(8, np.float64)
+        """Device ms of the last query (hash, probe select, tie replay, gather+rerank),
+        mean per-query us of the gather kernel's phases, the longest query's us
+        in that kernel and the longest first phase (replay + bucket gather) of
+        a tie-flagged query."""
+        t = np.zeros(10, np.float64)
         _check(load().fpp_get_timings(self.h, _ptr(t)))
         return dict(hash=t[0], probe=t[1], replay=t[2], gather=t[3], ph_gather_us=t[4], ph_compact_us=t[5],
-                    ph_score_us=t[6], ph_topk_us=t[7])
+                    ph_score_us=t[6], ph_topk_us=t[7], longest_query_us=t[8], longest_flagged_first_us=t[9])
 
     def set_precision(self, mode: int, refine: int = 0):
         """0 FP32 (reference), 1 BF16 + FP32 refine, 2 FP8 + FP32 refine."""

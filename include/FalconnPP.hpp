@@ -60,6 +60,15 @@ public:
         return out;
     }
 
+    // streaming insert (no reference counterpart; src/FalconnPP.cpp:711-713
+    // announces it): setInsertable(true) before build, then insert appends m
+    // points (D x m column-major, leading dimension ld) as ids n .. n+m-1
+    void setInsertable(bool on) { check(fpp_set_insertable(h_, on ? 1 : 0)); }
+    void insert(const float* X, int64_t ld, int m) {
+        check(fpp_insert(h_, X, ld, m));
+        n_points_ += m;
+    }
+
     // index files (no reference counterpart)
     void save(const std::string& path) { check(fpp_save(h_, path.c_str())); }
     void load(const std::string& path) { check(fpp_load(h_, path.c_str())); }

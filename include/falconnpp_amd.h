@@ -82,9 +82,27 @@ int fpp_query_2d(fpp_index* h, const float* Q, int64_t ldq, int nq, int k, int32
 int fpp_query_device(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_out, float* d_scores,
                      void* stream);
 
+/* ---- streaming insert (no reference counterpart: the reference keeps its 2D
+ * layout "to support delete and insert new points in future",
+ * src/FalconnPP.cpp:711-713, and implements none) ---------------------------
+ * fpp_set_insertable(h, 1) before build keeps every point's bucket entries
+ * from before the alpha / bucketLimit filter (n_points * n_tables * iProbes
+ * codes and values, 8 bytes each).  fpp_insert then appends m points (host,
+ * D x m col-major, leading dimension ld) as ids n_points .. n_points+m-1:
+ * centred with the index's centre (not recomputed), hashed with its sign
+ * bits, and every bucket filtered again over old and new entries in point
+ * order, so the index equals build2Layers_1D (src/FalconnPP.cpp:21-303) of
+ * all points with that centre.  Cost: that of a build without the centre
+ * and the old points' hashing.  FPP_ESTATE when the index keeps no entries. */
+int fpp_set_insertable(fpp_index* h, int on);
+int fpp_insert(fpp_index* h, const float* X, int64_t ld, int m);
+/* Same with the points in device memory; stream may be NULL. */
+int fpp_insert_device(fpp_index* h, const float* dX, int64_t ld, int m, void* stream);
+
 /* ---- inspection (parity tests and tools; no reference counterpart) ------ */
 /* index shape: out[0]=fhtDim, [1]=4*n_proj^2 buckets per table, [2]=n_tables,
- * [3]=number of stored ids, [4]=row stride of the centred copy */
+ * [3]=number of stored ids, [4]=row stride of the centred copy,
+ * [5]=n_points (grows with fpp_insert).  out must hold 6 values. */
 int fpp_get_shape(fpp_index* h, int64_t* out);
 /* centre of the data (n_features floats) */
 int fpp_get_center(fpp_index* h, float* out);
@@ -113,7 +131,10 @@ int fpp_get_stats(fpp_index* h, int64_t* out);
  * [1]=probe selection, [2]=tie replay, [3]=gather + dedup + rerank + top-k;
  * with on >= 2 also the mean per-query microseconds of that last kernel's
  * phases (device wall clock): [4]=bucket gather + dedup, [5]=candidate
- * compaction, [6]=scoring, [7]=top-k.  ms must hold 8 doubles. */
+ * compaction, [6]=scoring, [7]=top-k, [8]=the longest query's time in that
+ * kernel, [9]=the longest first phase of a tie-flagged query (its exact probe
+ * replay when done inside the gather, then its bucket gather).  ms must hold
+ * 10 doubles. */
 int fpp_set_profiling(fpp_index* h, int on);
 int fpp_get_timings(fpp_index* h, double* ms);
 /* Scoring precision (no reference counterpart; the reference scores in FP32):
@@ -131,26 +152,37 @@ int fpp_set_precision(fpp_index* h, int mode, int refine);
  * every row uses, or 0 when rows keep their own.  FPP_ESTATE before an FP8
  * query has made the copy. */
 int fpp_get_fp8_scale(fpp_index* h, float* shared);
-/* launch-shape knobs (no reference counterpart): FPP_TUNE_QUERY_ORDER 0/1
- * (process queries grouped by their best table-0 bucket, XCD-aware),
- * FPP_TUNE_GATHER_THREADS 256/512/1024 (workgroup of the gather kernel; 256
- * runs two workgroups per CU, the reduced-precision kernels use 256 or 512),
- * FPP_TUNE_PERSISTENT 0/1 (one persistent gather launch whose workgroups take
- * queries from a work queue, longest first), FPP_TUNE_GRAPH 0/1 (hashing and
- * probe selection of a batch captured once in a hipGraph and replayed while
- * the batch size and buffers stay the same), FPP_TUNE_VERBOSE 0/1 (the
- * autotuner prints one line per launch shape it times to stderr).  None
- * changes any result.  FPP_TUNE_FP8_PER_ROW 0/1: the FP8 copy keeps one
+/* launch-shape knobs (no reference counterpart).  The gather is one
+ * persistent launch whose workgroups take queries from a work queue.
+ * FPP_TUNE_QUERY_ORDER: the queue's order, 0 = query order, 1 = by the best
+ * table-0 bucket (queries sharing candidates together) in 8 runs, one per
+ * XCD (default), 2 = longest first (by C_tot).  FPP_TUNE_GATHER_THREADS
+ * 256/512/1024 (workgroup of the gather kernel; 256 runs two workgroups per
+ * CU, the reduced-precision kernels use 256 or 512).  FPP_TUNE_PERSISTENT 0/1
+ * kept for callers of earlier versions: 1 sets order 2, 0 sets order 1.
+ * FPP_TUNE_GRAPH 0/1: the whole query call (copy, hashing, probe selection,
+ * exact replay, ordering, gather) captured once in a hipGraph and replayed
+ * while the call's shape and buffers stay the same.  FPP_TUNE_VERBOSE 0/1:
+ * the autotuner prints one line per launch shape it times to stderr.
+ * FPP_TUNE_KEEP_CANDIDATES 0/1: every query's candidate list is kept after
+ * the call for fpp_get_last_candidates (costs one host read per call and
+ * memory for all lists).  None of these changes any result.  FPP_TUNE_FP8_PER_ROW 0/1: the FP8 copy keeps one
  * scale per row even where one shared scale would do (the copy is remade on
  * the next FP8 query); it changes the approximate scores, so it can change
- * which candidates reach the FP32 refine. */
+ * which candidates reach the FP32 refine.  FPP_TUNE_REPLAY: where the exact
+ * probe replay of the tie-flagged queries runs, 0 = chosen per call (default),
+ * 1 = separate kernels before the gather (each (query, table) on a wave of its
+ * own across the device), 2 = inside the gather, by the workgroup that claims
+ * the query, before it gathers it (flagged queries head the queue). */
 enum {
     FPP_TUNE_QUERY_ORDER = 1,
     FPP_TUNE_GATHER_THREADS = 2,
     FPP_TUNE_PERSISTENT = 3,
     FPP_TUNE_GRAPH = 4,
     FPP_TUNE_VERBOSE = 5,
-    FPP_TUNE_FP8_PER_ROW = 6
+    FPP_TUNE_FP8_PER_ROW = 6,
+    FPP_TUNE_KEEP_CANDIDATES = 7,
+    FPP_TUNE_REPLAY = 8
 };
 int fpp_set_tuning(fpp_index* h, int which, int value);
 /* Autotuner (no reference counterpart): times the gather workgroup sizes,

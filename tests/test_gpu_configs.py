@@ -48,6 +48,7 @@ def _engine(X, c):
     g = E.Index(n, d)
     g.setIndexParam(c["L"], c["n_proj"], c["bucketLimit"], c["alpha"], c["iProbes"], THREADS, synth.INDEX_SEED)
     g.build(X)
+    g.set_tuning(E.TUNE_KEEP_CANDIDATES, 1)
     return g
 
 
@@ -264,9 +265,11 @@ def test_c5_full_size_properties_and_sample():
     L, p = c["L"], c["iProbes"]
     _log(f"c5 n={n} d={d} L={L}: engine build")
     g = _engine(X, c)
+    _log("c5: built; reading the tables")
     cnt = g.bucket_counts()
     off, ids = g.buckets()
     nb = g.shape()["buckets_per_table"]
+    _log(f"c5: {len(ids)} ids; checking sizes and the iLimit rule")
     # every point lands in iProbes buckets per table before the filter
     per_table = cnt.reshape(L, nb).astype(np.int64).sum(1)
     assert np.all(per_table == n * p)
@@ -280,14 +283,17 @@ def test_c5_full_size_properties_and_sample():
     inside[off[:-1][size > 0].astype(np.int64)] = False
     assert np.all(np.diff(ids.astype(np.int64))[inside[1:]] > 0)
     # a point is kept at most iProbes times per table
+    _log("c5: ids per table")
     for t in range(L):
         a, b = int(off[t * nb]), int(off[(t + 1) * nb])
-        assert np.bincount(ids[a:b], minlength=n).max() <= p, t
+        srt = np.sort(ids[a:b])
+        assert len(srt) <= p or np.all(srt[p:] != srt[:-p]), t
     _log(f"c5: {len(ids)} ids kept of {n * L * p}; oracle adopts the index")
     center = g.center()
     o = _oracle(n, d, c)
     o.load_index(X, center, off, ids)
     del off, ids, cnt
+    _log("c5: sample queries against the oracle")
     k, sample = c["k"], 40
     Qs = Q[:sample]
     g.set_qProbes(c["qProbes"])
@@ -298,6 +304,7 @@ def test_c5_full_size_properties_and_sample():
         _check_query(g, o, X, center, Qs[j], j, probes[j], k, out[j])
     del o
     # FP8 with one scale per row (the forced per-vector scale), refine 200
+    _log("c5: FP64 truth, FP8 with per-row scales")
     truth, _ = _truth(X, Qs, k, out)
     g.set_tuning(E.TUNE_FP8_PER_ROW, 1)
     try:

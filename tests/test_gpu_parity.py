@@ -83,6 +83,7 @@ def _built(name):
     g = E.Index(n, d)
     g.setIndexParam(L, n_proj, blim, alpha, p, 8, 42)
     g.build(X)
+    g.set_tuning(E.TUNE_KEEP_CANDIDATES, 1)  # candidate lists read by the tests
     o = P.OracleIndex(n, d)
     o.setIndexParam(L, n_proj, blim, alpha, p, 8, 42)
     o.build(X, g.center())
@@ -368,6 +369,7 @@ def test_fp8_rows_keep_their_own_scales_when_magnitudes_spread():
     g = E.Index(n, d)
     g.setIndexParam(8, 128, 10, 0.2, 2, 8, 42)
     g.build(X)
+    g.set_tuning(E.TUNE_KEEP_CANDIDATES, 1)
     am = np.abs(X - g.center()).max(1)
     am = am[am > 0]
     assert am.max() > 256 * am.min()
@@ -546,11 +548,57 @@ def test_autotune_batch_keeps_results():
     m = min(len(Q), 256)
     ch = g.autotune_batch(dQ.data_ptr(), Q.shape[1], m, qps[:m], ks[:m], kmax)
     assert ch["gather_threads"] in (256, 512, 1024) and ch["us_per_batch"] > 0
-    assert ch["persistent"] in (0, 1) and (ch["persistent"] == 0 or ch["query_order"] == 0)
+    assert ch["query_order"] in (1, 2) and ch["persistent"] == (1 if ch["query_order"] == 2 else 0)
     try:
         out, sc = _run_batch(g, Q, qps, ks, kmax)
         assert np.array_equal(out, ref) and np.array_equal(sc, rsc)
     finally:
         g.set_tuning(E.TUNE_PERSISTENT, 0)
+        g.set_tuning(E.TUNE_QUERY_ORDER, 1)
+        g.set_tuning(E.TUNE_GATHER_THREADS, 512)
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_kept_candidates_and_graph_do_not_change_results(precision):
+    """The production call (each gather workgroup reuses its own scratch slot,
+    nothing read back) and its hipGraph replays answer exactly like the
+    inspection call that keeps every query's candidates."""
+    import torch
+    X, Q, g, qps, ks = _batch_case(seed=9)
+    kmax = int(ks.max())
+    g.set_precision(precision, 0)
+    try:
+        ref, rsc = _run_batch(g, Q, qps, ks, kmax)
+        rst = g.stats()
+        g.set_tuning(E.TUNE_KEEP_CANDIDATES, 0)
+        for graph in (0, 1, 1, 1):
+            g.set_tuning(E.TUNE_GRAPH, graph)
+            out, sc = _run_batch(g, Q, qps, ks, kmax)
+            assert np.array_equal(out, ref) and np.array_equal(sc, rsc), graph
+            st = g.stats()
+            assert (st["ctot"], st["cuniq"], st["query_ties"]) == (rst["ctot"], rst["cuniq"], rst["query_ties"])
+            t = torch.arange(1 << 16, device="cuda", dtype=torch.float32)
+            assert float((t + 1).sum().item()) == float((1 << 16) * ((1 << 16) + 1) // 2)
+        with pytest.raises(E.FppError) as e:
+            g.last_candidates(0)
+        assert e.value.code == E.FPP_ESTATE
+    finally:
+        g.set_tuning(E.TUNE_GRAPH, 0)
+        g.set_tuning(E.TUNE_KEEP_CANDIDATES, 1)
+        g.set_precision(0, 0)
+
+
+def test_order_modes_and_workgroup_sizes_do_not_change_results():
+    X, Q, g, qps, ks = _batch_case(seed=13)
+    kmax = int(ks.max())
+    ref, rsc = _run_batch(g, Q, qps, ks, kmax)
+    try:
+        for order in (0, 1, 2):
+            for t in (256, 512, 1024):
+                g.set_tuning(E.TUNE_QUERY_ORDER, order)
+                g.set_tuning(E.TUNE_GATHER_THREADS, t)
+                out, sc = _run_batch(g, Q, qps, ks, kmax)
+                assert np.array_equal(out, ref) and np.array_equal(sc, rsc), (order, t)
+    finally:
         g.set_tuning(E.TUNE_QUERY_ORDER, 1)
         g.set_tuning(E.TUNE_GATHER_THREADS, 512)
