@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <functional>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -1272,6 +1273,33 @@ void autotune(Index& I, hipStream_t st, Run run, int32_t* chosen) {
 
 using namespace fpp;
 
+// Workspace of fpp_query_device_noalloc: a device array of fixed size that
+// each GPU the library runs on holds until the process ends.  On the host the
+// array's name is not a device address; the address of each device's copy is
+// taken once with hipGetSymbolAddress (with that device current) and kept.
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+unsigned char* noalloc_scratch() {
+    int dev = 0;
+    FPP_HIP(hipGetDevice(&dev));
+    FPP_REQUIRE(dev >= 0 && dev < kMaxDevices, FPP_EINTERNAL, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        FPP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        FPP_REQUIRE(p != nullptr, FPP_EHIP, "no device address for the fixed workspace");
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
 struct fpp_index {
     Index I;
 };
@@ -1411,6 +1439,20 @@ int fpp_query_device(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, 
         // on the library's own stream the call returns with the results
         // written; on the caller's stream it is asynchronous
         do_query(h->I, dQ, ldq, nq, k, d_out, d_scores, pick_stream(h->I, stream), nullptr, nullptr, nullptr, 0,
+                 stream == nullptr);
+    });
+}
+
+int fpp_query_device_noalloc(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_out,
+                             float* d_scores, void* stream) {
+    return guard([&] {
+        FPP_REQUIRE(h && (nq == 0 || (dQ && d_out)), FPP_EINVAL, "null argument");
+        Index& I = h->I;
+        // no stream of the caller's and none made yet: the null stream
+        // (creating one is left to the allocating entry points)
+        hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : I.stream;
+        unsigned char* ws = noalloc_scratch();
+        do_query(I, dQ, ldq, nq, k, d_out, d_scores, st, nullptr, nullptr, ws, kNoAllocScratchBytes,
                  stream == nullptr);
     });
 }

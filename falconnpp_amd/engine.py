@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "libfalconnpp_amd.so")
 # Every symbol include/falconnpp_amd.h declares.
 EXPORTS = (
     "fpp_last_error", "fpp_create", "fpp_destroy", "fpp_set_index_param", "fpp_set_qprobes", "fpp_set_threads",
-    "fpp_clear", "fpp_build", "fpp_build_device", "fpp_set_insertable", "fpp_insert", "fpp_insert_device", "fpp_query", "fpp_query_2d", "fpp_query_device", "fpp_get_shape",
+    "fpp_clear", "fpp_build", "fpp_build_device", "fpp_set_insertable", "fpp_insert", "fpp_insert_device", "fpp_query", "fpp_query_2d", "fpp_query_device", "fpp_query_device_noalloc", "fpp_get_shape",
     "fpp_get_center", "fpp_get_sign_bits", "fpp_get_buckets", "fpp_get_bucket_counts", "fpp_hash_points", "fpp_get_stats",
     "fpp_get_last_probes", "fpp_get_last_candidates", "fpp_bruteforce_f64", "fpp_set_profiling",
     "fpp_get_timings", "fpp_set_tuning", "fpp_set_precision", "fpp_get_fp8_scale", "fpp_autotune", "fpp_autotune_batch",
@@ -81,6 +81,7 @@ def load():
     L.fpp_query.argtypes = [vp, vp, i64, i32, i32, vp, i32]
     L.fpp_query_2d.argtypes = [vp, vp, i64, i32, i32, vp, i32]
     L.fpp_query_device.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
+    L.fpp_query_device_noalloc.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
     L.fpp_get_shape.argtypes = [vp, vp]
     L.fpp_get_center.argtypes = [vp, vp]
     L.fpp_get_sign_bits.argtypes = [vp, vp, vp]
@@ -260,6 +261,13 @@ class Index:
                      stream: int = 0):
         _check(load().fpp_query_device(self.h, C.c_void_p(dQ_ptr), ldq, nq, k, C.c_void_p(dout_ptr),
                                        C.c_void_p(dscores_ptr or None), C.c_void_p(stream or None)))
+
+    def query_device_noalloc(self, dQ_ptr: int, ldq: int, nq: int, k: int, dout_ptr: int, dscores_ptr: int = 0,
+                             stream: int = 0):
+        """query_device with every scratch buffer in the library's fixed
+        device array: the call allocates nothing."""
+        _check(load().fpp_query_device_noalloc(self.h, C.c_void_p(dQ_ptr), ldq, nq, k, C.c_void_p(dout_ptr),
+                                               C.c_void_p(dscores_ptr or None), C.c_void_p(stream or None)))
 
     # ---- inspection ----
     def shape(self):

@@ -99,6 +99,26 @@ int fpp_insert(fpp_index* h, const float* X, int64_t ld, int m);
 /* Same with the points in device memory; stream may be NULL. */
 int fpp_insert_device(fpp_index* h, const float* dX, int64_t ld, int m, void* stream);
 
+/* No counterpart in the reference.  fpp_query_device that allocates nothing,
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, serving loops and harnesses that forbid allocation.  Every
+ * scratch buffer of the call (query copy, hash lists, probes, counts,
+ * ordering, replay work, candidate slots) lives in one fixed device array of
+ * 256 MiB that the library holds per GPU until the process ends.  A batch
+ * that does not fit (its per-query arrays, about nq * (n_tables * perLayer
+ * * 16 + qProbes * 4) bytes, plus one candidate slot) fails with FPP_EINVAL
+ * before any work; a batch that fits with fewer candidate slots than the
+ * allocating call uses runs on fewer gather workgroups.  FP32 scoring needs
+ * nothing else; BF16 / FP8 need the reduced-precision copy, made by one
+ * fpp_query_device call first (FPP_ESTATE otherwise).  The results equal
+ * fpp_query_device's.  Calls on one GPU share the array, which keeps the
+ * last call's contents: calls that overlap on one GPU (two streams, two
+ * threads) race and must be serialised by the caller.  stream may be NULL
+ * (the index's stream if it has one, else the null stream; the call then
+ * waits for its results). */
+int fpp_query_device_noalloc(fpp_index* h, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_out,
+                             float* d_scores, void* stream);
+
 /* ---- inspection (parity tests and tools; no reference counterpart) ------ */
 /* index shape: out[0]=fhtDim, [1]=4*n_proj^2 buckets per table, [2]=n_tables,
  * [3]=number of stored ids, [4]=row stride of the centred copy,
