@@ -345,7 +345,7 @@ void fill_tables(Index& I, const uint32_t* codes, const float* vals, int64_t N, 
     uint64_t kept = 0;
     FPP_HIP(hipMemcpyAsync(&kept, I.bucket_off.get() + nbk, 8, hipMemcpyDeviceToHost, st));
     FPP_HIP(hipStreamSynchronize(st));
-    I.ids.reserve(kept);
+    I.ids.reserve(kept + IDS_PAD);
     I.n_ids = (int64_t)kept;
     // per table: stable sort by bucket, copy / select / exact replay
     DevBuf<uint32_t> k0, v0, k1, v1, hist, sel, selc, ties, tiec;
@@ -823,6 +823,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     ga.phase_ts = stamps ? reinterpret_cast<uint64_t*>(W + lo.phase_ts) : nullptr;
     ga.heads = counters;
     ga.ncand = ncand;
+    ga.ctot = ctot;
     ga.nitems = (uint32_t)nq;
     ga.cand = reinterpret_cast<int32_t*>(W + lo.cand);
     ga.part = reinterpret_cast<int32_t*>(W + lo.part);
@@ -1189,7 +1190,7 @@ void load_index(Index& I, const char* path) {
     });
     FPP_REQUIRE((int64_t)prev == h.n_ids, FPP_EINVAL, "index file: bucket offsets do not end at the id count");
     FPP_REQUIRE(largest <= (uint64_t)h.n_points, FPP_EINVAL, "index file: a bucket holds more ids than points");
-    I.ids.reserve((size_t)std::max<int64_t>(h.n_ids, 1));
+    I.ids.reserve((size_t)(h.n_ids + IDS_PAD));
     read_device(in, I.ids.get(), (size_t)h.n_ids, [&](const int32_t* v, size_t c) {
         for (size_t i = 0; i < c; ++i)
             FPP_REQUIRE(v[i] >= 0 && (int64_t)v[i] < h.n_points, FPP_EINVAL, "index file: point id out of range");

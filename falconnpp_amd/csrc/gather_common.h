@@ -117,6 +117,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 //    lane per id; a binary search over the wave's prefix sums of bucket
 //    lengths maps a position to its probe (six dependent lane shuffles per
 //    id, so only where lane per bucket would leave most lanes idle).
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+
 template <int TB, int ULANE>
 struct BucketWalk {
     static constexpr int NW = TB / WAVE;
@@ -158,10 +160,21 @@ struct BucketWalk {
         for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, WAVE));
         mx = __builtin_amdgcn_readfirstlane(mx);
         if (mx <= LANE_MAX) {
+            // four ids per load (the id array is padded past its end), so a
+            // step issues a quarter of the load instructions, each touching
+            // one line per lane
+            static_assert(ULANE % 4 == 0, "ids are read four at a time");
             for (uint32_t j0 = 0; j0 < mx; j0 += ULANE) {
                 int32_t v[ULANE];
 #pragma unroll
-                for (int r = 0; r < ULANE; ++r) v[r] = j0 + r < len ? ids[s + j0 + r] : 0;
+                for (int r = 0; r < ULANE; r += 4) {
+                    i32x4 t = {0, 0, 0, 0};
+                    if (j0 + r < len) __builtin_memcpy(&t, ids + s + j0 + r, sizeof(t));
+                    v[r] = t[0];
+                    v[r + 1] = t[1];
+                    v[r + 2] = t[2];
+                    v[r + 3] = t[3];
+                }
 #pragma unroll
                 for (int r = 0; r < ULANE; ++r) f(v[r], j0 + r < len);
             }
@@ -324,28 +337,80 @@ __device__ __forceinline__ bool bitset_insert(uint32_t* bits, int64_t rel) {
     return (atomicOr(&bits[rel >> 5], m) & m) == 0u;
 }
 
+// Insert of id v into an LDS open-addressing set of 2^lg words (word 0 =
+// empty, id v stored as v + 1, linear probing): true when v was absent.  The
+// set is sized to at least twice the ids offered, so a free word is always
+// found; the probe count is bounded all the same.
+__device__ __forceinline__ bool hashset_insert(uint32_t* T, int lg, int32_t v) {
+    const uint32_t mask = (1u << lg) - 1u;
+    const uint32_t key = (uint32_t)v + 1u;
+    uint32_t h = ((uint32_t)v * 0x9E3779B1u) >> (32 - lg);
+    for (uint32_t n = 0; n <= mask; ++n) {
+        const uint32_t old = atomicCAS(&T[h], 0u, key);
+        if (old == 0u) return true;
+        if (old == key) return false;
+        h = (h + 1u) & mask;
+    }
+    return false;
+}
+
+// Zeroes words [0, n) of the dedup area.
+template <int TB>
+__device__ __forceinline__ void clear_words(uint32_t* bits, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += TB) bits[i] = 0u;
+}
+
 // Collect the unique candidates of one query into cq (capacity ccap, at
-// least min(C_tot, N)); returns their count.  Bucket ids go through an
-// LDS bitset window of W ids (the reference's boost::dynamic_bitset,
-// src/FalconnPP.cpp:552-611).
+// least min(C_tot, N)); returns their count.  Bucket ids go through the LDS
+// dedup area (the reference's boost::dynamic_bitset,
+// src/FalconnPP.cpp:552-611), whose first `dirty` words may hold anything on
+// entry (other phases reuse them); each path zeroes what it needs first and
+// updates dirty.
+//  * C_tot (ctot_q, duplicates included) at most a quarter of the area: one
+//    walk over the bucket ids through an open-addressing set of 2^lg >=
+//    2 C_tot words, first occurrences appended, then those words zeroed.
+//    Cost follows C_tot, not N.
 //  * N fits one window: ids are ORed in and the set bits compacted in id
-//    order, so cq is ascending.
+//    order, so cq is ascending; the area is left dirty.
 //  * Otherwise (W is then a power of two): two passes over the bucket ids
 //    count and then scatter them by window into part (capacity pcap, at
-//    least C_tot: duplicates included); each
-//    window's segment goes through a test-and-set and first occurrences are
-//    appended, so cq is ordered by window and unordered inside one.
+//    least C_tot: duplicates included); each window's segment goes through
+//    a test-and-set, first occurrences appended, and the words it touched
+//    are zeroed again.
 // Callers rank by (score, id), which does not depend on the order of cq.
 // wc holds window_counter_words(N, W) words, sh at least TB words.
 template <int TB>
 __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32_t* __restrict__ pr,
                                        const uint64_t* __restrict__ bucket_off, const int32_t* __restrict__ ids,
                                        uint32_t* bits, uint32_t* wc, uint32_t* sh, int32_t* __restrict__ cq,
-                                       int32_t* __restrict__ part, uint64_t ccap, uint64_t pcap, uint64_t* ts) {
+                                       int32_t* __restrict__ part, uint64_t ccap, uint64_t pcap, uint64_t* ts,
+                                       uint64_t ctot_q, uint32_t& dirty) {
     const int64_t W = a.window_bits;
+    const uint32_t area = (uint32_t)(W / 32);
+    // the largest power of two words in the area
+    const int area_lg = 31 - __clz((int)area);
+    if (ctot_q * 4 <= ((uint64_t)1 << area_lg)) {
+        int lg = 1;
+        while (((uint64_t)1 << lg) < 2 * ctot_q) ++lg;
+        const uint32_t words = 1u << lg;
+        clear_words<TB>(bits, min(dirty, words));
+        uint32_t* count = sh;
+        if (threadIdx.x == 0) *count = 0u;
+        __syncthreads();
+        for_each_bucket_id<TB, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+            wave_append(ok && hashset_insert(bits, lg, v), v, count, cq, ccap);
+        });
+        __syncthreads();
+        if (ts && threadIdx.x == 0) ts[1] = wall_clock64();
+        const uint32_t nc = *count;
+        clear_words<TB>(bits, words);
+        dirty = dirty > words ? dirty : 0u;
+        __syncthreads();
+        return nc;
+    }
     if (a.N <= W) {
         const int words = (int)(W / 32);
-        for (int i = threadIdx.x; i < words; i += TB) bits[i] = 0u;
+        clear_words<TB>(bits, min(dirty, (uint32_t)words));
         __syncthreads();
         // 24 ids per lane in flight: one step covers a group of buckets of
         // the usual sizes (the bucket filter keeps them near bucketLimit)
@@ -354,6 +419,7 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         });
         __syncthreads();
         if (ts && threadIdx.x == 0) ts[1] = wall_clock64();
+        dirty = max(dirty, (uint32_t)words);
         return compact_window<TB>(bits, words, 0, 0u, sh, cq, ccap);
     }
     const int nwin = (int)window_count(a.N, W);
@@ -366,7 +432,9 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         wfill[i] = 0u;
     }
     __syncthreads();
-    for_each_bucket_id<TB, 8>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+    // 24 ids per lane in flight (a bucket of the usual size in one round of
+    // loads), as in the other paths
+    for_each_bucket_id<TB, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
         if (ok) atomicAdd(&wcnt[v >> shift], 1u);
     });
     __syncthreads();
@@ -386,7 +454,7 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
             }
     }
     __syncthreads();
-    for_each_bucket_id<TB, 8>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+    for_each_bucket_id<TB, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
         if (ok) {
             const int w = v >> shift;
             const uint32_t pos = woff[w] + atomicAdd(&wfill[w], 1u);
@@ -395,6 +463,8 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
     });
     uint32_t* count = sh;  // appended ids
     if (threadIdx.x == 0) *count = 0u;
+    clear_words<TB>(bits, min(dirty, area));
+    dirty = 0u;
     __syncthreads();
     if (ts && threadIdx.x == 0) ts[1] = wall_clock64();
     for (int w = 0; w < nwin; ++w) {
@@ -402,12 +472,12 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         if (cnt == 0) continue;
         const int64_t w0 = (int64_t)w << shift;
         const int words = (int)((min(W, a.N - w0) + 31) / 32);
-        for (int i = threadIdx.x; i < words; i += TB) bits[i] = 0u;
-        __syncthreads();
         const uint32_t base = woff[w];
         constexpr int UNR = 8;
+        // the last round's ids stay in v: a window of one round is zeroed
+        // from them afterwards
+        int32_t v[UNR];
         for (uint32_t i0 = 0; i0 < cnt; i0 += TB * UNR) {
-            int32_t v[UNR];
 #pragma unroll
             for (int r = 0; r < UNR; ++r) {
                 const uint32_t i = i0 + r * TB + threadIdx.x;
@@ -416,6 +486,19 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
 #pragma unroll
             for (int r = 0; r < UNR; ++r)
                 wave_append(v[r] >= 0 && bitset_insert(bits, (int64_t)v[r] - w0), v[r], count, cq, ccap);
+        }
+        __syncthreads();
+        // zero what the window set: the words of its ids, or the whole window
+        // when that takes fewer stores
+        if (cnt > (uint32_t)words) {
+            clear_words<TB>(bits, (uint32_t)words);
+        } else if (cnt <= (uint32_t)(TB * UNR)) {
+#pragma unroll
+            for (int r = 0; r < UNR; ++r)
+                if (v[r] >= 0) bits[(v[r] - w0) >> 5] = 0u;
+        } else {
+            for (uint32_t i = threadIdx.x; i < cnt; i += TB)
+                if (base + i < pcap) bits[(part[base + i] - w0) >> 5] = 0u;
         }
         __syncthreads();
     }

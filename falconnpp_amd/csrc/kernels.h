@@ -160,6 +160,11 @@ int64_t probe_replay_work_bytes(const ProbeGeom& g);
 
 // One persistent gather launch: every workgroup takes queries from a work
 // queue (claim_query) until it is drained.
+// The bucket id array is allocated with IDS_PAD entries past its end: the
+// gather reads a bucket's ids four at a time and may run up to three past
+// the last id.
+constexpr int64_t IDS_PAD = 4;
+
 struct GatherArgs {
     int64_t N;
     int D;
@@ -186,6 +191,7 @@ struct GatherArgs {
     uint64_t slot_cand = 0, slot_part = 0;
     const uint64_t* seg_off = nullptr;
     uint64_t* ncand = nullptr;  // unique candidates per query
+    const uint64_t* ctot = nullptr;  // C_tot per query (sum of its probed bucket sizes)
 };
 // Query order sort keys: the best bucket of table 0 (queries with equal keys
 // share most candidates), or C_tot descending (longest first).

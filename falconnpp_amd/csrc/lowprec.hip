@@ -119,7 +119,7 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
                                                         const int32_t* __restrict__ ids, const float* __restrict__ Xc,
                                                         const float* __restrict__ Q, int64_t ldq,
                                                         int32_t* __restrict__ out_ids, float* __restrict__ out_scores,
-                                                        int q) {
+                                                        int q, uint32_t& dirty) {
     constexpr int NW = TB / WAVE;
     extern __shared__ uint32_t lds[];
     const int64_t W = a.window_bits;
@@ -143,6 +143,7 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
         replay_query<TB>(rp, (uint32_t)q, reinterpret_cast<unsigned char*>(lds),
                          reinterpret_cast<IFPair*>(rp.work + (int64_t)blockIdx.x * rp.wpq));
         __syncthreads();
+        dirty = (uint32_t)Wwords;
     }
     const QueryView qv_ = query_view(a.qp, q, a.qProbes, 0, 0, a.k);
     const int kq = qv_.k;
@@ -150,153 +151,163 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
     const uint32_t* pr = probes + qv_.probe_off;
     const QueryScratch qs = query_scratch(a, q);
     int32_t* cq = qs.cand;
-    const uint32_t nc =
-        collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, qs.part, qs.ccap, qs.pcap, ts);
+    const uint64_t ctot_q = a.ctot ? a.ctot[q] : ~uint64_t(0);
+    const uint32_t nc = collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, qs.part, qs.ccap,
+                                               qs.pcap, ts, ctot_q, dirty);
     if (threadIdx.x == 0) a.ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
 
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wv = threadIdx.x / WAVE;
-    const int g16 = lane >> 4;  // k-group of this lane in an MFMA operand
-    const int r16 = lane & 15;  // row (A) / column (B) of this lane
     const float* qf32 = Q + (int64_t)q * ldq;
-    // A candidate enters the MFMA as two half-rows: tile row 2c + h holds half
-    // h of candidate c's current 128-byte chunk and column n of the B operand
-    // holds the query's half n of that chunk (columns 2..15 are zero), so
-    // candidate c's score is the sum over chunks of C[2c][0] + C[2c+1][1].
-    // Lane (g16, r16) loads the 16 bytes at 64 h + 16 g16 of the chunk of
-    // candidate r16 / 2 (h = r16 % 2): one load instruction reads the whole
-    // 128-byte lines of 8 candidates.  Bytes past the row end re-read its last
-    // 32 bytes against a zero query (adding exactly 0).
-    constexpr int KC = lp_chunks(FMT, KT);
-    const int64_t rb = FMT == BF16 ? 2 * lp.ldl : lp.ldl;  // row bytes, a multiple of 32
-    // query in the scoring format, as the B operand
-    float qscale = 1.0f;
-    if constexpr (FMT == FP8) {
-        float m = 0.0f;
-        for (int d = threadIdx.x; d < a.D; d += TB) m = fmaxf(m, fabsf(qf32[d]));
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, WAVE));
-        if (lane == 0) red[wv] = __float_as_uint(m);
+    // the approximate pass only decides which candidates reach the exact
+    // rescoring: with no more candidates than that, all of them do
+    uint32_t rr;
+    if (nc <= (uint32_t)Rq) {
+        for (uint32_t i = threadIdx.x; i < nc; i += TB) rid[i] = cq[i];
+        rr = nc;
         __syncthreads();
-        m = 0.0f;
-        for (int w = 0; w < NW; ++w) m = fmaxf(m, __uint_as_float(red[w]));
-        qscale = m > 0.0f ? m / FP8_TARGET : 1.0f;
-        __syncthreads();
-    }
-    // B operand: lane (g16, n = r16) holds query elements 8 g16 .. 8 g16 + 7
-    // of half n of each chunk (FP8: two k-steps per chunk half, s = 0, 1)
-    bf16x8 qb[KC];
-    long q8[KC][2];
-    const float inv = 1.0f / qscale;
-#pragma unroll
-    for (int ch = 0; ch < KC; ++ch) {
-#pragma unroll
-        for (int s = 0; s < (FMT == BF16 ? 1 : 2); ++s) {
-            float v[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int d = FMT == BF16 ? 64 * ch + 32 * r16 + 8 * g16 + j
-                                          : 128 * ch + 64 * r16 + 16 * g16 + 8 * s + j;
-                v[j] = (r16 < 2 && d < a.D) ? qf32[d] : 0.0f;
-            }
-            if constexpr (FMT == BF16) {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) qb[ch][j] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(v[j]));
-            } else {
-                int lo = 0, hi = 0;
-                lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, lo, false);
-                lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, lo, true);
-                hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4] * inv, v[5] * inv, hi, false);
-                hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6] * inv, v[7] * inv, hi, true);
-                q8[ch][s] = (long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
-            }
+    } else {
+        const int lane = threadIdx.x & (WAVE - 1);
+        const int wv = threadIdx.x / WAVE;
+        const int g16 = lane >> 4;  // k-group of this lane in an MFMA operand
+        const int r16 = lane & 15;  // row (A) / column (B) of this lane
+        // A candidate enters the MFMA as two half-rows: tile row 2c + h holds half
+        // h of candidate c's current 128-byte chunk and column n of the B operand
+        // holds the query's half n of that chunk (columns 2..15 are zero), so
+        // candidate c's score is the sum over chunks of C[2c][0] + C[2c+1][1].
+        // Lane (g16, r16) loads the 16 bytes at 64 h + 16 g16 of the chunk of
+        // candidate r16 / 2 (h = r16 % 2): one load instruction reads the whole
+        // 128-byte lines of 8 candidates.  Bytes past the row end re-read its last
+        // 32 bytes against a zero query (adding exactly 0).
+        constexpr int KC = lp_chunks(FMT, KT);
+        const int64_t rb = FMT == BF16 ? 2 * lp.ldl : lp.ldl;  // row bytes, a multiple of 32
+        // query in the scoring format, as the B operand
+        float qscale = 1.0f;
+        if constexpr (FMT == FP8) {
+            float m = 0.0f;
+            for (int d = threadIdx.x; d < a.D; d += TB) m = fmaxf(m, fabsf(qf32[d]));
+            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, WAVE));
+            if (lane == 0) red[wv] = __float_as_uint(m);
+            __syncthreads();
+            m = 0.0f;
+            for (int w = 0; w < NW; ++w) m = fmaxf(m, __uint_as_float(red[w]));
+            qscale = m > 0.0f ? m / FP8_TARGET : 1.0f;
+            __syncthreads();
         }
-    }
-    // byte offset of this lane's 16 bytes in each chunk, clamped into the row
-    const int c8 = r16 >> 1;
-    int64_t boff[KC];
-#pragma unroll
-    for (int ch = 0; ch < KC; ++ch) {
-        const int64_t o = 128 * ch + 64 * (r16 & 1) + 16 * g16;
-        boff[ch] = o < rb ? o : rb - 32 + (o & 16);
-    }
-    const uint8_t* xbase = FMT == BF16 ? reinterpret_cast<const uint8_t*>(lp.xb) : lp.x8;
-    // approximate scores, NT tiles of 8 candidates per wave step (the ids of
-    // the next step fetched one step ahead), as sort keys by candidate
-    // position: in LDS over the dedup bitset when they fit, else in the
-    // query's scratch segment
-    const uint32_t lds_cap = (uint32_t)Wwords;
-    uint32_t* keys = nc <= lds_cap ? bits : reinterpret_cast<uint32_t*>(qs.part);
-    constexpr int NT = lp_tiles(FMT, KT);
-    constexpr uint32_t SPAN = 8 * NT;
-    const uint32_t stride = (uint32_t)NW * SPAN;
-    int32_t idn[NT];
-#pragma unroll
-    for (int h = 0; h < NT; ++h) {
-        const uint32_t i = (uint32_t)wv * SPAN + h * 8 + c8;
-        idn[h] = i < nc ? cq[i] : -1;
-    }
-    for (uint32_t t0 = (uint32_t)wv * SPAN; t0 < nc; t0 += stride) {
-        int32_t id[NT];
-        float rs[NT];
-#pragma unroll
-        for (int h = 0; h < NT; ++h) {
-            id[h] = idn[h];
-            const uint32_t i = t0 + stride + h * 8 + c8;
-            idn[h] = i < nc ? cq[i] : -1;
-            if constexpr (FMT == FP8) rs[h] = lp.uscale > 0.0f ? lp.uscale : (id[h] < 0 ? 0.0f : lp.scale[id[h]]);
-        }
-        f32x4 acc[NT];
-#pragma unroll
-        for (int h = 0; h < NT; ++h) acc[h] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // no branches: all KC * NT loads of a step are in flight before the
-        // first MFMA waits
-        int4 av[KC][NT];
-#pragma unroll
-        for (int ch = 0; ch < KC; ++ch)
-#pragma unroll
-            for (int h = 0; h < NT; ++h) {
-                const int64_t row = id[h] < 0 ? 0 : id[h];
-                av[ch][h] = *reinterpret_cast<const int4*>(xbase + row * rb + boff[ch]);
-            }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int ch = 0; ch < KC; ++ch)
-#pragma unroll
-            for (int h = 0; h < NT; ++h) {
+        // B operand: lane (g16, n = r16) holds query elements 8 g16 .. 8 g16 + 7
+        // of half n of each chunk (FP8: two k-steps per chunk half, s = 0, 1)
+        bf16x8 qb[KC];
+        long q8[KC][2];
+        const float inv = 1.0f / qscale;
+    #pragma unroll
+        for (int ch = 0; ch < KC; ++ch) {
+    #pragma unroll
+            for (int s = 0; s < (FMT == BF16 ? 1 : 2); ++s) {
+                float v[8];
+    #pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int d = FMT == BF16 ? 64 * ch + 32 * r16 + 8 * g16 + j
+                                              : 128 * ch + 64 * r16 + 16 * g16 + 8 * s + j;
+                    v[j] = (r16 < 2 && d < a.D) ? qf32[d] : 0.0f;
+                }
                 if constexpr (FMT == BF16) {
-                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[ch][h]), qb[ch],
-                                                                      acc[h], 0, 0, 0);
+    #pragma unroll
+                    for (int j = 0; j < 8; ++j) qb[ch][j] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(v[j]));
                 } else {
-                    const long lo = (long)(((uint64_t)(uint32_t)av[ch][h].y << 32) | (uint32_t)av[ch][h].x);
-                    const long hi = (long)(((uint64_t)(uint32_t)av[ch][h].w << 32) | (uint32_t)av[ch][h].z);
-                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(lo, q8[ch][0], acc[h], 0, 0, 0);
-                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(hi, q8[ch][1], acc[h], 0, 0, 0);
+                    int lo = 0, hi = 0;
+                    lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, lo, false);
+                    lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, lo, true);
+                    hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4] * inv, v[5] * inv, hi, false);
+                    hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6] * inv, v[7] * inv, hi, true);
+                    q8[ch][s] = (long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
                 }
             }
-        // accumulator i of lane (g16, n) holds C[4 g16 + i][n]: candidate
-        // 2 g16 is acc[0] of column 0 + acc[1] of column 1, candidate
-        // 2 g16 + 1 is acc[2] of column 0 + acc[3] of column 1; lane
-        // (g16, n < 2) stores candidate 2 g16 + n
-        const int c = 2 * g16 + (r16 & 1);
-#pragma unroll
-        for (int h = 0; h < NT; ++h) {
-            const float v0 = (r16 & 1) ? acc[h][1] : acc[h][0];
-            const float v1 = (r16 & 1) ? acc[h][3] : acc[h][2];
-            const float s0 = v0 + __shfl_xor(v0, 1, WAVE);
-            const float s1 = v1 + __shfl_xor(v1, 1, WAVE);
-            float sc = (r16 & 1) ? s1 : s0;
-            if constexpr (FMT == FP8) sc *= (lp.uscale > 0.0f ? lp.uscale : __shfl(rs[h], 2 * c, WAVE)) * qscale;
-            const uint32_t i = t0 + h * 8 + c;
-            if (r16 < 2 && i < nc) keys[i] = desc_key_any(sc);
         }
+        // byte offset of this lane's 16 bytes in each chunk, clamped into the row
+        const int c8 = r16 >> 1;
+        int64_t boff[KC];
+    #pragma unroll
+        for (int ch = 0; ch < KC; ++ch) {
+            const int64_t o = 128 * ch + 64 * (r16 & 1) + 16 * g16;
+            boff[ch] = o < rb ? o : rb - 32 + (o & 16);
+        }
+        const uint8_t* xbase = FMT == BF16 ? reinterpret_cast<const uint8_t*>(lp.xb) : lp.x8;
+        // approximate scores, NT tiles of 8 candidates per wave step (the ids of
+        // the next step fetched one step ahead), as sort keys by candidate
+        // position: in LDS over the dedup bitset when they fit, else in the
+        // query's scratch segment
+        const uint32_t lds_cap = (uint32_t)Wwords;
+        uint32_t* keys = nc <= lds_cap ? bits : reinterpret_cast<uint32_t*>(qs.part);
+        if (nc <= lds_cap) dirty = max(dirty, nc);
+        constexpr int NT = lp_tiles(FMT, KT);
+        constexpr uint32_t SPAN = 8 * NT;
+        const uint32_t stride = (uint32_t)NW * SPAN;
+        int32_t idn[NT];
+    #pragma unroll
+        for (int h = 0; h < NT; ++h) {
+            const uint32_t i = (uint32_t)wv * SPAN + h * 8 + c8;
+            idn[h] = i < nc ? cq[i] : -1;
+        }
+        for (uint32_t t0 = (uint32_t)wv * SPAN; t0 < nc; t0 += stride) {
+            int32_t id[NT];
+            float rs[NT];
+    #pragma unroll
+            for (int h = 0; h < NT; ++h) {
+                id[h] = idn[h];
+                const uint32_t i = t0 + stride + h * 8 + c8;
+                idn[h] = i < nc ? cq[i] : -1;
+                if constexpr (FMT == FP8) rs[h] = lp.uscale > 0.0f ? lp.uscale : (id[h] < 0 ? 0.0f : lp.scale[id[h]]);
+            }
+            f32x4 acc[NT];
+    #pragma unroll
+            for (int h = 0; h < NT; ++h) acc[h] = f32x4{0.f, 0.f, 0.f, 0.f};
+            // no branches: all KC * NT loads of a step are in flight before the
+            // first MFMA waits
+            int4 av[KC][NT];
+    #pragma unroll
+            for (int ch = 0; ch < KC; ++ch)
+    #pragma unroll
+                for (int h = 0; h < NT; ++h) {
+                    const int64_t row = id[h] < 0 ? 0 : id[h];
+                    av[ch][h] = *reinterpret_cast<const int4*>(xbase + row * rb + boff[ch]);
+                }
+            __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+            for (int ch = 0; ch < KC; ++ch)
+    #pragma unroll
+                for (int h = 0; h < NT; ++h) {
+                    if constexpr (FMT == BF16) {
+                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[ch][h]), qb[ch],
+                                                                          acc[h], 0, 0, 0);
+                    } else {
+                        const long lo = (long)(((uint64_t)(uint32_t)av[ch][h].y << 32) | (uint32_t)av[ch][h].x);
+                        const long hi = (long)(((uint64_t)(uint32_t)av[ch][h].w << 32) | (uint32_t)av[ch][h].z);
+                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(lo, q8[ch][0], acc[h], 0, 0, 0);
+                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(hi, q8[ch][1], acc[h], 0, 0, 0);
+                    }
+                }
+            // accumulator i of lane (g16, n) holds C[4 g16 + i][n]: candidate
+            // 2 g16 is acc[0] of column 0 + acc[1] of column 1, candidate
+            // 2 g16 + 1 is acc[2] of column 0 + acc[3] of column 1; lane
+            // (g16, n < 2) stores candidate 2 g16 + n
+            const int c = 2 * g16 + (r16 & 1);
+    #pragma unroll
+            for (int h = 0; h < NT; ++h) {
+                const float v0 = (r16 & 1) ? acc[h][1] : acc[h][0];
+                const float v1 = (r16 & 1) ? acc[h][3] : acc[h][2];
+                const float s0 = v0 + __shfl_xor(v0, 1, WAVE);
+                const float s1 = v1 + __shfl_xor(v1, 1, WAVE);
+                float sc = (r16 & 1) ? s1 : s0;
+                if constexpr (FMT == FP8) sc *= (lp.uscale > 0.0f ? lp.uscale : __shfl(rs[h], 2 * c, WAVE)) * qscale;
+                const uint32_t i = t0 + h * 8 + c;
+                if (r16 < 2 && i < nc) keys[i] = desc_key_any(sc);
+            }
+        }
+        __syncthreads();
+        // the R best approximate scores (ties: smaller id)
+        rr = nc <= lds_cap ? select_smallest<TB>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid)
+                           : select_smallest<TB>(reinterpret_cast<const uint32_t*>(qs.part), cq, nc, (uint32_t)Rq, hist,
+                                                 sh, &s_digit, &s_below, count, rid);
     }
-    __syncthreads();
-    // the R best approximate scores (ties: smaller id)
-    const uint32_t rr = nc <= lds_cap
-                            ? select_smallest<TB>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid)
-                            : select_smallest<TB>(reinterpret_cast<const uint32_t*>(qs.part), cq, nc,
-                                                  (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid);
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
     // exact FP32 rescoring of the survivors, 8 lanes per row
     const int g8 = threadIdx.x & 7;
@@ -352,10 +363,14 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(2, 2))) void
                                                         const float* __restrict__ Q, int64_t ldq,
                                                         int32_t* __restrict__ out_ids, float* __restrict__ out_scores) {
     __shared__ int s_q;
+    // leading words of the dedup area that may be nonzero (LDS starts
+    // undefined)
+    uint32_t dirty = (uint32_t)(a.window_bits / 32);
     for (;;) {
         const int q = claim_query(a, &s_q);
         if (q < 0) break;
-        k_gather_score_lp_one<TB, FMT, KT>(a, lp, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, q);
+        k_gather_score_lp_one<TB, FMT, KT>(a, lp, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, q,
+                                           dirty);
         __syncthreads();
     }
 }

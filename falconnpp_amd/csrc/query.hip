@@ -518,7 +518,7 @@ __device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, cons
                                                           const int32_t* __restrict__ ids,
                                                           const float* __restrict__ Xc, const float* __restrict__ Q,
                                                           int64_t ldq, int32_t* __restrict__ out_ids,
-                                                          float* __restrict__ out_scores, int q) {
+                                                          float* __restrict__ out_scores, int q, uint32_t& dirty) {
     extern __shared__ uint32_t lds[];
     const int64_t W = a.window_bits;
     const int Wwords = (int)(W / 32);
@@ -539,6 +539,7 @@ __device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, cons
         replay_query<TB>(rp, (uint32_t)q, reinterpret_cast<unsigned char*>(lds),
                          reinterpret_cast<IFPair*>(rp.work + (int64_t)blockIdx.x * rp.wpq));
         __syncthreads();
+        dirty = (uint32_t)Wwords;
     }
     const QueryView qv_ = query_view(a.qp, q, a.qProbes, 0, 0, a.k);
     const int kq = qv_.k;
@@ -548,8 +549,9 @@ __device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, cons
     const QueryScratch qs = query_scratch(a, q);
     int32_t* cq = qs.cand;
     float* sq = reinterpret_cast<float*>(qs.part);
-    const uint32_t nc =
-        collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, qs.part, qs.ccap, qs.pcap, ts);
+    const uint64_t ctot_q = a.ctot ? a.ctot[q] : ~uint64_t(0);
+    const uint32_t nc = collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, qs.part, qs.ccap,
+                                               qs.pcap, ts, ctot_q, dirty);
     if (threadIdx.x == 0) a.ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
     // FP32 scores: groups of 8 lanes per candidate row, CPL float4 chunks per
@@ -742,10 +744,13 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB <= 512 ? 
                                                           int64_t ldq, int32_t* __restrict__ out_ids,
                                                           float* __restrict__ out_scores) {
     __shared__ int s_q;
+    // leading words of the dedup area that may be nonzero (LDS starts
+    // undefined)
+    uint32_t dirty = (uint32_t)(a.window_bits / 32);
     for (;;) {
         const int q = claim_query(a, &s_q);
         if (q < 0) break;
-        k_gather_score_f32_one<TB, CPL>(a, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, q);
+        k_gather_score_f32_one<TB, CPL>(a, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, q, dirty);
         __syncthreads();
     }
 }
