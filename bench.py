@@ -212,6 +212,9 @@ def run_precisions(w, args, config_key, with_truth):
         refine = int(c.get("refine", 4 * w.k))
         for name, mode in PRECISIONS[1:]:
             w.idx.set_precision(mode, refine)
+            # the FP8 copy with one scale per row where the configuration
+            # asks for it (else one shared scale when the rows allow it)
+            w.idx.set_tuning(E.TUNE_FP8_PER_ROW, 1 if (mode == 2 and c.get("fp8_per_row")) else 0)
             vq, vms, _ = w.timed(args.steps, args.warmup)
             vst = w.idx.stats()
             shared = w.idx.fp8_shared_scale() if mode == 2 else 0.0

@@ -1786,6 +1786,10 @@ int fpp_set_tuning(fpp_index* h, int which, int value) {
             case FPP_TUNE_KEEP_CANDIDATES:
                 h->I.keep_candidates = value != 0;
                 break;
+            case FPP_TUNE_WORKSPACE_MB:
+                FPP_REQUIRE(value >= 1, FPP_EINVAL, "workspace bound must be at least 1 MiB");
+                h->I.ws_budget = size_t(value) << 20;
+                break;
             case FPP_TUNE_REPLAY:
                 FPP_REQUIRE(value >= 0 && value <= 2, FPP_EINVAL, "replay placement must be 0, 1 or 2");
                 h->I.replay_mode = value;

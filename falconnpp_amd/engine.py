@@ -30,6 +30,7 @@ PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
 TUNE_QUERY_ORDER, TUNE_GATHER_THREADS, TUNE_PERSISTENT, TUNE_GRAPH, TUNE_VERBOSE, TUNE_FP8_PER_ROW = 1, 2, 3, 4, 5, 6
 TUNE_KEEP_CANDIDATES = 7
 TUNE_REPLAY = 8  # 0 per call, 1 separate kernels, 2 inside the gather
+TUNE_WORKSPACE_MB = 9  # bound of the per-workgroup candidate scratch
 
 FPP_EINVAL, FPP_ESTATE, FPP_EHIP, FPP_ENOMEM, FPP_EINTERNAL, FPP_EIO = 1, 2, 3, 4, 5, 6
 

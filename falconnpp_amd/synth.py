@@ -106,7 +106,7 @@ CONFIGS = {
     # 6.4e9 (id, table) entries before the bucket filter; FP8 scoring keeps
     # 200 survivors for the FP32 refine.
     "c5": dict(n=50_000_000, d=96, kind="mixture", n_centres=20_000, nq=100_000, L=64, n_proj=128, iProbes=2,
-               alpha=0.01, bucketLimit=20, qProbes=4000, k=20, refine=200),
+               alpha=0.01, bucketLimit=20, qProbes=4000, k=20, refine=200, fp8_per_row=True),
 }
 
 

@@ -193,7 +193,10 @@ int fpp_get_fp8_scale(fpp_index* h, float* shared);
  * probe replay of the tie-flagged queries runs, 0 = chosen per call (default),
  * 1 = separate kernels before the gather (each (query, table) on a wave of its
  * own across the device), 2 = inside the gather, by the workgroup that claims
- * the query, before it gathers it (flagged queries head the queue). */
+ * the query, before it gathers it (flagged queries head the queue).
+ * FPP_TUNE_WORKSPACE_MB: bound, in MiB, of the per-workgroup candidate
+ * scratch of a query call (default 8192); a tighter bound runs the gather on
+ * fewer workgroups, with the same results. */
 enum {
     FPP_TUNE_QUERY_ORDER = 1,
     FPP_TUNE_GATHER_THREADS = 2,
@@ -202,7 +205,8 @@ enum {
     FPP_TUNE_VERBOSE = 5,
     FPP_TUNE_FP8_PER_ROW = 6,
     FPP_TUNE_KEEP_CANDIDATES = 7,
-    FPP_TUNE_REPLAY = 8
+    FPP_TUNE_REPLAY = 8,
+    FPP_TUNE_WORKSPACE_MB = 9
 };
 int fpp_set_tuning(fpp_index* h, int which, int value);
 /* Autotuner (no reference counterpart): times the gather workgroup sizes,

@@ -588,6 +588,30 @@ def test_kept_candidates_and_graph_do_not_change_results(precision):
         g.set_precision(0, 0)
 
 
+def test_workspace_bound_does_not_change_results():
+    """A tight bound on the per-workgroup candidate scratch runs the gather on
+    fewer workgroups (one or two at 1 MiB); the answers stay the same, for a
+    plain call and a mixed batch, in FP32 and FP8."""
+    X, Q, g, qps, ks = _batch_case(seed=17)
+    kmax = int(ks.max())
+    g.set_tuning(E.TUNE_KEEP_CANDIDATES, 0)  # kept lists bypass the bound
+    try:
+        for prec in (E.PREC_FP32, E.PREC_FP8):
+            g.set_precision(prec, 0)
+            g.set_tuning(E.TUNE_WORKSPACE_MB, 8192)
+            ref = _run_batch(g, Q, qps, ks, kmax)
+            ref1 = _run_single(g, Q, 400, 10)
+            g.set_tuning(E.TUNE_WORKSPACE_MB, 1)
+            out = _run_batch(g, Q, qps, ks, kmax)
+            out1 = _run_single(g, Q, 400, 10)
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), prec
+            assert np.array_equal(out1[0], ref1[0]) and np.array_equal(out1[1], ref1[1]), prec
+    finally:
+        g.set_tuning(E.TUNE_WORKSPACE_MB, 8192)
+        g.set_precision(E.PREC_FP32, 0)
+        g.set_tuning(E.TUNE_KEEP_CANDIDATES, 1)
+
+
 def test_order_modes_and_workgroup_sizes_do_not_change_results():
     X, Q, g, qps, ks = _batch_case(seed=13)
     kmax = int(ks.max())
