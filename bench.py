@@ -171,13 +171,21 @@ class Workload:
     def recall(self):
         return round(float(self.E.recall(self.truth, self.answer(), self.k)), 4) if self.truth is not None else None
 
+    def mark_fp32_workspace(self):
+        """Device memory taken by the FP32 calls (the query workspace and the
+        output arrays), before any reduced-precision copy exists."""
+        self.free_fp32 = self.torch.cuda.mem_get_info()[0]
+
     def memory(self):
         free_end = self.torch.cuda.mem_get_info()[0]
-        return {"device_total_gb": round(self.total_mem / 1e9, 1),
-                "data_and_queries_gb": round((self.X.nbytes + self.Q.nbytes) / 1e9, 2),
-                "index_gb": round((self.free0 - self.free_built) / 1e9, 2),
-                "query_workspace_and_copies_gb": round((self.free_built - free_end) / 1e9, 2),
-                "used_at_end_gb": round((self.total_mem - free_end) / 1e9, 2)}
+        m = {"device_total_gb": round(self.total_mem / 1e9, 1),
+             "data_and_queries_gb": round((self.X.nbytes + self.Q.nbytes) / 1e9, 2),
+             "index_gb": round((self.free0 - self.free_built) / 1e9, 2)}
+        if getattr(self, "free_fp32", None) is not None:
+            m["query_workspace_fp32_gb"] = round((self.free_built - self.free_fp32) / 1e9, 2)
+        m["query_workspace_and_copies_gb"] = round((self.free_built - free_end) / 1e9, 2)
+        m["used_at_end_gb"] = round((self.total_mem - free_end) / 1e9, 2)
+        return m
 
     def release(self):
         self.idx.close()
@@ -194,6 +202,7 @@ def run_precisions(w, args, config_key, with_truth):
     E, c = w.E, w.cfg
     w.idx.set_qProbes(c["qProbes"])
     qps, ms, stage = w.timed(args.steps, args.warmup, events=True)
+    w.mark_fp32_workspace()
     st = w.idx.stats()
     log(f"{w.name} fp32: {ms:.2f} ms/step, {qps:.0f} QPS, C_tot/q {st['ctot'] / w.nq:.0f}, "
         f"C_uniq/q {st['cuniq'] / w.nq:.0f}, tie-replayed queries {st['query_ties']}")
