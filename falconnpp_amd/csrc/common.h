@@ -61,6 +61,27 @@ __device__ __forceinline__ int lane_xor(int v, int m) {
     return __shfl_xor(v, m, WAVE);
 }
 
+// Global-memory loads and stores through address space 1 (global_load /
+// global_store).  Pointers that reach a device function through structures
+// or selects stay generic, and the compiler then emits flat accesses, which
+// count on the LDS counter as well: a flat load issued as a prefetch is then
+// waited for by the next wait on any earlier load or LDS access, which
+// serialises the prefetch.  p must point into global memory.
+#define FPP_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const FPP_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+    *(FPP_GLOBAL T*)p = v;
+}
+// Four consecutive int32 at a 4-byte aligned address, one load.
+typedef int32_t i32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ i32x4_a4 gld_i32x4(const int32_t* p) {
+    return *(const FPP_GLOBAL i32x4_a4*)p;
+}
+
 inline int ilog2_exact(int64_t x) {
     int l = 0;
     while ((int64_t(1) << l) < x) ++l;

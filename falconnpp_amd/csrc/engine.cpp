@@ -623,7 +623,7 @@ QueryLayout query_layout(int nq, int L, int PL, uint64_t nprobes, bool batched, 
     w.qlist = take(n * 4);
     w.counters = take(N_COUNTERS * 4);
     w.rwork = take((size_t)rareas * (size_t)wpq);
-    w.phase_ts = take(stamps ? n * 5 * 8 : 0);
+    w.phase_ts = take(stamps ? n * PHASE_STAMPS * 8 : 0);
     w.cand = take((size_t)slots * slot_cand * 4);
     w.part = take((size_t)slots * slot_part * 4);
     w.total = (off + 255) & ~size_t(255);
@@ -975,7 +975,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
             FPP_HIP(hipDeviceGetAttribute(&I.wall_khz, hipDeviceAttributeWallClockRate, dev));
         }
         std::vector<uint64_t>& ts = I.last_ts;
-        ts.resize((size_t)nq * 5);
+        ts.resize((size_t)nq * PHASE_STAMPS);
         FPP_HIP(hipMemcpyAsync(ts.data(), ga.phase_ts, ts.size() * 8, hipMemcpyDeviceToHost, st));
         FPP_HIP(hipStreamSynchronize(st));
         double acc[4] = {0, 0, 0, 0};
@@ -984,7 +984,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         fl.resize(nq);
         FPP_HIP(hipMemcpy(fl.data(), qflag, (size_t)nq * 4, hipMemcpyDeviceToHost));
         for (int q = 0; q < nq; ++q) {
-            const uint64_t* t = &ts[(size_t)q * 5];
+            const uint64_t* t = &ts[(size_t)q * PHASE_STAMPS];
             for (int i = 0; i < 4; ++i) acc[i] += (double)(t[i + 1] - t[i]);
             longest = std::max(longest, t[4] - t[0]);
             if (fl[q]) longest_flagged = std::max(longest_flagged, t[1] - t[0]);
@@ -1814,15 +1814,16 @@ int fpp_get_query_times(fpp_index* h, double* service_us, double* done_us) {
     return guard([&] {
         FPP_REQUIRE(h && service_us && done_us, FPP_EINVAL, "null argument");
         Index& I = h->I;
-        const size_t nq = I.last_ts.size() / 5;
+        const size_t nq = I.last_ts.size() / PHASE_STAMPS;
         FPP_REQUIRE(nq > 0 && nq == (size_t)I.last.nq, FPP_ESTATE,
                     "no per-query stamps: make the last query call with profiling level 2");
         uint64_t t0 = ~0ull;
-        for (size_t q = 0; q < nq; ++q) t0 = std::min(t0, I.last_ts[q * 5]);
+        for (size_t q = 0; q < nq; ++q) t0 = std::min(t0, I.last_ts[q * PHASE_STAMPS]);
         const double us = 1000.0 / std::max(1, I.wall_khz);
         for (size_t q = 0; q < nq; ++q) {
-            service_us[q] = (double)(I.last_ts[q * 5 + 4] - I.last_ts[q * 5]) * us;
-            done_us[q] = (double)(I.last_ts[q * 5 + 4] - t0) * us;
+            const uint64_t* t = &I.last_ts[q * PHASE_STAMPS];
+            service_us[q] = (double)(t[4] - t[0]) * us;
+            done_us[q] = (double)(t[4] - t0) * us;
         }
     });
 }

@@ -10,45 +10,59 @@
 namespace fpp {
 namespace gather {
 
+// ---- thread groups ---------------------------------------------------------
+// The group-level helpers below run on a group of threads G: G::size threads,
+// G::tid() in [0, size), G::sync() a barrier of the group that also orders
+// its LDS and global memory accesses.  The gather kernels use the whole
+// workgroup.
+template <int TB>
+struct Whole {
+    static constexpr int size = TB;
+    __device__ static __forceinline__ int tid() { return (int)threadIdx.x; }
+    __device__ static __forceinline__ void sync() { __syncthreads(); }
+};
+
 // ---- block / wave helpers -------------------------------------------------
-template <int T>
+template <class G>
 __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* sh, uint32_t& total) {
-    const int t = threadIdx.x;
+    constexpr int T = G::size;
+    const int t = G::tid();
     sh[t] = v;
-    __syncthreads();
+    G::sync();
     for (int off = 1; off < T; off <<= 1) {
         const uint32_t a = t >= off ? sh[t - off] : 0u;
-        __syncthreads();
+        G::sync();
         sh[t] += a;
-        __syncthreads();
+        G::sync();
     }
     total = sh[T - 1];
     const uint32_t incl = sh[t];
-    __syncthreads();
+    G::sync();
     return incl - v;
 }
 
 // Digit holding the k-th smallest key (1-based) of a 256-bin histogram.
-template <int T>
+template <class G>
 __device__ __forceinline__ void block_find_digit(const uint32_t* hist, uint32_t k, uint32_t* sh, int& digit,
                                                  uint32_t& below, uint32_t& eq, int* s_digit, uint32_t* s_below) {
-    static_assert(T >= 256, "needs >= 256 threads");
-    if (threadIdx.x == 0) {
+    static_assert(G::size >= 256, "needs >= 256 threads");
+    const int t = G::tid();
+    if (t == 0) {
         *s_digit = 255;
         *s_below = 0;
     }
-    const uint32_t h = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
+    const uint32_t h = t < 256 ? hist[t] : 0u;
     uint32_t total;
-    const uint32_t ex = block_excl_scan_u32<T>(h, sh, total);
-    if (threadIdx.x < 256 && ex < k && k <= ex + h) {
-        *s_digit = threadIdx.x;
+    const uint32_t ex = block_excl_scan_u32<G>(h, sh, total);
+    if (t < 256 && ex < k && k <= ex + h) {
+        *s_digit = t;
         *s_below = ex;
     }
-    __syncthreads();
+    G::sync();
     digit = *s_digit;
     below = *s_below;
     eq = hist[digit];
-    __syncthreads();
+    G::sync();
 }
 
 // Wave variant: lane t owns bins 4t..4t+3.
@@ -117,11 +131,10 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 //    lane per id; a binary search over the wave's prefix sums of bucket
 //    lengths maps a position to its probe (six dependent lane shuffles per
 //    id, so only where lane per bucket would leave most lanes idle).
-typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 
-template <int TB, int ULANE>
+template <class G, int ULANE>
 struct BucketWalk {
-    static constexpr int NW = TB / WAVE;
+    static constexpr int NW = G::size / WAVE;
     static constexpr int UNR = 8;  // flat walk: ids per lane in flight
     static constexpr uint32_t LANE_MAX = 64;
     const uint32_t* __restrict__ pr;
@@ -138,13 +151,13 @@ struct BucketWalk {
         ids = ids_;
         np = np_;
         const int lane = threadIdx.x & (WAVE - 1);
-        c0 = (int)(threadIdx.x / WAVE) * WAVE;
+        c0 = (G::tid() / WAVE) * WAVE;
         s = 0;
         len = 0;
         if (c0 + lane < np) {
-            const uint32_t b = pr[c0 + lane];
-            s = bucket_off[b];
-            len = (uint32_t)(bucket_off[b + 1] - s);
+            const uint32_t b = gld(pr + c0 + lane);
+            s = gld(bucket_off + b);
+            len = (uint32_t)(gld(bucket_off + b + 1) - s);
         }
     }
     // wave-uniform
@@ -154,7 +167,7 @@ struct BucketWalk {
     __device__ __forceinline__ void step(F&& f) {
         const int lane = threadIdx.x & (WAVE - 1);
         const int cn = c0 + NW * WAVE;
-        const uint32_t nb = cn + lane < np ? pr[cn + lane] : 0u;
+        const uint32_t nb = cn + lane < np ? gld(pr + cn + lane) : 0u;
         uint32_t mx = len;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, WAVE));
@@ -168,8 +181,8 @@ struct BucketWalk {
                 int32_t v[ULANE];
 #pragma unroll
                 for (int r = 0; r < ULANE; r += 4) {
-                    i32x4 t = {0, 0, 0, 0};
-                    if (j0 + r < len) __builtin_memcpy(&t, ids + s + j0 + r, sizeof(t));
+                    i32x4_a4 t = {0, 0, 0, 0};
+                    if (j0 + r < len) t = gld_i32x4(ids + s + j0 + r);
                     v[r] = t[0];
                     v[r + 1] = t[1];
                     v[r + 2] = t[2];
@@ -193,7 +206,7 @@ struct BucketWalk {
                     for (int st = WAVE / 2; st > 0; st >>= 1)
                         if (__shfl(incl, lo + st - 1, WAVE) <= fp) lo += st;
                     const uint64_t base = __shfl(sx, lo, WAVE);
-                    v[r] = fp < total ? ids[base + fp] : 0;
+                    v[r] = fp < total ? gld(ids + base + fp) : 0;
                 }
 #pragma unroll
                 for (int r = 0; r < UNR; ++r) f(v[r], f0 + r * WAVE + lane < total);
@@ -203,18 +216,18 @@ struct BucketWalk {
         s = 0;
         len = 0;
         if (cn + lane < np) {
-            s = bucket_off[nb];
-            len = (uint32_t)(bucket_off[nb + 1] - s);
+            s = gld(bucket_off + nb);
+            len = (uint32_t)(gld(bucket_off + nb + 1) - s);
         }
     }
 };
 
 // Every id of the probed buckets pr[0 .. np) through f (BucketWalk, all groups).
-template <int TB, int ULANE, class F>
+template <class G, int ULANE, class F>
 __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ pr,
                                                    const uint64_t* __restrict__ bucket_off,
                                                    const int32_t* __restrict__ ids, int np, F&& f) {
-    BucketWalk<TB, ULANE> w;
+    BucketWalk<G, ULANE> w;
     w.init(pr, bucket_off, ids, np);
     while (!w.done()) w.step(f);
 }
@@ -224,23 +237,26 @@ __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ 
 // into nqueues runs of consecutive positions; workgroup b takes from run
 // b % nqueues first (with 8 runs, the XCD the workgroup sits on), then from
 // the others.  Every successful claim consumes an item, so each workgroup
-// leaves the loop once the items are gone.  Thread 0 claims; s_q is a
-// __shared__ word.
-__device__ __forceinline__ int claim_query(const GatherArgs& a, int* s_q) {
-    if (threadIdx.x == 0) {
-        int q = -1;
-        const uint32_t R = (uint32_t)a.nqueues;
-        const uint32_t home = blockIdx.x % R;
-        for (uint32_t j = 0; j < R && q < 0; ++j) {
-            const uint32_t r = (home + j) % R;
-            const uint32_t lo = (uint32_t)((uint64_t)a.nitems * r / R);
-            const uint32_t hi = (uint32_t)((uint64_t)a.nitems * (r + 1) / R);
-            if (lo + __hip_atomic_load(&a.heads[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= hi) continue;
-            const uint32_t i = atomicAdd(&a.heads[r], 1u);
-            if (lo + i < hi) q = a.order ? (int)a.order[lo + i] : (int)(lo + i);
-        }
-        *s_q = q;
+// leaves the loop once the items are gone.  claim_item is the claim itself,
+// made by one thread; claim_query makes it with thread 0 and hands the result
+// to the whole workgroup through s_q, a __shared__ word.
+__device__ __forceinline__ int claim_item(const GatherArgs& a) {
+    int q = -1;
+    const uint32_t R = (uint32_t)a.nqueues;
+    const uint32_t home = blockIdx.x % R;
+    for (uint32_t j = 0; j < R && q < 0; ++j) {
+        const uint32_t r = (home + j) % R;
+        const uint32_t lo = (uint32_t)((uint64_t)a.nitems * r / R);
+        const uint32_t hi = (uint32_t)((uint64_t)a.nitems * (r + 1) / R);
+        if (lo + __hip_atomic_load(&a.heads[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= hi) continue;
+        const uint32_t i = atomicAdd(&a.heads[r], 1u);
+        if (lo + i < hi) q = a.order ? (int)a.order[lo + i] : (int)(lo + i);
     }
+    return q;
+}
+
+__device__ __forceinline__ int claim_query(const GatherArgs& a, int* s_q) {
+    if (threadIdx.x == 0) *s_q = claim_item(a);
     __syncthreads();
     const int q = __builtin_amdgcn_readfirstlane(*s_q);
     __syncthreads();
@@ -280,22 +296,22 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
 // cq[nc ...]: thread t owns the contiguous run of r words from t * r (r odd,
 // so at every step the lanes of a wave read 32 different LDS banks).  It
 // counts its bits; a wave scan and the waves' totals give its first output
-// position, and it writes its ids in order.  red holds TB / 64 words.
+// position, and it writes its ids in order.  red holds G::size / 64 words.
 // Returns the new count.
-template <int TB>
+template <class G>
 __device__ __forceinline__ uint32_t compact_window(const uint32_t* bits, int words, int64_t w0, uint32_t nc,
                                                    uint32_t* red, int32_t* __restrict__ cq, uint64_t ccap) {
-    constexpr int NW = TB / WAVE;
+    constexpr int NW = G::size / WAVE;
     const int lane = threadIdx.x & (WAVE - 1);
-    const int wv = threadIdx.x / WAVE;
-    const int r = (int)ceil_div(words, TB) | 1;
-    const int b = min(words, (int)threadIdx.x * r);
+    const int wv = G::tid() / WAVE;
+    const int r = (int)ceil_div(words, G::size) | 1;
+    const int b = min(words, G::tid() * r);
     const int e = min(words, b + r);
     uint32_t c = 0;
     for (int i = b; i < e; ++i) c += __popc(bits[i]);
     const uint32_t incl = wave_incl_scan(c);
     if (lane == WAVE - 1) red[wv] = incl;
-    __syncthreads();
+    G::sync();
     uint32_t base = nc + incl - c, tot = 0;
 #pragma unroll
     for (int v = 0; v < NW; ++v) {
@@ -308,16 +324,18 @@ __device__ __forceinline__ uint32_t compact_window(const uint32_t* bits, int wor
         while (word) {
             const int bit = __ffs(word) - 1;
             word &= word - 1;
-            if (base < ccap) cq[base] = (int32_t)(w0 + (int64_t)i * 32 + bit);
+            if (base < ccap) gst(cq + base, (int32_t)(w0 + (int64_t)i * 32 + bit));
             base++;
         }
     }
-    __syncthreads();
+    G::sync();
     return nc + tot;
 }
 
 // Appends v of the lanes with take set to cq at *count (LDS), one atomic per
-// wave.  Must be reached by the whole wave.
+// wave.  Must be reached by the whole wave.  GLOBAL: cq is in global memory
+// (stored with global_store); otherwise cq may be anywhere (LDS included).
+template <bool GLOBAL>
 __device__ __forceinline__ void wave_append(bool take, int32_t v, uint32_t* count, int32_t* __restrict__ cq,
                                             uint64_t ccap) {
     const uint64_t m = __ballot(take);
@@ -328,7 +346,12 @@ __device__ __forceinline__ void wave_append(bool take, int32_t v, uint32_t* coun
     if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
     base = __shfl(base, leader, WAVE);
     const uint32_t r = lane_rank(m);
-    if (take && base + r < ccap) cq[base + r] = v;
+    if (take && base + r < ccap) {
+        if constexpr (GLOBAL)
+            gst(cq + base + r, v);
+        else
+            cq[base + r] = v;
+    }
 }
 
 // Test-and-set of id rel in the LDS bitset: true when the bit was clear.
@@ -355,9 +378,9 @@ __device__ __forceinline__ bool hashset_insert(uint32_t* T, int lg, int32_t v) {
 }
 
 // Zeroes words [0, n) of the dedup area.
-template <int TB>
+template <class G>
 __device__ __forceinline__ void clear_words(uint32_t* bits, uint32_t n) {
-    for (uint32_t i = threadIdx.x; i < n; i += TB) bits[i] = 0u;
+    for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) bits[i] = 0u;
 }
 
 // Collect the unique candidates of one query into cq (capacity ccap, at
@@ -378,8 +401,8 @@ __device__ __forceinline__ void clear_words(uint32_t* bits, uint32_t n) {
 //    a test-and-set, first occurrences appended, and the words it touched
 //    are zeroed again.
 // Callers rank by (score, id), which does not depend on the order of cq.
-// wc holds window_counter_words(N, W) words, sh at least TB words.
-template <int TB>
+// wc holds window_counter_words(N, W) words, sh at least G::size words.
+template <class G>
 __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32_t* __restrict__ pr,
                                        const uint64_t* __restrict__ bucket_off, const int32_t* __restrict__ ids,
                                        uint32_t* bits, uint32_t* wc, uint32_t* sh, int32_t* __restrict__ cq,
@@ -393,80 +416,80 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         int lg = 1;
         while (((uint64_t)1 << lg) < 2 * ctot_q) ++lg;
         const uint32_t words = 1u << lg;
-        clear_words<TB>(bits, min(dirty, words));
+        clear_words<G>(bits, min(dirty, words));
         uint32_t* count = sh;
-        if (threadIdx.x == 0) *count = 0u;
-        __syncthreads();
-        for_each_bucket_id<TB, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
-            wave_append(ok && hashset_insert(bits, lg, v), v, count, cq, ccap);
+        if (G::tid() == 0) *count = 0u;
+        G::sync();
+        for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+            wave_append<true>(ok && hashset_insert(bits, lg, v), v, count, cq, ccap);
         });
-        __syncthreads();
-        if (ts && threadIdx.x == 0) ts[1] = wall_clock64();
+        G::sync();
+        if (ts && G::tid() == 0) ts[1] = wall_clock64();
         const uint32_t nc = *count;
-        clear_words<TB>(bits, words);
+        clear_words<G>(bits, words);
         dirty = dirty > words ? dirty : 0u;
-        __syncthreads();
+        G::sync();
         return nc;
     }
     if (a.N <= W) {
         const int words = (int)(W / 32);
-        clear_words<TB>(bits, min(dirty, (uint32_t)words));
-        __syncthreads();
+        clear_words<G>(bits, min(dirty, (uint32_t)words));
+        G::sync();
         // 24 ids per lane in flight: one step covers a group of buckets of
         // the usual sizes (the bucket filter keeps them near bucketLimit)
-        for_each_bucket_id<TB, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+        for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
             if (ok) atomicOr(&bits[v >> 5], 1u << (v & 31));
         });
-        __syncthreads();
-        if (ts && threadIdx.x == 0) ts[1] = wall_clock64();
+        G::sync();
+        if (ts && G::tid() == 0) ts[1] = wall_clock64();
         dirty = max(dirty, (uint32_t)words);
-        return compact_window<TB>(bits, words, 0, 0u, sh, cq, ccap);
+        return compact_window<G>(bits, words, 0, 0u, sh, cq, ccap);
     }
     const int nwin = (int)window_count(a.N, W);
     const int shift = __ffsll((long long)W) - 1;
     uint32_t* wcnt = wc;
     uint32_t* woff = wc + nwin;
     uint32_t* wfill = woff + nwin;
-    for (int i = threadIdx.x; i < nwin; i += TB) {
+    for (int i = G::tid(); i < nwin; i += G::size) {
         wcnt[i] = 0u;
         wfill[i] = 0u;
     }
-    __syncthreads();
+    G::sync();
     // 24 ids per lane in flight (a bucket of the usual size in one round of
     // loads), as in the other paths
-    for_each_bucket_id<TB, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+    for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
         if (ok) atomicAdd(&wcnt[v >> shift], 1u);
     });
-    __syncthreads();
+    G::sync();
     {
         // exclusive scan of the window counts: thread t sums a contiguous run
-        const int per = (nwin + TB - 1) / TB;
-        const int b0 = threadIdx.x * per;
+        const int per = (nwin + G::size - 1) / G::size;
+        const int b0 = G::tid() * per;
         uint32_t c = 0;
         for (int i = 0; i < per; ++i)
             if (b0 + i < nwin) c += wcnt[b0 + i];
         uint32_t tot;
-        uint32_t run = block_excl_scan_u32<TB>(c, sh, tot);
+        uint32_t run = block_excl_scan_u32<G>(c, sh, tot);
         for (int i = 0; i < per; ++i)
             if (b0 + i < nwin) {
                 woff[b0 + i] = run;
                 run += wcnt[b0 + i];
             }
     }
-    __syncthreads();
-    for_each_bucket_id<TB, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+    G::sync();
+    for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
         if (ok) {
             const int w = v >> shift;
             const uint32_t pos = woff[w] + atomicAdd(&wfill[w], 1u);
-            if (pos < pcap) part[pos] = v;
+            if (pos < pcap) gst(part + pos, v);
         }
     });
     uint32_t* count = sh;  // appended ids
-    if (threadIdx.x == 0) *count = 0u;
-    clear_words<TB>(bits, min(dirty, area));
+    if (G::tid() == 0) *count = 0u;
+    clear_words<G>(bits, min(dirty, area));
     dirty = 0u;
-    __syncthreads();
-    if (ts && threadIdx.x == 0) ts[1] = wall_clock64();
+    G::sync();
+    if (ts && G::tid() == 0) ts[1] = wall_clock64();
     for (int w = 0; w < nwin; ++w) {
         const uint32_t cnt = wcnt[w];
         if (cnt == 0) continue;
@@ -477,33 +500,33 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         // the last round's ids stay in v: a window of one round is zeroed
         // from them afterwards
         int32_t v[UNR];
-        for (uint32_t i0 = 0; i0 < cnt; i0 += TB * UNR) {
+        for (uint32_t i0 = 0; i0 < cnt; i0 += G::size * UNR) {
 #pragma unroll
             for (int r = 0; r < UNR; ++r) {
-                const uint32_t i = i0 + r * TB + threadIdx.x;
-                v[r] = (i < cnt && base + i < pcap) ? part[base + i] : -1;
+                const uint32_t i = i0 + r * G::size + G::tid();
+                v[r] = (i < cnt && base + i < pcap) ? gld(part + base + i) : -1;
             }
 #pragma unroll
             for (int r = 0; r < UNR; ++r)
-                wave_append(v[r] >= 0 && bitset_insert(bits, (int64_t)v[r] - w0), v[r], count, cq, ccap);
+                wave_append<true>(v[r] >= 0 && bitset_insert(bits, (int64_t)v[r] - w0), v[r], count, cq, ccap);
         }
-        __syncthreads();
+        G::sync();
         // zero what the window set: the words of its ids, or the whole window
         // when that takes fewer stores
         if (cnt > (uint32_t)words) {
-            clear_words<TB>(bits, (uint32_t)words);
-        } else if (cnt <= (uint32_t)(TB * UNR)) {
+            clear_words<G>(bits, (uint32_t)words);
+        } else if (cnt <= (uint32_t)(G::size * UNR)) {
 #pragma unroll
             for (int r = 0; r < UNR; ++r)
                 if (v[r] >= 0) bits[(v[r] - w0) >> 5] = 0u;
         } else {
-            for (uint32_t i = threadIdx.x; i < cnt; i += TB)
-                if (base + i < pcap) bits[(part[base + i] - w0) >> 5] = 0u;
+            for (uint32_t i = (uint32_t)G::tid(); i < cnt; i += G::size)
+                if (base + i < pcap) bits[(gld(part + base + i) - w0) >> 5] = 0u;
         }
-        __syncthreads();
+        G::sync();
     }
     const uint32_t nc = *count;
-    __syncthreads();
+    G::sync();
     return nc;
 }
 
@@ -511,9 +534,9 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
 // of position i is cq[i] (ids unique), and writes their ids to out_id in
 // unspecified order; returns min(want, n).  Radix select over the 32-bit key;
 // when more entries share the boundary key than are needed, a second radix
-// select over their ids keeps the smallest.  hist: 256 words, sh: TB words,
+// select over their ids keeps the smallest.  hist: 256 words, sh: G::size words,
 // count: one word.
-template <int TB>
+template <class G>
 __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restrict__ cq, uint32_t n, uint32_t want,
                                     uint32_t* hist, uint32_t* sh, int* s_digit, uint32_t* s_below, uint32_t* count,
                                     int32_t* out_id) {
@@ -521,16 +544,16 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
     if (kk == 0) return 0;
     uint32_t prefix = 0, pmask = 0, k = kk, eqc = 0;
     for (int shift = 24; shift >= 0; shift -= 8) {
-        for (int i = threadIdx.x; i < 256; i += TB) hist[i] = 0;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += TB) {
+        for (int i = G::tid(); i < 256; i += G::size) hist[i] = 0;
+        G::sync();
+        for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) {
             const uint32_t key = keys[i];
             if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
         }
-        __syncthreads();
+        G::sync();
         int d;
         uint32_t below;
-        block_find_digit<TB>(hist, k, sh, d, below, eqc, s_digit, s_below);
+        block_find_digit<G>(hist, k, sh, d, below, eqc, s_digit, s_below);
         prefix |= (uint32_t)d << shift;
         pmask |= 255u << shift;
         k -= below;
@@ -540,38 +563,38 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
     if (eqc > k) {
         uint32_t ip = 0, im = 0, kr = k;
         for (int shift = 24; shift >= 0; shift -= 8) {
-            for (int i = threadIdx.x; i < 256; i += TB) hist[i] = 0;
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < n; i += TB) {
-                const uint32_t id = (uint32_t)cq[i];
+            for (int i = G::tid(); i < 256; i += G::size) hist[i] = 0;
+            G::sync();
+            for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) {
+                const uint32_t id = (uint32_t)gld(cq + i);
                 if (keys[i] == T && (id & im) == ip) atomicAdd(&hist[(id >> shift) & 255u], 1u);
             }
-            __syncthreads();
+            G::sync();
             int d;
             uint32_t below, eqd;
-            block_find_digit<TB>(hist, kr, sh, d, below, eqd, s_digit, s_below);
+            block_find_digit<G>(hist, kr, sh, d, below, eqd, s_digit, s_below);
             ip |= (uint32_t)d << shift;
             im |= 255u << shift;
             kr -= below;
         }
         IT = ip;
     }
-    if (threadIdx.x == 0) *count = 0u;
-    __syncthreads();
-    for (uint32_t c0 = 0; c0 < n; c0 += TB) {
-        const uint32_t i = c0 + threadIdx.x;
+    if (G::tid() == 0) *count = 0u;
+    G::sync();
+    for (uint32_t c0 = 0; c0 < n; c0 += G::size) {
+        const uint32_t i = c0 + G::tid();
         bool take = false;
         int32_t id = 0;
         if (i < n) {
             const uint32_t key = keys[i];
             if (key <= T) {
-                id = cq[i];
+                id = gld(cq + i);
                 take = key < T || (uint32_t)id <= IT;
             }
         }
-        wave_append(take, id, count, out_id, kk);
+        wave_append<false>(take, id, count, out_id, kk);
     }
-    __syncthreads();
+    G::sync();
     return kk;
 }
 

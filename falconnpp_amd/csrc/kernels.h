@@ -165,6 +165,10 @@ int64_t probe_replay_work_bytes(const ProbeGeom& g);
 // the last id.
 constexpr int64_t IDS_PAD = 4;
 
+// Wall-clock stamps per query of a gather: 0 start, 1 bucket walk done,
+// 2 candidates collected, 3 scored, 4 done.
+constexpr int PHASE_STAMPS = 5;
+
 struct GatherArgs {
     int64_t N;
     int D;
@@ -172,7 +176,7 @@ struct GatherArgs {
     int qProbes;
     int k;
     int64_t window_bits;  // LDS bitset window
-    uint64_t* phase_ts = nullptr;     // per query 5 wall-clock stamps (null: off)
+    uint64_t* phase_ts = nullptr;     // per query PHASE_STAMPS wall-clock stamps (null: off)
     const QueryParams* qp = nullptr;  // per-query probe counts and k (null: qProbes, k above)
     int ldo = 0;                      // row stride of the outputs (0: k)
     // work queue: order[0 .. nitems) (identity when null) in nqueues runs;

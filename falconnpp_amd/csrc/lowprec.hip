@@ -135,12 +135,12 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
     uint32_t* count = reinterpret_cast<uint32_t*>(rid + R);   // 1
     __shared__ int s_digit;
     __shared__ uint32_t s_below;
-    uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * 5 : nullptr;
+    uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * PHASE_STAMPS : nullptr;
     if (ts && threadIdx.x == 0) ts[0] = wall_clock64();
     // a tie-flagged query's probes are replayed exactly first (the LDS is
     // free until the dedup below)
     if (rp.qflag && rp.qflag[q]) {
-        replay_query<TB>(rp, (uint32_t)q, reinterpret_cast<unsigned char*>(lds),
+        replay_query<Whole<TB>>(rp, (uint32_t)q, reinterpret_cast<unsigned char*>(lds),
                          reinterpret_cast<IFPair*>(rp.work + (int64_t)blockIdx.x * rp.wpq));
         __syncthreads();
         dirty = (uint32_t)Wwords;
@@ -152,7 +152,7 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
     const QueryScratch qs = query_scratch(a, q);
     int32_t* cq = qs.cand;
     const uint64_t ctot_q = a.ctot ? a.ctot[q] : ~uint64_t(0);
-    const uint32_t nc = collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, qs.part, qs.ccap,
+    const uint32_t nc = collect_candidates<Whole<TB>>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, qs.part, qs.ccap,
                                                qs.pcap, ts, ctot_q, dirty);
     if (threadIdx.x == 0) a.ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
@@ -162,7 +162,7 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
     // rescoring: with no more candidates than that, all of them do
     uint32_t rr;
     if (nc <= (uint32_t)Rq) {
-        for (uint32_t i = threadIdx.x; i < nc; i += TB) rid[i] = cq[i];
+        for (uint32_t i = threadIdx.x; i < nc; i += TB) rid[i] = gld(cq + i);
         rr = nc;
         __syncthreads();
     } else {
@@ -236,16 +236,19 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
         // position: in LDS over the dedup bitset when they fit, else in the
         // query's scratch segment
         const uint32_t lds_cap = (uint32_t)Wwords;
-        uint32_t* keys = nc <= lds_cap ? bits : reinterpret_cast<uint32_t*>(qs.part);
         if (nc <= lds_cap) dirty = max(dirty, nc);
         constexpr int NT = lp_tiles(FMT, KT);
         constexpr uint32_t SPAN = 8 * NT;
         const uint32_t stride = (uint32_t)NW * SPAN;
+        // candidate ids one step ahead; every load is issued (clamped to the
+        // last candidate) and the id replaced by -1 past the end
+        const uint32_t last = nc - 1;
         int32_t idn[NT];
     #pragma unroll
         for (int h = 0; h < NT; ++h) {
             const uint32_t i = (uint32_t)wv * SPAN + h * 8 + c8;
-            idn[h] = i < nc ? cq[i] : -1;
+            const int32_t v = gld(cq + min(i, last));
+            idn[h] = i < nc ? v : -1;
         }
         for (uint32_t t0 = (uint32_t)wv * SPAN; t0 < nc; t0 += stride) {
             int32_t id[NT];
@@ -254,8 +257,10 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
             for (int h = 0; h < NT; ++h) {
                 id[h] = idn[h];
                 const uint32_t i = t0 + stride + h * 8 + c8;
-                idn[h] = i < nc ? cq[i] : -1;
-                if constexpr (FMT == FP8) rs[h] = lp.uscale > 0.0f ? lp.uscale : (id[h] < 0 ? 0.0f : lp.scale[id[h]]);
+                const int32_t v = gld(cq + min(i, last));
+                idn[h] = i < nc ? v : -1;
+                if constexpr (FMT == FP8)
+                    rs[h] = lp.uscale > 0.0f ? lp.uscale : (id[h] < 0 ? 0.0f : gld(lp.scale + id[h]));
             }
             f32x4 acc[NT];
     #pragma unroll
@@ -299,13 +304,18 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
                 float sc = (r16 & 1) ? s1 : s0;
                 if constexpr (FMT == FP8) sc *= (lp.uscale > 0.0f ? lp.uscale : __shfl(rs[h], 2 * c, WAVE)) * qscale;
                 const uint32_t i = t0 + h * 8 + c;
-                if (r16 < 2 && i < nc) keys[i] = desc_key_any(sc);
+                if (r16 < 2 && i < nc) {
+                    if (nc <= lds_cap)
+                        bits[i] = desc_key_any(sc);
+                    else
+                        gst(reinterpret_cast<uint32_t*>(qs.part) + i, desc_key_any(sc));
+                }
             }
         }
         __syncthreads();
         // the R best approximate scores (ties: smaller id)
-        rr = nc <= lds_cap ? select_smallest<TB>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid)
-                           : select_smallest<TB>(reinterpret_cast<const uint32_t*>(qs.part), cq, nc, (uint32_t)Rq, hist,
+        rr = nc <= lds_cap ? select_smallest<Whole<TB>>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid)
+                           : select_smallest<Whole<TB>>(reinterpret_cast<const uint32_t*>(qs.part), cq, nc, (uint32_t)Rq, hist,
                                                  sh, &s_digit, &s_below, count, rid);
     }
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();

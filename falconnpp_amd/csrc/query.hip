@@ -415,7 +415,7 @@ __global__ __launch_bounds__(RS_SEL_T) void k_replay_select(ProbeGeom g, const u
     const uint32_t q = qlist[f];
     const QueryView qv_ = query_view(g.qp, (int)q, g.qProbes, g.perTable, g.PL, 0);
     const IFPair* pool = reinterpret_cast<const IFPair*>(work + (int64_t)f * wpq);
-    replay_select<RS_SEL_T>(g, qv_, q, pool, bucket_off, probes, ctot, sl, merge_staged);
+    replay_select<Whole<RS_SEL_T>>(g, qv_, q, pool, bucket_off, probes, ctot, sl, merge_staged);
 }
 
 constexpr int RX_T = 512;
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(RX_T) void k_probe_replay(HashGeom hg, ProbeGeom g,
                 replay_table(hg, g, qv_, Q, ldq, masks, q, l, flag, qc, qvals, rlds + (int64_t)w * table_lds,
                              pool + (int64_t)l * qv_.perTable);
         __syncthreads();
-        replay_select<RX_T>(g, qv_, q, pool, bucket_off, probes, ctot, rlds, merge_staged);
+        replay_select<Whole<RX_T>>(g, qv_, q, pool, bucket_off, probes, ctot, rlds, merge_staged);
     }
 }
 
@@ -531,12 +531,12 @@ __device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, cons
     uint32_t* tk_id = tk_key + 1024;    // kcap
     __shared__ int s_digit;
     __shared__ uint32_t s_below;
-    uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * 5 : nullptr;
+    uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * PHASE_STAMPS : nullptr;
     if (ts && threadIdx.x == 0) ts[0] = wall_clock64();
     // a tie-flagged query's probes are replayed exactly first (the LDS is
     // free until the dedup below)
     if (rp.qflag && rp.qflag[q]) {
-        replay_query<TB>(rp, (uint32_t)q, reinterpret_cast<unsigned char*>(lds),
+        replay_query<Whole<TB>>(rp, (uint32_t)q, reinterpret_cast<unsigned char*>(lds),
                          reinterpret_cast<IFPair*>(rp.work + (int64_t)blockIdx.x * rp.wpq));
         __syncthreads();
         dirty = (uint32_t)Wwords;
@@ -550,7 +550,7 @@ __device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, cons
     int32_t* cq = qs.cand;
     float* sq = reinterpret_cast<float*>(qs.part);
     const uint64_t ctot_q = a.ctot ? a.ctot[q] : ~uint64_t(0);
-    const uint32_t nc = collect_candidates<TB>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, qs.part, qs.ccap,
+    const uint32_t nc = collect_candidates<Whole<TB>>(a, qv_.qProbes, pr, bucket_off, ids, bits, wc, sh, cq, qs.part, qs.ccap,
                                                qs.pcap, ts, ctot_q, dirty);
     if (threadIdx.x == 0) a.ncand[q] = nc;
     if (ts && threadIdx.x == 0) ts[2] = wall_clock64();
@@ -579,11 +579,15 @@ __device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, cons
     const bool wsel = kq <= WAVE;
     WaveTopK<1> tk;
     tk.init(kq);
+    // candidate ids one step ahead; every load is issued (clamped to the
+    // last candidate) and the id replaced by -1 past the end
+    const uint32_t last = nc > 0 ? nc - 1 : 0;
     int idn[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const uint32_t ii = (uint32_t)(u * NG + grp);
-        idn[u] = ii < nc ? cq[ii] : -1;
+        const int32_t v = gld(cq + min(ii, last));
+        idn[u] = ii < nc ? v : -1;
     }
     for (uint32_t i0 = 0; i0 < nc; i0 += STEP) {
         int id[U];
@@ -591,7 +595,8 @@ __device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, cons
         for (int u = 0; u < U; ++u) {
             id[u] = idn[u];
             const uint32_t ii = i0 + STEP + (uint32_t)(u * NG + grp);
-            idn[u] = ii < nc ? cq[ii] : -1;
+            const int32_t v = gld(cq + min(ii, last));
+            idn[u] = ii < nc ? v : -1;
         }
         float sc[U];
 #pragma unroll
@@ -625,7 +630,7 @@ __device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, cons
             if (g8 == 0) {
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    if (id[u] >= 0) sq[i0 + u * NG + grp] = sc[u];
+                    if (id[u] >= 0) gst(sq + i0 + u * NG + grp, sc[u]);
             }
             continue;
         }
@@ -665,13 +670,13 @@ __device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, cons
                 for (int i = threadIdx.x; i < 256; i += TB) hist[i] = 0;
                 __syncthreads();
                 for (uint32_t i = threadIdx.x; i < nc; i += TB) {
-                    const uint32_t key = desc_key_any(sq[i]);
+                    const uint32_t key = desc_key_any(gld(sq + i));
                     if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
                 }
                 __syncthreads();
                 int d;
                 uint32_t below;
-                block_find_digit<TB>(hist, k, sh, d, below, eqc, &s_digit, &s_below);
+                block_find_digit<Whole<TB>>(hist, k, sh, d, below, eqc, &s_digit, &s_below);
                 prefix |= (uint32_t)d << shift;
                 pmask |= 255u << shift;
                 k -= below;
@@ -687,13 +692,13 @@ __device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, cons
                 for (int i = threadIdx.x; i < 256; i += TB) hist[i] = 0;
                 __syncthreads();
                 for (uint32_t i = threadIdx.x; i < nc; i += TB) {
-                    const uint32_t id = (uint32_t)cq[i];
-                    if (desc_key_any(sq[i]) == T && (id & im) == ip) atomicAdd(&hist[(id >> shift) & 255u], 1u);
+                    const uint32_t id = (uint32_t)gld(cq + i);
+                    if (desc_key_any(gld(sq + i)) == T && (id & im) == ip) atomicAdd(&hist[(id >> shift) & 255u], 1u);
                 }
                 __syncthreads();
                 int d;
                 uint32_t below, eqd;
-                block_find_digit<TB>(hist, kr, sh, d, below, eqd, &s_digit, &s_below);
+                block_find_digit<Whole<TB>>(hist, kr, sh, d, below, eqd, &s_digit, &s_below);
                 ip |= (uint32_t)d << shift;
                 im |= 255u << shift;
                 kr -= below;
@@ -703,11 +708,11 @@ __device__ __forceinline__ void k_gather_score_f32_one(const GatherArgs& a, cons
         uint32_t written = 0;
         for (uint32_t c0 = 0; c0 < nc && kk > 0; c0 += TB) {
             const uint32_t i = c0 + threadIdx.x;
-            const uint32_t key = i < nc ? desc_key_any(sq[i]) : 0xffffffffu;
-            const uint32_t id = i < nc ? (uint32_t)cq[i] : 0xffffffffu;
+            const uint32_t key = i < nc ? desc_key_any(gld(sq + i)) : 0xffffffffu;
+            const uint32_t id = i < nc ? (uint32_t)gld(cq + i) : 0xffffffffu;
             const uint32_t take = (i < nc && (key < T || (key == T && id <= IT))) ? 1u : 0u;
             uint32_t tot_take;
-            const uint32_t ex_take = block_excl_scan_u32<TB>(take, sh, tot_take);
+            const uint32_t ex_take = block_excl_scan_u32<Whole<TB>>(take, sh, tot_take);
             if (take && written + ex_take < kk) {
                 tk_key[written + ex_take] = key;
                 tk_id[written + ex_take] = id;

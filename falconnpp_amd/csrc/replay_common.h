@@ -443,12 +443,12 @@ __device__ __forceinline__ void merge_lists(const IFPair* list, int L, int PT, i
     wave_lds_sync();
 }
 
-// One workgroup, after the per-table lists of query q are in pool: the
+// One thread group G, after the per-table lists of query q are in pool: the
 // global cut by radix select; when it is unambiguous (no equal values on both
 // sides) the probe set is every pair above it and every pair on it, else
-// wave 0 replays the merge (lists staged in LDS when merge_staged).  Then
+// the group's wave 0 replays the merge (lists staged in LDS when merge_staged).  Then
 // ctot[q].  Shared words: hist[256], and s_* below.
-template <int TB, int MAXKS = 4>
+template <class G, int MAXKS = 4>
 __device__ void replay_select(const ProbeGeom& g, const QueryView& qv_, uint32_t q, const IFPair* pool,
                               const uint64_t* __restrict__ bucket_off, uint32_t* __restrict__ probes,
                               uint64_t* __restrict__ ctot, unsigned char* mlds_raw, int merge_staged) {
@@ -456,18 +456,19 @@ __device__ void replay_select(const ProbeGeom& g, const QueryView& qv_, uint32_t
     __shared__ int s_digit;
     __shared__ uint32_t s_below, s_eqc, s_cnt;
     __shared__ unsigned long long s_ct;
-    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), w = G::tid() / WAVE, t = G::tid();
+    constexpr int TB = G::size;
     const int PT = qv_.perTable, QP = qv_.qProbes;
     const int n = g.L * PT;
     uint32_t prefix = 0, pmask = 0, k = (uint32_t)QP, eqc = 0;
     for (int shift = 24; shift >= 0; shift -= 8) {
-        if (threadIdx.x < 256) hist[threadIdx.x] = 0;
-        __syncthreads();
-        for (int i = threadIdx.x; i < n; i += TB) {
+        if (t < 256) hist[t] = 0;
+        G::sync();
+        for (int i = t; i < n; i += TB) {
             const uint32_t key = ~heap_key(pool[i].val);
             if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
         }
-        __syncthreads();
+        G::sync();
         if (w == 0) {
             int d;
             uint32_t below, e;
@@ -478,20 +479,20 @@ __device__ void replay_select(const ProbeGeom& g, const QueryView& qv_, uint32_t
                 s_eqc = e;
             }
         }
-        __syncthreads();
+        G::sync();
         prefix |= (uint32_t)s_digit << shift;
         pmask |= 255u << shift;
         k -= s_below;
         eqc = s_eqc;
-        __syncthreads();
+        G::sync();
     }
     uint32_t* out = probes + qv_.probe_off;
     if (eqc == k) {
         const uint32_t T = prefix;
-        if (threadIdx.x == 0) s_cnt = 0;
-        __syncthreads();
+        if (t == 0) s_cnt = 0;
+        G::sync();
         for (int c0 = 0; c0 < n; c0 += TB) {
-            const int i = c0 + threadIdx.x;
+            const int i = c0 + t;
             IFPair e{0, 0.0f};
             bool take = false;
             if (i < n) {
@@ -512,34 +513,34 @@ __device__ void replay_select(const ProbeGeom& g, const QueryView& qv_, uint32_t
         IFPair* mq = merge_staged ? mlds + (int64_t)g.L * PT : mlds;
         int* track = reinterpret_cast<int*>(mq + merge_heap_entries(g.L));
         if (merge_staged)
-            for (int64_t i = threadIdx.x; i < (int64_t)g.L * PT; i += TB) mlds[i] = pool[i];
-        __syncthreads();
+            for (int64_t i = t; i < (int64_t)g.L * PT; i += TB) mlds[i] = pool[i];
+        G::sync();
         if (w == 0) merge_lists<MAXKS>(merge_staged ? mlds : pool, g.L, PT, QP, g.NB, mq, track, out);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) s_ct = 0;
-    __syncthreads();
+    G::sync();
+    if (t == 0) s_ct = 0;
+    G::sync();
     unsigned long long ct = 0;
-    for (int c = threadIdx.x; c < QP; c += TB) {
+    for (int c = t; c < QP; c += TB) {
         const uint32_t b = out[c];
         ct += bucket_off[b + 1] - bucket_off[b];
     }
     atomicAdd(&s_ct, ct);
-    __syncthreads();
-    if (threadIdx.x == 0) ctot[q] = s_ct;
-    __syncthreads();
+    G::sync();
+    if (t == 0) ctot[q] = s_ct;
+    G::sync();
 }
 
 
-// A flagged query's probes replayed by the whole workgroup (TB threads) of a
+// A flagged query's probes replayed by a thread group G (the workgroup) of a
 // gather kernel before it gathers the query: its waves rebuild the per-table
-// lists into pool, then replay_select.  lds: the workgroup's dynamic LDS,
-// at least r.twaves * r.table_lds bytes and the merge's share.
+// lists into pool, then replay_select.  lds: LDS of the group, at least
+// r.twaves * r.table_lds bytes and the merge's share.
 // Kept out of line: inlined, its registers would be allocated across the
 // gather's scoring loop.
-template <int TB>
+template <class G>
 __device__ __noinline__ void replay_query(const ReplayArgs& r, uint32_t q, unsigned char* lds, IFPair* pool) {
-    const int w = threadIdx.x / WAVE;
+    const int w = G::tid() / WAVE;
     const QueryView qv_ = query_view(r.pg.qp, (int)q, r.pg.qProbes, r.pg.perTable, r.pg.PL, 0);
     const uint32_t flag = r.qflag[q];
     // the replay is a chain of dependent steps on few waves while the other
@@ -549,8 +550,8 @@ __device__ __noinline__ void replay_query(const ReplayArgs& r, uint32_t q, unsig
         for (int l = w; l < r.pg.L; l += r.twaves)
             replay_table(r.hg, r.pg, qv_, r.Q, r.ldq, r.masks, q, l, flag, r.qc, r.qv,
                          lds + (int64_t)w * r.table_lds, pool + (int64_t)l * qv_.perTable);
-    __syncthreads();
-    replay_select<TB, 2>(r.pg, qv_, q, pool, r.bucket_off, r.probes, r.ctot, lds, r.merge_staged);
+    G::sync();
+    replay_select<G, 2>(r.pg, qv_, q, pool, r.bucket_off, r.probes, r.ctot, lds, r.merge_staged);
     __builtin_amdgcn_s_setprio(0);
 }
 
