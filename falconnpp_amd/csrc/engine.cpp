@@ -98,6 +98,7 @@ struct Index {
     int order_mode = 1;            // ORDER_* below
     bool keep_candidates = false;  // every query's candidates kept after the call (inspection)
     int replay_mode = 0;           // tie replay: 0 per call, 1 separate kernels, 2 inside the gather
+    bool small_queries = true;     // queries of at most SMALL_MAX_CTOT bucket ids served one per wave
     DevBuf<int32_t> keep_cand, keep_part;
     DevBuf<uint64_t> keep_off;
     int wall_khz = 0;
@@ -125,13 +126,14 @@ struct Index {
         const uint64_t* ctot = nullptr;
         const uint64_t* ncand = nullptr;
         const uint32_t* qflag = nullptr;
+        const uint32_t* nsmall = nullptr;  // small queries (device word), or null
         hipStream_t st = nullptr;
         bool summed = true;
         std::vector<uint64_t> seg;  // keep-candidates: segment offsets, nq + 1
     } last;
     std::vector<uint64_t> last_ts;  // per-query phase stamps of the last stamped call
 
-    int64_t stat_hash_ties = 0, stat_bucket_ties = 0, stat_ctot = 0, stat_cuniq = 0, stat_qties = 0;
+    int64_t stat_hash_ties = 0, stat_bucket_ties = 0, stat_ctot = 0, stat_cuniq = 0, stat_qties = 0, stat_small = 0;
     int stat_probe_path = -1;  // probe selection of the last query call (PROBE_*)
 
     // reduced-precision scoring: 0 = FP32 (reference semantics), 1 = BF16,
@@ -591,7 +593,9 @@ struct QueryLayout {
     size_t Qpad, qc, qv, qflag, probes, ctot, ncand, qparams, okey, oval, okey2, oval2, ohist, ohist_off, oscan, qlist,
         counters, rwork, phase_ts, cand, part, total;
 };
-constexpr int N_COUNTERS = 16;  // [0, 8) queue heads, [8] replay claim, [9] flagged queries
+// [0, 8) queue heads, [8] replay claim, [9] flagged queries, [10] small
+// queries, [11] small-query queue head
+constexpr int N_COUNTERS = 16;
 
 QueryLayout query_layout(int nq, int L, int PL, uint64_t nprobes, bool batched, unsigned rareas, int64_t wpq,
                          bool stamps, unsigned slots, uint64_t slot_cand, uint64_t slot_part) {
@@ -842,6 +846,16 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     rp.wpq = wpq;
     const int order_mode = exact_only ? ORDER_LONGEST : I.order_mode;
     const bool marks = I.profiling && !fixed;
+    // small queries (at most SMALL_MAX_CTOT bucket ids, not tie-flagged,
+    // k <= 64) go to the end of the sorted order and are served one per wave
+    // by launch_gather_small; the workgroup gather takes the rest
+    const bool small = I.small_queries && !exact_only && order_mode != ORDER_IDENTITY && nq > 1;
+    SmallPick sp;
+    sp.ctot = ctot;
+    sp.qp = batched ? qparams : nullptr;
+    sp.k = k;
+    sp.max_ctot = small ? SMALL_MAX_CTOT : 0u;
+    sp.count = counters + 10;
     // 1. hashing, probe selection, exact replay of the flagged queries
     auto front = [&](bool ev) {
         launch_pad_rows(dQ, ldq, nq, I.n_features, Qpad, I.ldx, st);
@@ -871,11 +885,11 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
             uint32_t* okey = reinterpret_cast<uint32_t*>(W + lo.okey);
             int bits = 32;
             if (order_mode == ORDER_BUCKET) {
-                launch_order_keys(qc, qflag, nq, L, PL, 2 * I.n_proj, okey, st);
-                bits = ilog2_exact(I.NB()) + 1;
+                launch_order_keys(qc, qflag, nq, L, PL, 2 * I.n_proj, okey, sp, st);
+                bits = ilog2_exact(I.NB()) + 1 + (small ? 1 : 0);
                 nqueues = nq >= 64 ? 8 : 1;
             } else {
-                launch_ctot_keys(ctot, qflag, nq, okey, st);
+                launch_ctot_keys(ctot, qflag, nq, okey, sp, st);
             }
             uint32_t *sk = nullptr, *sv = nullptr;
             radix_sort_pairs(okey, reinterpret_cast<uint32_t*>(W + lo.oval), reinterpret_cast<uint32_t*>(W + lo.okey2),
@@ -887,6 +901,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         GatherArgs a = ga;
         a.order = order;
         a.nqueues = nqueues;
+        a.small_at = small ? 10 : -1;
         if (ev) I.mark(5, st);
         if (lowp)
             launch_gather_score_lp(a, lp, rp, glds, grid, probes, I.bucket_off.get(), I.ids.get(), I.Xc.get(), Qpad,
@@ -894,6 +909,9 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         else
             launch_gather_score_f32(a, rp, glds, grid, probes, I.bucket_off.get(), I.ids.get(), I.Xc.get(), Qpad,
                                     I.ldx, d_out, d_scores, gthreads, st);
+        if (small)
+            launch_gather_small(a, probes, I.bucket_off.get(), I.ids.get(), I.Xc.get(), Qpad, I.ldx, d_out, d_scores,
+                                st);
         if (ev) I.mark(6, st);
     };
     if (keep) {
@@ -924,7 +942,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
             (uint64_t)(uintptr_t)(lowp ? (I.precision == 1 ? (const void*)lp.xb : (const void*)lp.x8) : nullptr),
             (uint64_t)__builtin_bit_cast(uint32_t, lp.uscale), (uint64_t)gthreads, (uint64_t)order_mode, grid,
             slot_cand, slot_part, (uint64_t)stamps, (uint64_t)probe_path, (uint64_t)ga.window_bits, rcap, rover,
-            (uint64_t)in_gather, glds};
+            (uint64_t)in_gather, glds, (uint64_t)small};
         if (!I.graph_exec || key != I.graph_key) {
             I.drop_graph();
             FPP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -954,6 +972,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     I.last.ctot = ctot;
     I.last.ncand = ncand;
     I.last.qflag = qflag;
+    I.last.nsmall = small ? counters + 10 : nullptr;
     I.last.st = st;
     I.last.summed = false;
     if (marks) {
@@ -1003,7 +1022,7 @@ void sum_last_call(Index& I) {
     Index::LastCall& c = I.last;
     if (c.summed) return;
     c.summed = true;
-    I.stat_ctot = I.stat_cuniq = I.stat_qties = 0;
+    I.stat_ctot = I.stat_cuniq = I.stat_qties = I.stat_small = 0;
     if (c.nq == 0 || !c.ctot) return;
     std::vector<uint64_t> ct(c.nq), nc(c.nq);
     std::vector<uint32_t> fl(c.nq);
@@ -1015,6 +1034,11 @@ void sum_last_call(Index& I) {
         I.stat_ctot += (int64_t)ct[q];
         I.stat_cuniq += (int64_t)nc[q];
         I.stat_qties += fl[q] ? 1 : 0;
+    }
+    if (c.nsmall) {
+        uint32_t ns = 0;
+        FPP_HIP(hipMemcpy(&ns, c.nsmall, 4, hipMemcpyDeviceToHost));
+        I.stat_small = ns;
     }
 }
 
@@ -1730,6 +1754,7 @@ int fpp_get_stats(fpp_index* h, int64_t* out) {
         out[4] = I.stat_qties;
         out[5] = I.last.nq;
         out[6] = I.stat_probe_path;
+        out[7] = I.stat_small;
     });
 }
 
@@ -1803,6 +1828,9 @@ int fpp_set_tuning(fpp_index* h, int which, int value) {
             case FPP_TUNE_GRAPH:
                 h->I.use_graph = value != 0;
                 if (!h->I.use_graph) h->I.drop_graph();
+                break;
+            case FPP_TUNE_SMALL_QUERIES:
+                h->I.small_queries = value != 0;
                 break;
             default:
                 throw Error(FPP_EINVAL, "unknown tuning knob");

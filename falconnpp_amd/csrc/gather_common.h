@@ -233,7 +233,8 @@ __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ 
 }
 
 // Next query of a persistent gather workgroup, or -1 when every run of the
-// work queue is drained.  order[0 .. nitems) (identity when null) is split
+// work queue is drained.  order[0 .. n) (identity when null; n = nitems less
+// the small queries at the end when small_at is set) is split
 // into nqueues runs of consecutive positions; workgroup b takes from run
 // b % nqueues first (with 8 runs, the XCD the workgroup sits on), then from
 // the others.  Every successful claim consumes an item, so each workgroup
@@ -244,10 +245,15 @@ __device__ __forceinline__ int claim_item(const GatherArgs& a) {
     int q = -1;
     const uint32_t R = (uint32_t)a.nqueues;
     const uint32_t home = blockIdx.x % R;
+    // the small queries at the end of the order are not this kernel's
+    const uint32_t n =
+        a.small_at >= 0
+            ? a.nitems - __hip_atomic_load(&a.heads[a.small_at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+            : a.nitems;
     for (uint32_t j = 0; j < R && q < 0; ++j) {
         const uint32_t r = (home + j) % R;
-        const uint32_t lo = (uint32_t)((uint64_t)a.nitems * r / R);
-        const uint32_t hi = (uint32_t)((uint64_t)a.nitems * (r + 1) / R);
+        const uint32_t lo = (uint32_t)((uint64_t)n * r / R);
+        const uint32_t hi = (uint32_t)((uint64_t)n * (r + 1) / R);
         if (lo + __hip_atomic_load(&a.heads[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= hi) continue;
         const uint32_t i = atomicAdd(&a.heads[r], 1u);
         if (lo + i < hi) q = a.order ? (int)a.order[lo + i] : (int)(lo + i);

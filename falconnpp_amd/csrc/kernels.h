@@ -179,8 +179,12 @@ struct GatherArgs {
     uint64_t* phase_ts = nullptr;     // per query PHASE_STAMPS wall-clock stamps (null: off)
     const QueryParams* qp = nullptr;  // per-query probe counts and k (null: qProbes, k above)
     int ldo = 0;                      // row stride of the outputs (0: k)
+    int small_at = -1;                // see the work queue below
     // work queue: order[0 .. nitems) (identity when null) in nqueues runs;
     // heads[nqueues] zeroed beforehand
+    // small_at >= 0: the last heads[small_at] items of order are small
+    // queries, served one per wave by launch_gather_small (whose queue head is
+    // heads[small_at + 1]); the workgroup kernels take the items before them
     const uint32_t* order = nullptr;
     uint32_t nitems = 0;
     int nqueues = 1;
@@ -197,13 +201,28 @@ struct GatherArgs {
     uint64_t* ncand = nullptr;  // unique candidates per query
     const uint64_t* ctot = nullptr;  // C_tot per query (sum of its probed bucket sizes)
 };
+// Small queries: at most SMALL_MAX_CTOT bucket ids (C_tot), not tie-flagged,
+// k <= 64.  One wave gathers, dedups and scores such a query (FP32, every
+// candidate, in every precision mode).  SmallPick: the test (max_ctot 0: no
+// query is small); count receives the number of small queries.
+constexpr uint32_t SMALL_MAX_CTOT = 512;
+struct SmallPick {
+    const uint64_t* ctot = nullptr;
+    const QueryParams* qp = nullptr;  // per-query k (null: k)
+    int k = 0;
+    uint32_t max_ctot = 0;
+    uint32_t* count = nullptr;
+};
 // Query order sort keys: the best bucket of table 0 (queries with equal keys
 // share most candidates), or C_tot descending (longest first).
 // Queries whose qflag is set (replayed inside the gather) get key 0, ahead of
-// the others.
+// the others; small queries (sp) get keys above every other, so they end the
+// order: for the bucket keys the bit above the bucket's (sort one more bit),
+// for the C_tot keys every other query's key is held below them.
 void launch_order_keys(const int32_t* qc, const uint32_t* qflag, int nq, int L, int PL, int two_np, uint32_t* key,
-                       hipStream_t st);
-void launch_ctot_keys(const uint64_t* ctot, const uint32_t* qflag, int nq, uint32_t* key, hipStream_t st);
+                       const SmallPick& sp, hipStream_t st);
+void launch_ctot_keys(const uint64_t* ctot, const uint32_t* qflag, int nq, uint32_t* key, const SmallPick& sp,
+                      hipStream_t st);
 // dst[r*ldd + c] = src[r*lds + c] for c < D, 0 up to ldd (queries padded to
 // the scored row stride).
 void launch_pad_rows(const float* src, int64_t lds, int64_t rows, int D, float* dst, int64_t ldd, hipStream_t st);
@@ -237,6 +256,14 @@ int64_t gather_window_bits(int64_t N, int64_t extra_words, int per_cu);
 // persistent launch of `grid` workgroups (persistent_grid with the LDS of
 // gather_f32_lds); a.ncand[q] = unique candidates.
 size_t gather_f32_lds(const GatherArgs& a, int threads);
+// The small queries of a call (the end of a.order, a.small_at), one wave
+// each, persistent over the queue head heads[small_at + 1]: candidates
+// deduped in the wave's LDS, FP32 scores in the gather kernel's order of
+// addition, top-k in the wave.  a.ncand[q] = unique candidates; with
+// a.seg_off the candidates are also kept at a.cand + seg_off[q].
+void launch_gather_small(const GatherArgs& a, const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids,
+                         const float* Xc, const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores,
+                         hipStream_t st);
 void launch_gather_score_f32(const GatherArgs& a, const ReplayArgs& rp, size_t lds, unsigned grid,
                              const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids, const float* Xc,
                              const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores, int threads,

@@ -487,12 +487,32 @@ __global__ __launch_bounds__(CF_T) void k_compact_flagged(int nq, const uint32_t
 
 // Sort key for longest-first order: C_tot descending, tie-flagged queries
 // (replayed in the gather, C_tot not known yet) first.
+// Is query q small (served one per wave)?  See SmallPick.
+__device__ __forceinline__ bool small_query(const SmallPick& sp, const uint32_t* __restrict__ qflag, int q) {
+    if (sp.max_ctot == 0 || qflag[q]) return false;
+    const int kq = sp.qp ? sp.qp[q].k : sp.k;
+    return kq <= WAVE && sp.ctot[q] <= sp.max_ctot;
+}
+
+// Adds the wave's small queries to *count (one atomic per wave); the whole
+// wave calls it.
+__device__ __forceinline__ void count_small(bool small, uint32_t* count) {
+    const uint64_t m = __ballot(small);
+    if (m && (threadIdx.x & (WAVE - 1)) == (unsigned)(__ffsll((long long)m) - 1))
+        atomicAdd(count, (uint32_t)__popcll(m));
+}
+
 __global__ void k_ctot_keys(const uint64_t* __restrict__ ctot, const uint32_t* __restrict__ qflag, int nq,
-                            uint32_t* __restrict__ key) {
+                            uint32_t* __restrict__ key, SmallPick sp) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool small = q < nq && small_query(sp, qflag, q);
+    if (sp.max_ctot) count_small(small, sp.count);
     if (q >= nq) return;
     const uint64_t c = ctot[q];
-    key[q] = qflag[q] ? 0u : 0xffffffffu - (uint32_t)(c >= 0xffffffffull ? 0xfffffffeull : c);
+    uint32_t k = 0xffffffffu - (uint32_t)(c >= 0xffffffffull ? 0xfffffffeull : c);
+    // every key but a small query's stays below 2^32 - 1 - max_ctot
+    if (sp.max_ctot && !small) k = min(k, 0xfffffffeu - sp.max_ctot);
+    key[q] = qflag[q] ? 0u : k;
 }
 
 // Probes of a call whose bucket is empty (the reference's iNumEmptyBucket,
@@ -760,14 +780,173 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(TB <= 512 ? 
     }
 }
 
+// ---- small queries: one wave per query -------------------------------------
+// A query with at most SMALL_MAX_CTOT bucket ids costs a workgroup of the
+// kernel above mostly barriers and dependent loads; here each wave serves a
+// query of its own: lane p walks bucket p, an open-addressing set in the
+// wave's LDS keeps the first occurrence of every id (the reference's bitset,
+// src/FalconnPP.cpp:552-611), and the rows are scored with the lane layout
+// and order of addition of k_gather_score_f32 (8 lanes per row, the same
+// chunks, the same shuffles), so scores and the (score, smaller id) ranking
+// are bit for bit the same; the top k stay in the wave's registers.
+constexpr int SMALL_T = 256;
+constexpr int SMALL_WAVES = SMALL_T / WAVE;
+constexpr int SMALL_HASH_LG = 10;  // 1024 words, at least twice SMALL_MAX_CTOT
+static_assert((1u << SMALL_HASH_LG) >= 2 * SMALL_MAX_CTOT, "small-query set too small");
+
+template <int CPL>
+__global__ __launch_bounds__(SMALL_T) void k_gather_small_f32(GatherArgs a, const uint32_t* __restrict__ probes,
+                                                              const uint64_t* __restrict__ bucket_off,
+                                                              const int32_t* __restrict__ ids,
+                                                              const float* __restrict__ Xc,
+                                                              const float* __restrict__ Q, int64_t ldq,
+                                                              int32_t* __restrict__ out_ids,
+                                                              float* __restrict__ out_scores) {
+    __shared__ uint32_t hs[SMALL_WAVES][1 << SMALL_HASH_LG];
+    __shared__ int32_t cl[SMALL_WAVES][SMALL_MAX_CTOT];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = threadIdx.x / WAVE;
+    uint32_t* H = hs[wv];
+    int32_t* C = cl[wv];
+    for (int e = lane; e < (1 << SMALL_HASH_LG); e += WAVE) H[e] = 0u;
+    wave_lds_sync();
+    uint32_t* head = a.heads + a.small_at + 1;
+    const uint32_t nsmall = __hip_atomic_load(a.heads + a.small_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t begin = a.nitems - nsmall;
+    const int g8 = lane & 7;
+    const int grp = lane >> 3;
+    const int64_t nchunk = a.ldx / 4;
+    // two rows per lane group in flight (one from 64 features per lane up),
+    // so that four waves per SIMD fit at d <= 256
+    constexpr int U = CPL >= 16 ? 1 : 2;
+    const int ldo = a.ldo ? a.ldo : a.k;
+    for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(head, 1u);
+        it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it);
+        if (it >= nsmall) break;
+        const int q = (int)gld(a.order + begin + it);
+        uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * PHASE_STAMPS : nullptr;
+        if (ts && lane == 0) ts[0] = wall_clock64();
+        const QueryView qv_ = query_view(a.qp, q, a.qProbes, 0, 0, a.k);
+        const uint32_t* pr = probes + qv_.probe_off;
+        // bucket walk and dedup: lane p takes bucket c0 + p, four ids per load
+        // (the id array is padded past its end); first occurrences appended
+        uint32_t nc = 0;
+        for (int c0 = 0; c0 < qv_.qProbes; c0 += WAVE) {
+            uint64_t s = 0;
+            uint32_t len = 0;
+            if (c0 + lane < qv_.qProbes) {
+                const uint32_t b = gld(pr + c0 + lane);
+                s = gld(bucket_off + b);
+                len = (uint32_t)(gld(bucket_off + b + 1) - s);
+            }
+            uint32_t mx = len;
+            for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, WAVE));
+            mx = (uint32_t)__builtin_amdgcn_readfirstlane((int)mx);
+            for (uint32_t j = 0; j < mx; j += 4) {
+                i32x4_a4 t = {0, 0, 0, 0};
+                if (j < len) t = gld_i32x4(ids + s + j);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int32_t v = t[r];
+                    const bool fresh = j + r < len && hashset_insert(H, SMALL_HASH_LG, v);
+                    const uint64_t m = __ballot(fresh);
+                    const uint32_t pos = nc + lane_rank(m);
+                    if (fresh && pos < SMALL_MAX_CTOT) C[pos] = v;
+                    nc += (uint32_t)__popcll(m);
+                }
+            }
+        }
+        nc = min(nc, SMALL_MAX_CTOT);
+        wave_lds_sync();
+        if (ts && lane == 0) ts[1] = ts[2] = wall_clock64();
+        if (lane == 0) gst(a.ncand + q, (uint64_t)nc);
+        if (a.seg_off) {
+            int32_t* dst = a.cand + gld(a.seg_off + q);
+            for (uint32_t e = lane; e < nc; e += WAVE) gst(dst + e, C[e]);
+        }
+        // FP32 scores, as in k_gather_score_f32: 8 lanes per row, CPL float4
+        // chunks per lane (past the row end: the last chunk against a zero
+        // query chunk), U rows per lane group in flight
+        const float4* q4 = reinterpret_cast<const float4*>(Q + (int64_t)q * ldq);
+        float4 qr[CPL];
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int64_t ch = g8 + 8 * t;
+            qr[t] = ch < nchunk ? q4[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        WaveTopK<1> tk;
+        tk.init(qv_.k);
+        for (uint32_t i0 = 0; i0 < nc; i0 += 8 * U) {
+            int id[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t ii = i0 + (uint32_t)(u * 8 + grp);
+                id[u] = ii < nc ? C[ii] : -1;
+            }
+            float sc[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) sc[u] = 0.f;
+            float4 x[CPL][U];
+#pragma unroll
+            for (int t = 0; t < CPL; ++t) {
+                const int64_t ch = min<int64_t>(g8 + 8 * t, nchunk - 1);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    x[t][u] = reinterpret_cast<const float4*>(Xc + (int64_t)(id[u] < 0 ? 0 : id[u]) * a.ldx)[ch];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < CPL; ++t) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    sc[u] = fmaf(qr[t].x, x[t][u].x, sc[u]);
+                    sc[u] = fmaf(qr[t].y, x[t][u].y, sc[u]);
+                    sc[u] = fmaf(qr[t].z, x[t][u].z, sc[u]);
+                    sc[u] = fmaf(qr[t].w, x[t][u].w, sc[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                sc[u] += __shfl_xor(sc[u], 1, WAVE);
+                sc[u] += __shfl_xor(sc[u], 2, WAVE);
+                sc[u] += __shfl_xor(sc[u], 4, WAVE);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) tk.offer(score_key(sc[u], id[u]), 0x0101010101010101ull);
+        }
+        if (ts && lane == 0) ts[3] = wall_clock64();
+        // lane j holds the j-th best key
+        int32_t* oq = out_ids + (size_t)q * ldo;
+        float* osq = out_scores ? out_scores + (size_t)q * ldo : nullptr;
+        const uint32_t kk = min((uint32_t)qv_.k, nc);
+        for (uint32_t j = (uint32_t)lane; j < (uint32_t)ldo; j += WAVE) {
+            const bool hit = j < kk;
+            const uint64_t ke = tk.w[0];
+            gst(oq + j, hit ? (int32_t)(uint32_t)(ke & 0xffffffffu) : 0);
+            if (osq) gst(osq + j, hit ? desc_key_any_value((uint32_t)(ke >> 32)) : 0.0f);
+        }
+        if (ts && lane == 0) ts[4] = wall_clock64();
+        // the set is empty again for the next query
+        wave_lds_sync();
+        for (int e = lane; e < (1 << SMALL_HASH_LG); e += WAVE) H[e] = 0u;
+        wave_lds_sync();
+    }
+}
+
 // Sort key of a query: its best bucket of table 0 (layer-1 and layer-2 top
 // codes).  Queries with equal keys share most candidates.
+// sbit: the bit above the bucket keys (set for small queries).
 __global__ void k_order_keys(const int32_t* __restrict__ qc, const uint32_t* __restrict__ qflag, int nq, int L,
-                             int PL, int two_np, uint32_t* __restrict__ key) {
+                             int PL, int two_np, uint32_t* __restrict__ key, SmallPick sp, int sbit) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool small = q < nq && small_query(sp, qflag, q);
+    if (sp.max_ctot) count_small(small, sp.count);
     if (q >= nq) return;
     const size_t base = (size_t)q * L * 2 * PL;
-    key[q] = qflag[q] ? 0u : 1u + (uint32_t)(qc[base] * two_np + qc[base + PL]);
+    const uint32_t k = qflag[q] ? 0u : 1u + (uint32_t)(qc[base] * two_np + qc[base + PL]);
+    key[q] = small ? k | (1u << sbit) : k;
 }
 
 // dst row r = src row r (D floats) followed by zeros up to ldd.
@@ -847,9 +1026,13 @@ void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const fl
 }
 
 void launch_order_keys(const int32_t* qc, const uint32_t* qflag, int nq, int L, int PL, int two_np, uint32_t* key,
-                       hipStream_t st) {
+                       const SmallPick& sp, hipStream_t st) {
     if (nq == 0) return;
-    k_order_keys<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(qc, qflag, nq, L, PL, two_np, key);
+    // bucket keys 1 + b run up to NB = two_np^2, which takes log2(NB) + 1
+    // bits; sbit is the bit above them
+    const int sbit = ilog2_exact((int64_t)two_np * two_np) + 1;
+    FPP_REQUIRE(sbit >= 1 && sbit < 32, FPP_EINTERNAL, "order key width");
+    k_order_keys<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(qc, qflag, nq, L, PL, two_np, key, sp, sbit);
     FPP_HIP(hipGetLastError());
 }
 
@@ -944,9 +1127,10 @@ void launch_compact_flagged(int nq, const uint32_t* qflag, uint32_t* list, uint3
     FPP_HIP(hipGetLastError());
 }
 
-void launch_ctot_keys(const uint64_t* ctot, const uint32_t* qflag, int nq, uint32_t* key, hipStream_t st) {
+void launch_ctot_keys(const uint64_t* ctot, const uint32_t* qflag, int nq, uint32_t* key, const SmallPick& sp,
+                      hipStream_t st) {
     if (nq <= 0) return;
-    k_ctot_keys<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(ctot, qflag, nq, key);
+    k_ctot_keys<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(ctot, qflag, nq, key, sp);
     FPP_HIP(hipGetLastError());
 }
 
@@ -963,6 +1147,40 @@ int64_t gather_window_bits(int64_t N, int64_t extra_words, int per_cu) {
 }
 
 size_t gather_f32_lds(const GatherArgs& a, int threads) { return gather_lds_bytes(a, threads); }
+
+void launch_gather_small(const GatherArgs& a, const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids,
+                         const float* Xc, const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores,
+                         hipStream_t st) {
+    FPP_REQUIRE(a.small_at >= 0 && a.order && a.ncand && a.heads, FPP_EINTERNAL, "small-query gather arguments");
+    FPP_REQUIRE(a.ldx % 4 == 0 && a.ldx <= 1024, FPP_EINVAL, "n_features must be <= 1024");
+    if (a.nitems == 0) return;
+    // persistent: up to four workgroups (sixteen waves) per CU, no more than
+    // one wave per query
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        FPP_HIP(hipGetDevice(&dev));
+        FPP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(a.nitems, SMALL_WAVES), 4 * cus));
+    const int cpl = (int)ceil_div(a.ldx / 4, 8);
+#define FPP_S(C) \
+    k_gather_small_f32<C><<<grid, SMALL_T, 0, st>>>(a, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores)
+    if (cpl <= 1)
+        FPP_S(1);
+    else if (cpl <= 2)
+        FPP_S(2);
+    else if (cpl <= 4)
+        FPP_S(4);
+    else if (cpl <= 8)
+        FPP_S(8);
+    else if (cpl <= 16)
+        FPP_S(16);
+    else
+        FPP_S(32);
+#undef FPP_S
+    FPP_HIP(hipGetLastError());
+}
 
 void launch_gather_score_f32(const GatherArgs& a, const ReplayArgs& rp, size_t lds, unsigned grid,
                              const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids, const float* Xc,

@@ -31,6 +31,7 @@ TUNE_QUERY_ORDER, TUNE_GATHER_THREADS, TUNE_PERSISTENT, TUNE_GRAPH, TUNE_VERBOSE
 TUNE_KEEP_CANDIDATES = 7
 TUNE_REPLAY = 8  # 0 per call, 1 separate kernels, 2 inside the gather
 TUNE_WORKSPACE_MB = 9  # bound of the per-workgroup candidate scratch
+TUNE_SMALL_QUERIES = 10  # queries of at most 512 bucket ids served one per wave (default on)
 
 FPP_EINVAL, FPP_ESTATE, FPP_EHIP, FPP_ENOMEM, FPP_EINTERNAL, FPP_EIO = 1, 2, 3, 4, 5, 6
 
@@ -314,10 +315,10 @@ class Index:
         return codes.reshape(self.L, m, self.iProbes), vals.reshape(self.L, m, self.iProbes)
 
     def stats(self):
-        s = np.zeros(7, np.int64)
+        s = np.zeros(8, np.int64)
         _check(load().fpp_get_stats(self.h, _ptr(s)))
         return dict(hash_ties=int(s[0]), bucket_ties=int(s[1]), ctot=int(s[2]), cuniq=int(s[3]),
-                    query_ties=int(s[4]), nq=int(s[5]), probe_path=int(s[6]))
+                    query_ties=int(s[4]), nq=int(s[5]), probe_path=int(s[6]), small_queries=int(s[7]))
 
     def set_profiling(self, level: int):
         """0 off, 1 HIP events per query stage, 2 also per-query phase stamps."""

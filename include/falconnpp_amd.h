@@ -144,7 +144,9 @@ int fpp_hash_points(fpp_index* h, const float* V, int64_t ld, int m, uint32_t* c
  * for ties, [2]=sum C_tot of the last query call, [3]=sum C_uniq, [4]=queries
  * replayed for ties, [5]=queries in the last call, [6]=probe selection of the
  * last call (0: pool of every pair, 1: per-table top lists, 2: exact replay of
- * every query because neither pool fits the LDS).  out must hold 7 values. */
+ * every query because neither pool fits the LDS), [7]=queries of the last
+ * call served one per wave (FPP_TUNE_SMALL_QUERIES).  out must hold 8
+ * values. */
 int fpp_get_stats(fpp_index* h, int64_t* out);
 /* per-stage device time of the last query call when profiling is on
  * (on >= 1: HIP events on the call's stream): ms[0]=query hashing,
@@ -196,7 +198,13 @@ int fpp_get_fp8_scale(fpp_index* h, float* shared);
  * the query, before it gathers it (flagged queries head the queue).
  * FPP_TUNE_WORKSPACE_MB: bound, in MiB, of the per-workgroup candidate
  * scratch of a query call (default 8192); a tighter bound runs the gather on
- * fewer workgroups, with the same results. */
+ * fewer workgroups, with the same results.  FPP_TUNE_SMALL_QUERIES 0/1
+ * (default 1): a query whose probed buckets hold at most 512 ids in all, that
+ * needs no exact tie replay and asks for at most 64 neighbours is gathered
+ * and scored by one wave instead of a workgroup (with a sorted query order,
+ * FPP_TUNE_QUERY_ORDER 1 or 2, and more than one query).  FP32 results are
+ * the same either way; in the reduced-precision modes such a query's
+ * candidates are all scored exactly in FP32. */
 enum {
     FPP_TUNE_QUERY_ORDER = 1,
     FPP_TUNE_GATHER_THREADS = 2,
@@ -206,7 +214,8 @@ enum {
     FPP_TUNE_FP8_PER_ROW = 6,
     FPP_TUNE_KEEP_CANDIDATES = 7,
     FPP_TUNE_REPLAY = 8,
-    FPP_TUNE_WORKSPACE_MB = 9
+    FPP_TUNE_WORKSPACE_MB = 9,
+    FPP_TUNE_SMALL_QUERIES = 10
 };
 int fpp_set_tuning(fpp_index* h, int which, int value);
 /* Autotuner (no reference counterpart): times the gather workgroup sizes,

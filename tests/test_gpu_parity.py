@@ -356,6 +356,84 @@ def test_low_precision_recall_and_exact_scores(case, mode, min_recall):
         assert np.all(got[j, m:] == 0) and np.all(scores[j, m:] == 0)
 
 
+# ---- small queries: one wave per query ----------------------------------
+
+SMALL_CASES = [
+    # (case, qProbes, k): few probes, so (most) queries hold at most 512 ids
+    ("mix_d128_p2", 6, 10),
+    ("sphere_d100_p1", 20, 20),
+    ("mix_d200_p3", 12, 20),
+    ("l100_qp10000", 60, 20),        # 100 tables, about one id per bucket
+    ("int_grid_ties", 10, 10),       # tie-flagged queries stay on the workgroup path
+    ("nproj16_d40", 30, 15),
+]
+
+
+@pytest.mark.parametrize("keep", [0, 1])
+@pytest.mark.parametrize("case,qp,k", SMALL_CASES)
+def test_small_queries_equal_workgroup_path(case, qp, k, keep):
+    """Queries of at most 512 bucket ids served one per wave return the ids,
+    scores and candidate counts of the workgroup gather."""
+    X, Q, g, o = _built(case)
+    g.set_tuning(E.TUNE_KEEP_CANDIDATES, keep)
+    try:
+        res = {}
+        for on in (0, 1):
+            g.set_tuning(E.TUNE_SMALL_QUERIES, on)
+            ids, sc = _run_single(g, Q, qp, k)
+            st = g.stats()
+            cands = [g.last_candidates(j) for j in range(len(Q))] if keep else None
+            res[on] = (ids, sc, st, cands)
+    finally:
+        g.set_tuning(E.TUNE_SMALL_QUERIES, 1)
+        g.set_tuning(E.TUNE_KEEP_CANDIDATES, 1)
+    assert res[0][2]["small_queries"] == 0
+    assert res[1][2]["small_queries"] > 0, case
+    assert res[0][2]["cuniq"] == res[1][2]["cuniq"] and res[0][2]["ctot"] == res[1][2]["ctot"]
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+    if keep:
+        for j in range(len(Q)):
+            assert np.array_equal(res[0][3][j], res[1][3][j]), j
+
+
+@pytest.mark.parametrize("case,qp,k", [("mix_d128_p2", 6, 10), ("l100_qp10000", 60, 20),
+                                       ("sphere_d100_p1", 20, 20)])
+def test_small_queries_match_oracle(case, qp, k):
+    """The one-wave path against the oracle: probes, candidate sets bit-exact,
+    top-k within the score tolerance."""
+    X, Q, g, o = _built(case)
+    g.set_qProbes(qp)
+    o.set_qProbes(qp)
+    out = g.query(Q, k)
+    probes = g.last_probes(len(Q))
+    assert g.stats()["small_queries"] > 0
+    Xc = X - g.center()
+    for j, q in enumerate(Q):
+        d = o.query_debug(q, k)
+        assert np.array_equal(np.sort(probes[j]), np.sort(d["probes"].astype(np.uint32))), j
+        assert np.array_equal(g.last_candidates(j), d["cands"]), j
+        m = min(k, len(d["cands"]))
+        _check_topk(Xc, q, out[j, :m], d["topk"][:m], d["scores"][:m], k)
+        assert np.all(out[j, m:] == 0)
+
+
+@pytest.mark.parametrize("mode", [E.PREC_BF16, E.PREC_FP8])
+def test_small_queries_are_scored_exactly_in_every_precision(mode):
+    """A small query's candidates are all scored in FP32, whatever the
+    precision mode: with every query small the answer is the FP32 answer."""
+    X, Q, g, o = _built("mix_d128_p2")
+    ref, rsc = _run_single(g, Q, 6, 10)
+    assert g.stats()["small_queries"] == len(Q)
+    try:
+        g.set_precision(mode, 40)
+        got, gsc = _run_single(g, Q, 6, 10)
+    finally:
+        g.set_precision(E.PREC_FP32, 0)
+    assert g.stats()["small_queries"] == len(Q)
+    assert np.array_equal(got, ref) and np.array_equal(gsc, rsc)
+
+
 def test_fp8_rows_keep_their_own_scales_when_magnitudes_spread():
     # rows scaled by 2^-6 .. 2^6, with their negatives so that the centre is
     # about zero: the centred rows' largest magnitudes span more than 2^8, so
