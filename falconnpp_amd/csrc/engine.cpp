@@ -858,10 +858,10 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     sp.count = counters + 10;
     // 1. hashing, probe selection, exact replay of the flagged queries
     auto front = [&](bool ev) {
-        launch_pad_rows(dQ, ldq, nq, I.n_features, Qpad, I.ldx, st);
-        // bit 0: per-layer lists rebuilt by the replay; bit 1: probes replayed
-        launch_fill_u32(qflag, exact_only ? 3u : (probe_path == PROBE_EXACT ? 2u : 0u), nq, st);
-        launch_fill_u32(counters, 0u, N_COUNTERS, st);
+        // padded queries, the flags (bit 0: per-layer lists rebuilt by the
+        // replay; bit 1: probes replayed) and the counters, in one kernel
+        launch_pad_rows_init(dQ, ldq, nq, I.n_features, Qpad, I.ldx, qflag,
+                             exact_only ? 3u : (probe_path == PROBE_EXACT ? 2u : 0u), nq, counters, N_COUNTERS, st);
         if (ev) I.mark(0, st);
         if (!exact_only)
             launch_hash_query(hg, Qpad, I.ldx, nq, PL, I.masks.get(), qc, qv, qflag, st);
@@ -891,12 +891,20 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
             } else {
                 launch_ctot_keys(ctot, qflag, nq, okey, sp, st);
             }
-            uint32_t *sk = nullptr, *sv = nullptr;
-            radix_sort_pairs(okey, reinterpret_cast<uint32_t*>(W + lo.oval), reinterpret_cast<uint32_t*>(W + lo.okey2),
-                             reinterpret_cast<uint32_t*>(W + lo.oval2), nq, bits, true,
-                             reinterpret_cast<uint32_t*>(W + lo.ohist), reinterpret_cast<uint64_t*>(W + lo.ohist_off),
-                             reinterpret_cast<uint64_t*>(W + lo.oscan), &sk, &sv, st);
-            order = sv;
+            if (nq <= SORT1_MAX) {
+                // one workgroup, keys in LDS (one launch instead of five per pass)
+                uint32_t* sv = reinterpret_cast<uint32_t*>(W + lo.oval);
+                launch_sort1(okey, nq, bits, sv, st);
+                order = sv;
+            } else {
+                uint32_t *sk = nullptr, *sv = nullptr;
+                radix_sort_pairs(okey, reinterpret_cast<uint32_t*>(W + lo.oval),
+                                 reinterpret_cast<uint32_t*>(W + lo.okey2), reinterpret_cast<uint32_t*>(W + lo.oval2),
+                                 nq, bits, true, reinterpret_cast<uint32_t*>(W + lo.ohist),
+                                 reinterpret_cast<uint64_t*>(W + lo.ohist_off), reinterpret_cast<uint64_t*>(W + lo.oscan),
+                                 &sk, &sv, st);
+                order = sv;
+            }
         }
         GatherArgs a = ga;
         a.order = order;

@@ -223,9 +223,18 @@ void launch_order_keys(const int32_t* qc, const uint32_t* qflag, int nq, int L, 
                        const SmallPick& sp, hipStream_t st);
 void launch_ctot_keys(const uint64_t* ctot, const uint32_t* qflag, int nq, uint32_t* key, const SmallPick& sp,
                       hipStream_t st);
+// out[0 .. n) = 0 .. n-1 stably ordered by the low `bits` bits of keys, in
+// one workgroup (n <= SORT1_MAX; the same order as radix_sort_pairs).
+constexpr int SORT1_MAX = 11264;
+void launch_sort1(const uint32_t* keys, int n, int bits, uint32_t* out, hipStream_t st);
 // dst[r*ldd + c] = src[r*lds + c] for c < D, 0 up to ldd (queries padded to
 // the scored row stride).
 void launch_pad_rows(const float* src, int64_t lds, int64_t rows, int D, float* dst, int64_t ldd, hipStream_t st);
+// launch_pad_rows, and in the same kernel flags[0 .. nflags) = fv and
+// counters[0 .. ncounters) = 0.
+void launch_pad_rows_init(const float* src, int64_t lds, int64_t rows, int D, float* dst, int64_t ldd,
+                          uint32_t* flags, uint32_t fv, int nflags, uint32_t* counters, int ncounters,
+                          hipStream_t st);
 // p[0 .. n) = v (a kernel, so it can be captured in a graph).
 void launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
 // Workgroups of a persistent launch over nitems: every CU holds as many as

@@ -372,6 +372,97 @@ __global__ __launch_bounds__(PS_T) void k_probe_select(ProbeGeom g, const int32_
     }
 }
 
+// Probe selection with one probe per layer (perTable = 1, qProbes <= L / 2):
+// every table contributes its single best pair, so no table ever holds more
+// than perTable pairs at or above the cut and the answer is the global
+// top-qProbes of the L pair sums.  One wave per query, up to PS1_S tables per
+// lane in registers, ranks by counting over the wave: the same cut, tie flag,
+// probe set and C_tot as k_probe_select (the probes in table order).
+constexpr int PS1_T = 256;
+constexpr int PS1_S = 4;  // L <= 256
+__global__ __launch_bounds__(PS1_T) void k_probe_select_pl1(ProbeGeom g, int nq, const int32_t* __restrict__ qc,
+                                                            const float* __restrict__ qv,
+                                                            const uint64_t* __restrict__ bucket_off,
+                                                            uint32_t* __restrict__ probes, uint64_t* __restrict__ ctot,
+                                                            uint32_t* __restrict__ qflag) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int q = blockIdx.x * (PS1_T / WAVE) + (int)(threadIdx.x / WAVE);
+    if (q >= nq) return;  // wave-uniform
+    const QueryView qv_ = query_view(g.qp, q, g.qProbes, g.perTable, g.PL, 0);
+    const uint32_t QP = (uint32_t)qv_.qProbes;
+    // key of table l's pair: desc_key(v1 + v2) in the reference's order of
+    // addition (src/FalconnPP.cpp:478-485); 0xffffffff (above every key)
+    // past the last table
+    uint32_t key[PS1_S], bkt[PS1_S];
+#pragma unroll
+    for (int s = 0; s < PS1_S; ++s) {
+        const int l = s * WAVE + lane;
+        key[s] = 0xffffffffu;
+        bkt[s] = 0u;
+        if (l < g.L) {
+            const size_t base = ((size_t)q * g.L + l) * 2 * g.PL;
+            const float a = gld(qv + base), b = gld(qv + base + g.PL);
+            const int32_t c1 = gld(qc + base), c2 = gld(qc + base + g.PL);
+            key[s] = desc_key(a + b);
+            bkt[s] = (uint32_t)l * (uint32_t)g.NB + (uint32_t)c1 * (uint32_t)g.two_np + (uint32_t)c2;
+        }
+    }
+    // lt / le: keys below / at most each key
+    uint32_t lt[PS1_S], le[PS1_S];
+#pragma unroll
+    for (int s = 0; s < PS1_S; ++s) lt[s] = le[s] = 0u;
+    const int nslots = (g.L + WAVE - 1) / WAVE;
+    for (int s2 = 0; s2 < nslots; ++s2) {
+        for (int o = 0; o < WAVE; ++o) {
+            if (s2 * WAVE + o >= g.L) break;  // uniform
+            uint32_t kj = 0;
+#pragma unroll
+            for (int u = 0; u < PS1_S; ++u)
+                if (u == s2) kj = (uint32_t)__builtin_amdgcn_readlane((int)key[u], o);
+#pragma unroll
+            for (int s = 0; s < PS1_S; ++s) {
+                lt[s] += kj < key[s] ? 1u : 0u;
+                le[s] += kj <= key[s] ? 1u : 0u;
+            }
+        }
+    }
+    // the cut T: the key whose [lt, le) holds rank QP - 1; below it klt keys,
+    // on it eqc keys
+    uint32_t T = 0, klt = 0, eqc = 0;
+#pragma unroll
+    for (int s = 0; s < PS1_S; ++s) {
+        const bool cut = s * WAVE + lane < g.L && lt[s] < QP && QP <= le[s];
+        const uint64_t m = __ballot(cut);
+        if (m) {
+            const int src = (int)__ffsll((long long)m) - 1;
+            T = (uint32_t)__builtin_amdgcn_readlane((int)key[s], src);
+            klt = (uint32_t)__builtin_amdgcn_readlane((int)lt[s], src);
+            eqc = (uint32_t)__builtin_amdgcn_readlane((int)le[s], src) - klt;
+        }
+    }
+    const bool amb = eqc != QP - klt;
+    // every key above the cut and on it, in table order; the first QP kept
+    uint32_t* out = probes + qv_.probe_off;
+    uint32_t base = 0;
+    unsigned long long c = 0;
+#pragma unroll
+    for (int s = 0; s < PS1_S; ++s) {
+        const bool take = s * WAVE + lane < g.L && key[s] <= T;
+        const uint64_t m = __ballot(take);
+        const uint32_t pos = base + (uint32_t)__popcll(m & lanes_below_mask());
+        if (take && pos < QP) {
+            gst(out + pos, bkt[s]);
+            c += gld(bucket_off + bkt[s] + 1) - gld(bucket_off + bkt[s]);
+        }
+        base += (uint32_t)__popcll(m);
+    }
+    for (int off = 32; off > 0; off >>= 1) c += (unsigned long long)__shfl_xor((long long)c, off, WAVE);
+    if (lane == 0) {
+        gst(ctot + q, (uint64_t)c);
+        if (amb) atomicOr(qflag + q, 2u);
+    }
+}
+
 // Exact replay of the flagged queries' probing sequences in three kernels,
 // all sized by the device's flag count, not the host's:
 //  k_replay_tables   one wave per (flagged query f < cap, table): the table
@@ -844,17 +935,27 @@ __global__ __launch_bounds__(SMALL_T) void k_gather_small_f32(GatherArgs a, cons
             uint32_t mx = len;
             for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, WAVE));
             mx = (uint32_t)__builtin_amdgcn_readfirstlane((int)mx);
-            for (uint32_t j = 0; j < mx; j += 4) {
-                i32x4_a4 t = {0, 0, 0, 0};
-                if (j < len) t = gld_i32x4(ids + s + j);
+            // up to 32 ids per lane in flight before the first insert
+            for (uint32_t j0 = 0; j0 < mx; j0 += 32) {
+                i32x4_a4 t[8];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int32_t v = t[r];
-                    const bool fresh = j + r < len && hashset_insert(H, SMALL_HASH_LG, v);
-                    const uint64_t m = __ballot(fresh);
-                    const uint32_t pos = nc + lane_rank(m);
-                    if (fresh && pos < SMALL_MAX_CTOT) C[pos] = v;
-                    nc += (uint32_t)__popcll(m);
+                for (int g = 0; g < 8; ++g) {
+                    t[g] = i32x4_a4{0, 0, 0, 0};
+                    if (j0 + 4 * g < len) t[g] = gld_i32x4(ids + s + j0 + 4 * g);
+                }
+#pragma unroll
+                for (int g = 0; g < 8; ++g) {
+                    if (j0 + 4 * g >= mx) break;  // wave-uniform
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const uint32_t j = j0 + 4 * g + r;
+                        const int32_t v = t[g][r];
+                        const bool fresh = j < len && hashset_insert(H, SMALL_HASH_LG, v);
+                        const uint64_t m = __ballot(fresh);
+                        const uint32_t pos = nc + lane_rank(m);
+                        if (fresh && pos < SMALL_MAX_CTOT) C[pos] = v;
+                        nc += (uint32_t)__popcll(m);
+                    }
                 }
             }
         }
@@ -960,6 +1061,119 @@ __global__ void k_pad_rows(const float* __restrict__ src, int64_t lds, int64_t r
     }
 }
 
+// ---- query order: one-workgroup sort ----------------------------------------
+// out[0 .. n) = the positions 0 .. n-1 ordered by keys[], stably (the order
+// radix_sort_pairs gives), for n <= SORT1_MAX: LSD passes of 8 bits over the
+// low `bits` bits with the keys and 16-bit positions in LDS.  A pass: digit
+// histogram, its exclusive scan, then tiles of SORT1_T elements in order; in
+// a tile, eight ballots give each lane the lanes of its wave with the same
+// digit, the waves' counts per digit are scanned across the waves, and each
+// element lands at digit start + earlier tiles + earlier waves + its rank.
+constexpr int SORT1_T = 1024;
+constexpr int SORT1_WAVES = SORT1_T / WAVE;
+
+__global__ __launch_bounds__(SORT1_T) void k_sort1(const uint32_t* __restrict__ keys, int n, int bits,
+                                                   uint32_t* __restrict__ out) {
+    extern __shared__ uint32_t sm[];
+    __shared__ uint32_t start[256], run[256], tot[256];
+    uint32_t* k0 = sm;
+    uint32_t* k1 = k0 + n;
+    uint32_t* wcnt = k1 + n;  // SORT1_WAVES * 256
+    uint16_t* v0 = reinterpret_cast<uint16_t*>(wcnt + SORT1_WAVES * 256);
+    uint16_t* v1 = v0 + n;
+    const int t = threadIdx.x;
+    const int lane = t & (WAVE - 1);
+    const int wv = t / WAVE;
+    for (int i = t; i < n; i += SORT1_T) {
+        k0[i] = gld(keys + i);
+        v0[i] = (uint16_t)i;
+    }
+    __syncthreads();
+    for (int shift = 0; shift < bits; shift += 8) {
+        if (t < 256) start[t] = 0u;
+        __syncthreads();
+        for (int i = t; i < n; i += SORT1_T) atomicAdd(&start[(k0[i] >> shift) & 255u], 1u);
+        __syncthreads();
+        if (wv == 0) {
+            // exclusive scan of the 256 counts, four per lane
+            uint32_t c[4], s = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                c[j] = start[lane * 4 + j];
+                s += c[j];
+            }
+            const uint32_t ex = wave_incl_scan(s) - s;
+            uint32_t r = ex;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                start[lane * 4 + j] = r;
+                run[lane * 4 + j] = 0u;
+                r += c[j];
+            }
+        }
+        __syncthreads();
+        for (int i0 = 0; i0 < n; i0 += SORT1_T) {
+            const int i = i0 + t;
+            const bool valid = i < n;
+            const uint32_t key = valid ? k0[i] : 0u;
+            const uint32_t d = (key >> shift) & 255u;
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint64_t set = __ballot((d >> b) & 1u);
+                peers &= ((d >> b) & 1u) ? set : ~set;
+            }
+            const uint32_t rank = (uint32_t)__popcll(peers & lanes_below_mask());
+            for (int j = t; j < SORT1_WAVES * 256; j += SORT1_T) wcnt[j] = 0u;
+            __syncthreads();
+            if (valid && rank == 0) wcnt[wv * 256 + d] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            if (t < 256) {
+                uint32_t r = 0;
+                for (int w = 0; w < SORT1_WAVES; ++w) {
+                    const uint32_t c = wcnt[w * 256 + t];
+                    wcnt[w * 256 + t] = r;
+                    r += c;
+                }
+                tot[t] = r;
+            }
+            __syncthreads();
+            if (valid) {
+                const uint32_t pos = start[d] + run[d] + wcnt[wv * 256 + d] + rank;
+                k1[pos] = key;
+                v1[pos] = v0[i];
+            }
+            __syncthreads();
+            if (t < 256) run[t] += tot[t];
+        }
+        __syncthreads();
+        uint32_t* tk = k0;
+        k0 = k1;
+        k1 = tk;
+        uint16_t* tv = v0;
+        v0 = v1;
+        v1 = tv;
+    }
+    for (int i = t; i < n; i += SORT1_T) gst(out + i, (uint32_t)v0[i]);
+}
+
+// k_pad_rows plus the call's initial state in the same launch: flags[0 ..
+// nflags) = fv, counters[0 .. ncounters) = 0.
+__global__ void k_pad_rows_init(const float* __restrict__ src, int64_t lds, int64_t rows, int D,
+                                float* __restrict__ dst, int64_t ldd, uint32_t* __restrict__ flags, uint32_t fv,
+                                int nflags, uint32_t* __restrict__ counters, int ncounters) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = rows * ldd;
+    for (int64_t i = t0; i < total; i += stride) {
+        const int64_t r = i / ldd;
+        const int c = (int)(i - r * ldd);
+        dst[i] = c < D ? src[r * lds + c] : 0.0f;
+    }
+    for (int64_t i = t0; i < nflags; i += stride) flags[i] = fv;
+    if (t0 < ncounters) counters[t0] = 0u;
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1018,7 +1232,11 @@ void launch_probe_select(const ProbeGeom& g, int nq, const int32_t* qc, const fl
     const int path = probe_select_path(g);
     FPP_REQUIRE(path != PROBE_EXACT, FPP_EINTERNAL, "probe pool does not fit the LDS: replay the queries instead");
     if (nq == 0) return;
-    if (path == PROBE_FULL_POOL)
+    if (g.PL == 1 && g.L <= PS1_S * WAVE)
+        // one probe per layer for every query of the call: a wave per query
+        k_probe_select_pl1<<<(unsigned)ceil_div(nq, PS1_T / WAVE), PS1_T, 0, st>>>(g, nq, qc, qv, bucket_off, probes,
+                                                                                  ctot, qflag);
+    else if (path == PROBE_FULL_POOL)
         k_probe_select<true><<<nq, PS_T, probe_full_lds(g), st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
     else
         k_probe_select<false><<<nq, PS_T, probe_two_stage_lds(g), st>>>(g, qc, qv, bucket_off, probes, ctot, qflag);
@@ -1033,6 +1251,23 @@ void launch_order_keys(const int32_t* qc, const uint32_t* qflag, int nq, int L, 
     const int sbit = ilog2_exact((int64_t)two_np * two_np) + 1;
     FPP_REQUIRE(sbit >= 1 && sbit < 32, FPP_EINTERNAL, "order key width");
     k_order_keys<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(qc, qflag, nq, L, PL, two_np, key, sp, sbit);
+    FPP_HIP(hipGetLastError());
+}
+
+size_t sort1_lds_bytes(int n) { return (size_t)n * 12 + (size_t)SORT1_WAVES * 256 * 4; }
+
+void launch_sort1(const uint32_t* keys, int n, int bits, uint32_t* out, hipStream_t st) {
+    FPP_REQUIRE(n >= 1 && n <= SORT1_MAX && bits >= 1 && bits <= 32, FPP_EINTERNAL, "one-workgroup sort size");
+    k_sort1<<<1, SORT1_T, sort1_lds_bytes(n), st>>>(keys, n, bits, out);
+    FPP_HIP(hipGetLastError());
+}
+
+void launch_pad_rows_init(const float* src, int64_t lds, int64_t rows, int D, float* dst, int64_t ldd,
+                          uint32_t* flags, uint32_t fv, int nflags, uint32_t* counters, int ncounters,
+                          hipStream_t st) {
+    const int64_t work = std::max<int64_t>({rows * ldd, (int64_t)nflags, (int64_t)ncounters, 1});
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, 256), 4096));
+    k_pad_rows_init<<<grid, 256, 0, st>>>(src, lds, rows, D, dst, ldd, flags, fv, nflags, counters, ncounters);
     FPP_HIP(hipGetLastError());
 }
 

@@ -418,6 +418,34 @@ def test_small_queries_match_oracle(case, qp, k):
         assert np.all(out[j, m:] == 0)
 
 
+@pytest.mark.parametrize("case,qp,k", [("l100_qp10000", 40, 20), ("int_ties_l200", 60, 10),
+                                       ("int_grid_ties", 2, 10), ("full_pool_l64_pl18", 20, 20),
+                                       ("sphere_d100_p1", 5, 20)])
+def test_one_probe_per_layer_matches_oracle(case, qp, k):
+    """qProbes <= L / 2: one probe per layer, every table's best pair; the
+    one-wave selection (and the tie flag on integer data, whose queries the
+    exact replay then serves) against the oracle."""
+    X, Q, g, o = _built(case)
+    L = CASES[case][1]
+    assert math.ceil(2 * qp / L) == 1
+    g.set_qProbes(qp)
+    o.set_qProbes(qp)
+    out = g.query(Q, k)
+    probes = g.last_probes(len(Q))
+    Xc = X - g.center()
+    ctot = 0
+    for j, q in enumerate(Q):
+        d = o.query_debug(q, k)
+        assert np.array_equal(np.sort(probes[j]), np.sort(d["probes"].astype(np.uint32))), j
+        assert np.array_equal(g.last_candidates(j), d["cands"]), j
+        ctot += d["ctot"]
+        m = min(k, len(d["cands"]))
+        _check_topk(Xc, q, out[j, :m], d["topk"][:m], d["scores"][:m], k)
+    st = g.stats()
+    assert st["ctot"] == ctot
+    print(f"[parity] {case} qProbes={qp}: {st['query_ties']} of {len(Q)} queries replayed for ties", flush=True)
+
+
 @pytest.mark.parametrize("mode", [E.PREC_BF16, E.PREC_FP8])
 def test_small_queries_are_scored_exactly_in_every_precision(mode):
     """A small query's candidates are all scored in FP32, whatever the
