@@ -151,6 +151,17 @@ class Workload:
         self.idx.set_profiling(0)
         return ms
 
+    def stage_times(self, calls=3):
+        """Per-stage HIP-event times of `calls` more calls (after the timed
+        loop, which runs without events: reading them synchronises each step)."""
+        self.idx.set_profiling(1)
+        out = []
+        for _ in range(calls):
+            self.call()
+            out.append(self.idx.timings())
+        self.idx.set_profiling(0)
+        return out
+
     def phases(self):
         self.idx.set_profiling(2)
         self.call()
@@ -201,7 +212,8 @@ def run_precisions(w, args, config_key, with_truth):
     refine at the same qProbes."""
     E, c = w.E, w.cfg
     w.idx.set_qProbes(c["qProbes"])
-    qps, ms, stage = w.timed(args.steps, args.warmup, events=True)
+    qps, ms, _ = w.timed(args.steps, args.warmup)
+    stage = w.stage_times()
     w.mark_fp32_workspace()
     st = w.idx.stats()
     log(f"{w.name} fp32: {ms:.2f} ms/step, {qps:.0f} QPS, C_tot/q {st['ctot'] / w.nq:.0f}, "
@@ -519,17 +531,22 @@ def serve_batched(args, E, synth, torch):
     def timed_run():
         """Warm-up, the timed steps, then one more call with per-query
         stamps; returns (QPS, ms per step, stage ms, latency)."""
-        idx.set_profiling(1)
+        idx.set_profiling(0)
         for _ in range(max(1, args.warmup)):
             call()
         torch.cuda.synchronize()
-        stage = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             call()
-            stage.append(idx.timings())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        # stage times from two more calls with events (reading them
+        # synchronises, so the timed loop runs without)
+        idx.set_profiling(1)
+        stage = []
+        for _ in range(2):
+            call()
+            stage.append(idx.timings())
         stages = {key: round(float(np.mean([s[key] for s in stage])), 3)
                   for key in ("hash", "probe", "replay", "gather")}
         idx.set_profiling(2)

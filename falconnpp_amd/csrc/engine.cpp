@@ -883,20 +883,27 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         int nqueues = 1;
         if (order_mode != ORDER_IDENTITY && nq > 1) {
             uint32_t* okey = reinterpret_cast<uint32_t*>(W + lo.okey);
+            // key bits, and the smallest key of a small query (see SmallPick)
             int bits = 32;
+            uint32_t small_min = 0xffffffffu - SMALL_MAX_CTOT;
             if (order_mode == ORDER_BUCKET) {
                 launch_order_keys(qc, qflag, nq, L, PL, 2 * I.n_proj, okey, sp, st);
-                bits = ilog2_exact(I.NB()) + 1 + (small ? 1 : 0);
+                bits = ilog2_exact(I.NB()) + 1;
+                small_min = 1u << bits;
                 nqueues = nq >= 64 ? 8 : 1;
             } else {
                 launch_ctot_keys(ctot, qflag, nq, okey, sp, st);
             }
             if (nq <= SORT1_MAX) {
-                // one workgroup, keys in LDS (one launch instead of five per pass)
+                // one workgroup, keys in LDS (one launch instead of five per
+                // pass); the small queries are only moved to the end
                 uint32_t* sv = reinterpret_cast<uint32_t*>(W + lo.oval);
-                launch_sort1(okey, nq, bits, sv, st);
+                launch_sort1(okey, nq, bits, small, small_min, sv, st);
                 order = sv;
             } else {
+                // every key sorted; small queries' bucket keys carry the bit
+                // above the bucket's
+                if (small && order_mode == ORDER_BUCKET) bits += 1;
                 uint32_t *sk = nullptr, *sv = nullptr;
                 radix_sort_pairs(okey, reinterpret_cast<uint32_t*>(W + lo.oval),
                                  reinterpret_cast<uint32_t*>(W + lo.okey2), reinterpret_cast<uint32_t*>(W + lo.oval2),

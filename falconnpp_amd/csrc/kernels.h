@@ -224,9 +224,12 @@ void launch_order_keys(const int32_t* qc, const uint32_t* qflag, int nq, int L, 
 void launch_ctot_keys(const uint64_t* ctot, const uint32_t* qflag, int nq, uint32_t* key, const SmallPick& sp,
                       hipStream_t st);
 // out[0 .. n) = 0 .. n-1 stably ordered by the low `bits` bits of keys, in
-// one workgroup (n <= SORT1_MAX; the same order as radix_sort_pairs).
+// one workgroup (n <= SORT1_MAX; the order radix_sort_pairs gives).  With
+// split, the positions whose key is >= small_min (small queries) come last in
+// position order, and only the others are sorted.
 constexpr int SORT1_MAX = 11264;
-void launch_sort1(const uint32_t* keys, int n, int bits, uint32_t* out, hipStream_t st);
+void launch_sort1(const uint32_t* keys, int n, int bits, bool split, uint32_t small_min, uint32_t* out,
+                  hipStream_t st);
 // dst[r*ldd + c] = src[r*lds + c] for c < D, 0 up to ldd (queries padded to
 // the scored row stride).
 void launch_pad_rows(const float* src, int64_t lds, int64_t rows, int D, float* dst, int64_t ldd, hipStream_t st);

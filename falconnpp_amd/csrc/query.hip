@@ -1062,34 +1062,65 @@ __global__ void k_pad_rows(const float* __restrict__ src, int64_t lds, int64_t r
 }
 
 // ---- query order: one-workgroup sort ----------------------------------------
-// out[0 .. n) = the positions 0 .. n-1 ordered by keys[], stably (the order
-// radix_sort_pairs gives), for n <= SORT1_MAX: LSD passes of 8 bits over the
-// low `bits` bits with the keys and 16-bit positions in LDS.  A pass: digit
-// histogram, its exclusive scan, then tiles of SORT1_T elements in order; in
-// a tile, eight ballots give each lane the lanes of its wave with the same
-// digit, the waves' counts per digit are scanned across the waves, and each
-// element lands at digit start + earlier tiles + earlier waves + its rank.
+// out[0 .. n) = the positions 0 .. n-1 ordered by keys[], stably, for n <=
+// SORT1_MAX; with split, the positions whose key is >= small_min (the small
+// queries) come last in position order and only the others are sorted.  LSD
+// passes of 8 bits over the low `bits` bits with the keys and 16-bit positions
+// in LDS.  A pass: digit histogram, its exclusive scan, then tiles of SORT1_T
+// elements in order; in a tile, eight ballots give each lane the lanes of its
+// wave with the same digit, the waves' counts per digit are scanned across
+// the waves, and each element lands at digit start + earlier tiles + earlier
+// waves + its rank.
 constexpr int SORT1_T = 1024;
 constexpr int SORT1_WAVES = SORT1_T / WAVE;
 
-__global__ __launch_bounds__(SORT1_T) void k_sort1(const uint32_t* __restrict__ keys, int n, int bits,
-                                                   uint32_t* __restrict__ out) {
+__global__ __launch_bounds__(SORT1_T) void k_sort1(const uint32_t* __restrict__ keys, int n_all, int bits, int split,
+                                                   uint32_t small_min, uint32_t* __restrict__ out) {
     extern __shared__ uint32_t sm[];
     __shared__ uint32_t start[256], run[256], tot[256];
+    __shared__ uint32_t wb[SORT1_WAVES], ws[SORT1_WAVES];
     uint32_t* k0 = sm;
-    uint32_t* k1 = k0 + n;
-    uint32_t* wcnt = k1 + n;  // SORT1_WAVES * 256
+    uint32_t* k1 = k0 + n_all;
+    uint32_t* wcnt = k1 + n_all;  // SORT1_WAVES * 256
     uint16_t* v0 = reinterpret_cast<uint16_t*>(wcnt + SORT1_WAVES * 256);
-    uint16_t* v1 = v0 + n;
+    uint16_t* v1 = v0 + n_all;
     const int t = threadIdx.x;
     const int lane = t & (WAVE - 1);
     const int wv = t / WAVE;
-    for (int i = t; i < n; i += SORT1_T) {
-        k0[i] = gld(keys + i);
-        v0[i] = (uint16_t)i;
+    // the others to k0 / v0, the small ones' positions to v1, each in order
+    int n = 0, ns = 0;
+    for (int i0 = 0; i0 < n_all; i0 += SORT1_T) {
+        const int i = i0 + t;
+        const uint32_t key = i < n_all ? gld(keys + i) : 0u;
+        const bool sml = i < n_all && split && key >= small_min;
+        const bool oth = i < n_all && !sml;
+        const uint64_t mo = __ballot(oth), msm = __ballot(sml);
+        if (lane == 0) {
+            wb[wv] = (uint32_t)__popcll(mo);
+            ws[wv] = (uint32_t)__popcll(msm);
+        }
+        __syncthreads();
+        uint32_t ob = 0, os = 0, tb = 0, tsm = 0;
+        for (int w = 0; w < SORT1_WAVES; ++w) {
+            ob += w < wv ? wb[w] : 0u;
+            os += w < wv ? ws[w] : 0u;
+            tb += wb[w];
+            tsm += ws[w];
+        }
+        const uint64_t below = lanes_below_mask();
+        if (oth) {
+            const uint32_t p = (uint32_t)n + ob + (uint32_t)__popcll(mo & below);
+            k0[p] = key;
+            v0[p] = (uint16_t)i;
+        }
+        if (sml) v1[ns + os + (uint32_t)__popcll(msm & below)] = (uint16_t)i;
+        n += (int)tb;
+        ns += (int)tsm;
+        __syncthreads();
     }
+    for (int j = t; j < ns; j += SORT1_T) gst(out + n + j, (uint32_t)v1[j]);
     __syncthreads();
-    for (int shift = 0; shift < bits; shift += 8) {
+    for (int shift = 0; n > 1 && shift < bits; shift += 8) {
         if (t < 256) start[t] = 0u;
         __syncthreads();
         for (int i = t; i < n; i += SORT1_T) atomicAdd(&start[(k0[i] >> shift) & 255u], 1u);
@@ -1256,9 +1287,10 @@ void launch_order_keys(const int32_t* qc, const uint32_t* qflag, int nq, int L, 
 
 size_t sort1_lds_bytes(int n) { return (size_t)n * 12 + (size_t)SORT1_WAVES * 256 * 4; }
 
-void launch_sort1(const uint32_t* keys, int n, int bits, uint32_t* out, hipStream_t st) {
+void launch_sort1(const uint32_t* keys, int n, int bits, bool split, uint32_t small_min, uint32_t* out,
+                  hipStream_t st) {
     FPP_REQUIRE(n >= 1 && n <= SORT1_MAX && bits >= 1 && bits <= 32, FPP_EINTERNAL, "one-workgroup sort size");
-    k_sort1<<<1, SORT1_T, sort1_lds_bytes(n), st>>>(keys, n, bits, out);
+    k_sort1<<<1, SORT1_T, sort1_lds_bytes(n), st>>>(keys, n, bits, split ? 1 : 0, small_min, out);
     FPP_HIP(hipGetLastError());
 }
 
