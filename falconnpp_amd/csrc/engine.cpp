@@ -594,7 +594,7 @@ struct QueryLayout {
         counters, rwork, phase_ts, cand, part, total;
 };
 // [0, 8) queue heads, [8] replay claim, [9] flagged queries, [10] small
-// queries, [11] small-query queue head
+// queries
 constexpr int N_COUNTERS = 16;
 
 QueryLayout query_layout(int nq, int L, int PL, uint64_t nprobes, bool batched, unsigned rareas, int64_t wpq,

@@ -6,8 +6,8 @@
 // the top-p |projections| among the first n_proj coordinates.
 //
 // Layout: a hash lives in LPH = F/E adjacent lanes, each holding E consecutive
-// coordinates in VGPRs: E = min(F, 128) for the build, and 64 for the query
-// at F = 128 and 256 (query_lane_coords).  The butterflies run in the reference's
+// coordinates in VGPRs: E = min(F, 128) for the build; for the query see
+// query_lane_coords.  The butterflies run in the reference's
 // ascending-stride order (FFHT == ascending radix-2 butterflies, bit for bit):
 // strides 1..E/2 inside the lane, strides E..F/2 across lanes (DPP quad_perm
 // for lane-xor 1 and 2, ds_bpermute beyond).  The 4 waves of a workgroup share
@@ -35,10 +35,15 @@ namespace hashk {
 
 constexpr int HB_THREADS = 256;
 
-// Coordinates per lane of the query hash: 64 at F = 128 and 256 (two and
-// four lanes per hash; half the registers of 128 per lane, so more waves per
-// SIMD), else as the build.
-constexpr int query_lane_coords(int F) { return (F == 128 || F == 256) ? 64 : (F < 128 ? F : 128); }
+// Coordinates per lane of the query hash.  One lane per hash at F <= 128
+// when every coordinate is selected (FULL): no cross-lane butterflies, and a
+// sign flip is one XOR with a wave-uniform word, about a quarter fewer VALU
+// instructions than two lanes.  64 at F = 256, and at F = 128 when
+// coordinates past n_proj are masked or the top list is longer than 24 (128
+// per lane would spill there).
+constexpr int query_lane_coords(int F, bool full, int pm) {
+    return F == 256 || (F == 128 && (!full || pm > 24)) ? 64 : (F < 128 ? F : 128);
+}
 
 template <int S>
 __device__ __forceinline__ float xshfl(float v) {
@@ -347,7 +352,7 @@ __global__ __launch_bounds__(HB_THREADS) __attribute__((amdgpu_waves_per_eu(FULL
                                                             int nq, int PL, const uint32_t* __restrict__ masks,
                                                             int32_t* __restrict__ qc, float* __restrict__ qv,
                                                             uint32_t* __restrict__ qflag) {
-    constexpr int E = query_lane_coords(F);
+    constexpr int E = query_lane_coords(F, FULL, PM);
     constexpr int LPH = F / E;
     constexpr int HPW = WAVE / LPH;
     constexpr int MW = (F + 31) / 32;
@@ -413,7 +418,7 @@ void launch_build_variant(const HashGeom& g, const float* Xc, int64_t ldx, int64
 template <int F, int PM, bool FULL>
 void launch_query_variant(const HashGeom& g, const float* Q, int64_t ldq, int nq, int PL, const uint32_t* masks,
                           int32_t* qc, float* qv, uint32_t* qflag, hipStream_t st) {
-    constexpr int E = query_lane_coords(F);
+    constexpr int E = query_lane_coords(F, FULL, PM);
     constexpr int HPW = WAVE / (F / E);
     constexpr int NWV = HB_THREADS / WAVE;
     // split the tables over workgroups until about four waves per SIMD of
@@ -458,7 +463,9 @@ void dispatch_query_f(const HashGeom& g, const float* Q, int64_t ldq, int nq, in
         launch_query_variant<F, PMV, true>(g, Q, ldq, nq, PL, masks, qc, qv, qflag, st);    \
     else                                                                                    \
         launch_query_variant<F, PMV, false>(g, Q, ldq, nq, PL, masks, qc, qv, qflag, st);
-    if (PL + 1 <= 4) {
+    if (PL + 1 <= 2) {
+        FPP_V(2)
+    } else if (PL + 1 <= 4) {
         FPP_V(4)
     } else if (PL + 1 <= 8) {
         FPP_V(8)

@@ -183,8 +183,8 @@ struct GatherArgs {
     // work queue: order[0 .. nitems) (identity when null) in nqueues runs;
     // heads[nqueues] zeroed beforehand
     // small_at >= 0: the last heads[small_at] items of order are small
-    // queries, served one per wave by launch_gather_small (whose queue head is
-    // heads[small_at + 1]); the workgroup kernels take the items before them
+    // queries, served one per wave by launch_gather_small; the workgroup
+    // kernels take the items before them
     const uint32_t* order = nullptr;
     uint32_t nitems = 0;
     int nqueues = 1;
@@ -269,7 +269,7 @@ int64_t gather_window_bits(int64_t N, int64_t extra_words, int per_cu);
 // gather_f32_lds); a.ncand[q] = unique candidates.
 size_t gather_f32_lds(const GatherArgs& a, int threads);
 // The small queries of a call (the end of a.order, a.small_at), one wave
-// each, persistent over the queue head heads[small_at + 1]: candidates
+// each, wave g of the launch taking items g, g + waves, ...: candidates
 // deduped in the wave's LDS, FP32 scores in the gather kernel's order of
 // addition, top-k in the wave.  a.ncand[q] = unique candidates; with
 // a.seg_off the candidates are also kept at a.cand + seg_off[q].

@@ -901,9 +901,12 @@ __global__ __launch_bounds__(SMALL_T) void k_gather_small_f32(GatherArgs a, cons
     int32_t* C = cl[wv];
     for (int e = lane; e < (1 << SMALL_HASH_LG); e += WAVE) H[e] = 0u;
     wave_lds_sync();
-    uint32_t* head = a.heads + a.small_at + 1;
     const uint32_t nsmall = __hip_atomic_load(a.heads + a.small_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t begin = a.nitems - nsmall;
+    // small queries cost about the same: wave g takes items g, g + waves, ...
+    // (a shared claim counter serialised thousands of waves on one atomic)
+    const uint32_t gw = blockIdx.x * SMALL_WAVES + (uint32_t)wv;
+    const uint32_t nw = gridDim.x * SMALL_WAVES;
     const int g8 = lane & 7;
     const int grp = lane >> 3;
     const int64_t nchunk = a.ldx / 4;
@@ -911,11 +914,7 @@ __global__ __launch_bounds__(SMALL_T) void k_gather_small_f32(GatherArgs a, cons
     // so that four waves per SIMD fit at d <= 256
     constexpr int U = CPL >= 16 ? 1 : 2;
     const int ldo = a.ldo ? a.ldo : a.k;
-    for (;;) {
-        uint32_t it = 0;
-        if (lane == 0) it = atomicAdd(head, 1u);
-        it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it);
-        if (it >= nsmall) break;
+    for (uint32_t it = gw; it < nsmall; it += nw) {
         const int q = (int)gld(a.order + begin + it);
         uint64_t* ts = a.phase_ts ? a.phase_ts + (size_t)q * PHASE_STAMPS : nullptr;
         if (ts && lane == 0) ts[0] = wall_clock64();
