@@ -76,6 +76,40 @@ template <class T>
 __device__ __forceinline__ void gst(T* p, T v) {
     *(FPP_GLOBAL T*)p = v;
 }
+// LDS through pointers the compiler cannot prove to be LDS (parameters of a
+// function that is not inlined).  A generic pointer compiles to flat
+// accesses, which count on the vector memory counter as well as the LDS one,
+// so every wait for such an access also waits for the global loads in
+// flight.  lds_ptr asserts that p points into LDS, which lets the compiler
+// turn most accesses through p into ds_* instructions; the lds_* accesses
+// below go through address space 3 explicitly.  p must point into LDS.
+#define FPP_LDS __attribute__((address_space(3)))
+template <class T>
+__device__ __forceinline__ T* lds_ptr(T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_assume(__builtin_amdgcn_is_shared((const void*)p));
+#endif
+    return p;
+}
+template <class T>
+__device__ __forceinline__ T lds_ld(const T* p) {
+    return *(const FPP_LDS T*)p;
+}
+template <class T>
+__device__ __forceinline__ void lds_st(T* p, T v) {
+    *(FPP_LDS T*)p = v;
+}
+__device__ __forceinline__ uint32_t lds_add(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add((FPP_LDS uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t lds_or(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_or((FPP_LDS uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t lds_cas(uint32_t* p, uint32_t expected, uint32_t v) {
+    __hip_atomic_compare_exchange_strong((FPP_LDS uint32_t*)p, &expected, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    return expected;
+}
 // Four consecutive int32 at a 4-byte aligned address, one load.
 typedef int32_t i32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 __device__ __forceinline__ i32x4_a4 gld_i32x4(const int32_t* p) {

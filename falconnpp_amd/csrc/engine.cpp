@@ -94,7 +94,7 @@ struct Index {
     size_t ws_budget = size_t(8) << 30;
     DevBuf<int32_t> out_ids;  // host-pointer query calls
     std::vector<QueryParams> hqp;  // per-query probe counts and k (batched serving)
-    int gather_threads = 512;
+    int gather_threads = 0;        // 0: chosen per call (gather_workgroup_threads)
     int order_mode = 1;            // ORDER_* below
     bool keep_candidates = false;  // every query's candidates kept after the call (inspection)
     int replay_mode = 0;           // tie replay: 0 per call, 1 separate kernels, 2 inside the gather
@@ -634,6 +634,17 @@ QueryLayout query_layout(int nq, int L, int PL, uint64_t nprobes, bool batched, 
     return w;
 }
 
+// Workgroup size of the gather: the tuned one, or automatically 256 for the
+// FP32 gather when even a workgroup alone on its CU needs several dedup
+// windows (two workgroups per CU then overlap one query's candidate
+// collection with another's scoring, which outweighs their smaller windows),
+// else 512.
+int gather_workgroup_threads(const Index& I, bool lowp) {
+    if (I.gather_threads) return I.gather_threads;
+    if (lowp) return 512;
+    return I.n_points > gather_window_bits(I.n_points, 512 + 256 + 2048, 1) ? 256 : 512;
+}
+
 // One batch of queries.  k is the number of neighbours, or with per-query
 // counts (h_qprobes, h_k: host arrays of nq) the largest k and the row stride
 // of the outputs.  Everything runs on st in one sequence with no host read:
@@ -711,7 +722,8 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     }
     // 256-thread workgroups run two per CU (dedup window sized for half the
     // LDS), larger ones one per CU
-    const int gthreads = lowp ? (I.gather_threads <= 256 ? 256 : 512) : I.gather_threads;
+    const int tsel = gather_workgroup_threads(I, lowp);
+    const int gthreads = lowp ? (tsel <= 256 ? 256 : 512) : tsel;
     GatherArgs ga;
     ga.N = I.n_points;
     ga.D = I.n_features;
@@ -721,7 +733,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     if (batched) ga.ldo = k;
     ga.window_bits = gather_window_bits(I.n_points,
                                         lowp ? lowprec_extra_words(I.refine_count(k), gthreads)
-                                             : (int64_t)I.gather_threads + 256 + 2048,
+                                             : (int64_t)tsel + 256 + 2048,
                                         gthreads <= 256 ? 2 : 1);
     LowPrecArgs lp;
     if (lowp) {
@@ -751,11 +763,20 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         std::max(lowp ? gather_lp_lds(ga, lp) : gather_f32_lds(ga, gthreads), in_gather ? rlds : size_t(0));
     unsigned grid = persistent_grid(nq, glds, gthreads);
     // per-workgroup scratch: the unique candidates (at most min(C_tot, N)) and
-    // the window scatter (C_tot with several dedup windows) or score area
+    // the window scatter or score area: with several dedup windows, a bin of
+    // C_tot per window when the budget holds them for every workgroup (the
+    // ids are then scattered by one walk over the buckets instead of two),
+    // else C_tot
     const uint64_t bound = std::max<uint64_t>(1, ctot_bound(I, (uint64_t)qp_max));
     const bool one_window = I.n_points <= ga.window_bits;
     const uint64_t slot_cand = std::min<uint64_t>(bound, (uint64_t)I.n_points);
-    const uint64_t slot_part = one_window ? slot_cand : bound;
+    uint64_t slot_part = one_window ? slot_cand : bound;
+    if (!one_window && !fixed && !keep) {
+        const uint64_t nwin = (uint64_t)window_count(I.n_points, ga.window_bits);
+        const uint64_t binned = nwin * bound;
+        if ((size_t)grid * (slot_cand + binned) * 4 + (in_gather ? (size_t)grid * wpq : 0) <= I.ws_budget)
+            slot_part = binned;
+    }
     // separate replay kernels: one work area for each of the first rcap
     // flagged queries, and rover persistent workgroups for any beyond
     unsigned rcap = in_gather ? 0u : (unsigned)std::min(nq, 512);
@@ -1815,8 +1836,8 @@ int fpp_set_tuning(fpp_index* h, int which, int value) {
                 h->I.order_mode = value;
                 break;
             case FPP_TUNE_GATHER_THREADS:
-                FPP_REQUIRE(value == 256 || value == 512 || value == 1024, FPP_EINVAL,
-                            "gather threads must be 256, 512 or 1024");
+                FPP_REQUIRE(value == 0 || value == 256 || value == 512 || value == 1024, FPP_EINVAL,
+                            "gather threads must be 0 (automatic), 256, 512 or 1024");
                 h->I.gather_threads = value;
                 break;
             case FPP_TUNE_PERSISTENT:

@@ -13,13 +13,19 @@ namespace gather {
 // ---- thread groups ---------------------------------------------------------
 // The group-level helpers below run on a group of threads G: G::size threads,
 // G::tid() in [0, size), G::sync() a barrier of the group that also orders
-// its LDS and global memory accesses.  The gather kernels use the whole
-// workgroup.
+// its LDS and global memory accesses, G::lds_sync() one that orders its LDS
+// accesses only (global loads and stores in flight stay in flight across
+// it).  The gather kernels use the whole workgroup.
 template <int TB>
 struct Whole {
     static constexpr int size = TB;
     __device__ static __forceinline__ int tid() { return (int)threadIdx.x; }
     __device__ static __forceinline__ void sync() { __syncthreads(); }
+    __device__ static __forceinline__ void lds_sync() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    }
 };
 
 // ---- block / wave helpers -------------------------------------------------
@@ -122,7 +128,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 // per step of a wave (wave v takes groups v, v + NW, ...); step(f) calls
 // f(id, true) for every id of its group, with all lanes of the wave present
 // (lanes without an id pass false), in no particular order.  The next group's
-// buckets are fetched while the current ones are walked.
+// bucket extents are fetched while the current ids are walked, and its
+// probes one step before that, so a step waits for one round of loads.
 //  * Lane per bucket (the longest of the 64 buckets holds at most LANE_MAX
 //    ids): lane p reads bucket p's ids itself, ULANE at a time, so nothing
 //    crosses lanes and a step is a load and f.  Lanes whose bucket is done
@@ -141,9 +148,10 @@ struct BucketWalk {
     const uint64_t* __restrict__ bucket_off;
     const int32_t* __restrict__ ids;
     int np;
-    int c0;
-    uint64_t s;
+    int c0;         // first probe of this step's group
+    uint64_t s;     // this lane's bucket in the group: first id, length
     uint32_t len;
+    uint32_t nb;    // this lane's bucket in the next group
 
     __device__ __forceinline__ void init(const uint32_t* pr_, const uint64_t* off_, const int32_t* ids_, int np_) {
         pr = pr_;
@@ -154,6 +162,8 @@ struct BucketWalk {
         c0 = (G::tid() / WAVE) * WAVE;
         s = 0;
         len = 0;
+        const int cn = c0 + NW * WAVE;
+        nb = cn + lane < np ? gld(pr + cn + lane) : 0u;
         if (c0 + lane < np) {
             const uint32_t b = gld(pr + c0 + lane);
             s = gld(bucket_off + b);
@@ -163,21 +173,39 @@ struct BucketWalk {
     // wave-uniform
     __device__ __forceinline__ bool done() const { return c0 >= np; }
 
+    // Issues the loads of the next group's bucket extents and of the probe
+    // of the group after it; nothing waits for them until the next step.
+    __device__ __forceinline__ void fetch_next(uint64_t& s_n, uint32_t& len_n, uint32_t& nb_n) const {
+        const int lane = threadIdx.x & (WAVE - 1);
+        const int cn = c0 + NW * WAVE;
+        const int cnn = cn + NW * WAVE;
+        s_n = 0;
+        len_n = 0;
+        if (cn + lane < np) {
+            s_n = gld(bucket_off + nb);
+            len_n = (uint32_t)(gld(bucket_off + nb + 1) - s_n);
+        }
+        nb_n = cnn + lane < np ? gld(pr + cnn + lane) : 0u;
+    }
+
     template <class F>
     __device__ __forceinline__ void step(F&& f) {
         const int lane = threadIdx.x & (WAVE - 1);
-        const int cn = c0 + NW * WAVE;
-        const uint32_t nb = cn + lane < np ? gld(pr + cn + lane) : 0u;
         uint32_t mx = len;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, WAVE));
         mx = __builtin_amdgcn_readfirstlane(mx);
-        if (mx <= LANE_MAX) {
+        uint64_t s_n;
+        uint32_t len_n, nb_n;
+        if (mx == 0) {
+            fetch_next(s_n, len_n, nb_n);
+        } else if (mx <= LANE_MAX) {
             // four ids per load (the id array is padded past its end), so a
             // step issues a quarter of the load instructions, each touching
-            // one line per lane
+            // one line per lane; the first round's loads go out before the
+            // next group's extents, so f waits only for its own ids
             static_assert(ULANE % 4 == 0, "ids are read four at a time");
-            for (uint32_t j0 = 0; j0 < mx; j0 += ULANE) {
+            for (uint32_t j0 = 0;; j0 += ULANE) {
                 int32_t v[ULANE];
 #pragma unroll
                 for (int r = 0; r < ULANE; r += 4) {
@@ -188,10 +216,13 @@ struct BucketWalk {
                     v[r + 2] = t[2];
                     v[r + 3] = t[3];
                 }
+                if (j0 == 0) fetch_next(s_n, len_n, nb_n);
 #pragma unroll
                 for (int r = 0; r < ULANE; ++r) f(v[r], j0 + r < len);
+                if (j0 + ULANE >= mx) break;
             }
         } else {
+            fetch_next(s_n, len_n, nb_n);
             const uint32_t incl = wave_incl_scan(len);
             const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
             const uint64_t sx = s - (uint64_t)(incl - len);  // ids index of flat position f is sx + f
@@ -212,13 +243,10 @@ struct BucketWalk {
                 for (int r = 0; r < UNR; ++r) f(v[r], f0 + r * WAVE + lane < total);
             }
         }
-        c0 = cn;
-        s = 0;
-        len = 0;
-        if (cn + lane < np) {
-            s = gld(bucket_off + nb);
-            len = (uint32_t)(gld(bucket_off + nb + 1) - s);
-        }
+        c0 += NW * WAVE;
+        s = s_n;
+        len = len_n;
+        nb = nb_n;
     }
 };
 
@@ -231,7 +259,6 @@ __device__ __forceinline__ void for_each_bucket_id(const uint32_t* __restrict__ 
     w.init(pr, bucket_off, ids, np);
     while (!w.done()) w.step(f);
 }
-
 // Next query of a persistent gather workgroup, or -1 when every run of the
 // work queue is drained.  order[0 .. n) (identity when null; n = nitems less
 // the small queries at the end when small_at is set) is split
@@ -349,7 +376,7 @@ __device__ __forceinline__ void wave_append(bool take, int32_t v, uint32_t* coun
     const int lane = threadIdx.x & (WAVE - 1);
     const int leader = (int)__ffsll((long long)m) - 1;
     uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    if (lane == leader) base = lds_add(count, (uint32_t)__popcll(m));
     base = __shfl(base, leader, WAVE);
     const uint32_t r = lane_rank(m);
     if (take && base + r < ccap) {
@@ -360,10 +387,36 @@ __device__ __forceinline__ void wave_append(bool take, int32_t v, uint32_t* coun
     }
 }
 
+// wave_append of N (take, v) pairs per lane with one atomic per wave: the
+// pairs of round r follow those of rounds < r.  cq is in global memory.
+template <int N>
+__device__ __forceinline__ void wave_append_n(const bool (&take)[N], const int32_t (&v)[N], uint32_t* count,
+                                              int32_t* __restrict__ cq, uint64_t ccap) {
+    uint64_t m[N];
+    uint32_t pre[N];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        m[r] = __ballot(take[r]);
+        pre[r] = tot;
+        tot += (uint32_t)__popcll(m[r]);
+    }
+    if (tot == 0) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    uint32_t base = 0;
+    if (lane == 0) base = lds_add(count, tot);
+    base = __shfl(base, 0, WAVE);
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        const uint32_t pos = base + pre[r] + lane_rank(m[r]);
+        if (take[r] && pos < ccap) gst(cq + pos, v[r]);
+    }
+}
+
 // Test-and-set of id rel in the LDS bitset: true when the bit was clear.
 __device__ __forceinline__ bool bitset_insert(uint32_t* bits, int64_t rel) {
     const uint32_t m = 1u << (rel & 31);
-    return (atomicOr(&bits[rel >> 5], m) & m) == 0u;
+    return (lds_or(&bits[rel >> 5], m) & m) == 0u;
 }
 
 // Insert of id v into an LDS open-addressing set of 2^lg words (word 0 =
@@ -375,7 +428,7 @@ __device__ __forceinline__ bool hashset_insert(uint32_t* T, int lg, int32_t v) {
     const uint32_t key = (uint32_t)v + 1u;
     uint32_t h = ((uint32_t)v * 0x9E3779B1u) >> (32 - lg);
     for (uint32_t n = 0; n <= mask; ++n) {
-        const uint32_t old = atomicCAS(&T[h], 0u, key);
+        const uint32_t old = lds_cas(&T[h], 0u, key);
         if (old == 0u) return true;
         if (old == key) return false;
         h = (h + 1u) & mask;
@@ -386,7 +439,7 @@ __device__ __forceinline__ bool hashset_insert(uint32_t* T, int lg, int32_t v) {
 // Zeroes words [0, n) of the dedup area.
 template <class G>
 __device__ __forceinline__ void clear_words(uint32_t* bits, uint32_t n) {
-    for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) bits[i] = 0u;
+    for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) lds_st(bits + i, 0u);
 }
 
 // Collect the unique candidates of one query into cq (capacity ccap, at
@@ -401,11 +454,12 @@ __device__ __forceinline__ void clear_words(uint32_t* bits, uint32_t n) {
 //    Cost follows C_tot, not N.
 //  * N fits one window: ids are ORed in and the set bits compacted in id
 //    order, so cq is ascending; the area is left dirty.
-//  * Otherwise (W is then a power of two): two passes over the bucket ids
-//    count and then scatter them by window into part (capacity pcap, at
-//    least C_tot: duplicates included); each window's segment goes through
-//    a test-and-set, first occurrences appended, and the words it touched
-//    are zeroed again.
+//  * Otherwise (W is then a power of two): the bucket ids are scattered by
+//    window into part (capacity pcap, at least C_tot: duplicates
+//    included): into a bin of C_tot ids per window by one walk when pcap
+//    holds them all, else into segments laid out by a first, counting walk;
+//    each window's ids go through a test-and-set, first occurrences
+//    appended, and the words it touched are zeroed again.
 // Callers rank by (score, id), which does not depend on the order of cq.
 // wc holds window_counter_words(N, W) words, sh at least G::size words.
 template <class G>
@@ -414,6 +468,9 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
                                        uint32_t* bits, uint32_t* wc, uint32_t* sh, int32_t* __restrict__ cq,
                                        int32_t* __restrict__ part, uint64_t ccap, uint64_t pcap, uint64_t* ts,
                                        uint64_t ctot_q, uint32_t& dirty) {
+    bits = lds_ptr(bits);
+    wc = lds_ptr(wc);
+    sh = lds_ptr(sh);
     const int64_t W = a.window_bits;
     const uint32_t area = (uint32_t)(W / 32);
     // the largest power of two words in the area
@@ -424,14 +481,14 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         const uint32_t words = 1u << lg;
         clear_words<G>(bits, min(dirty, words));
         uint32_t* count = sh;
-        if (G::tid() == 0) *count = 0u;
+        if (G::tid() == 0) lds_st(count, 0u);
         G::sync();
         for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
             wave_append<true>(ok && hashset_insert(bits, lg, v), v, count, cq, ccap);
         });
         G::sync();
         if (ts && G::tid() == 0) ts[1] = wall_clock64();
-        const uint32_t nc = *count;
+        const uint32_t nc = lds_ld(count);
         clear_words<G>(bits, words);
         dirty = dirty > words ? dirty : 0u;
         G::sync();
@@ -444,7 +501,7 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         // 24 ids per lane in flight: one step covers a group of buckets of
         // the usual sizes (the bucket filter keeps them near bucketLimit)
         for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
-            if (ok) atomicOr(&bits[v >> 5], 1u << (v & 31));
+            if (ok) lds_or(&bits[v >> 5], 1u << (v & 31));
         });
         G::sync();
         if (ts && G::tid() == 0) ts[1] = wall_clock64();
@@ -461,77 +518,122 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         wfill[i] = 0u;
     }
     G::sync();
-    // 24 ids per lane in flight (a bucket of the usual size in one round of
-    // loads), as in the other paths
-    for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
-        if (ok) atomicAdd(&wcnt[v >> shift], 1u);
-    });
-    G::sync();
-    {
-        // exclusive scan of the window counts: thread t sums a contiguous run
-        const int per = (nwin + G::size - 1) / G::size;
-        const int b0 = G::tid() * per;
-        uint32_t c = 0;
-        for (int i = 0; i < per; ++i)
-            if (b0 + i < nwin) c += wcnt[b0 + i];
-        uint32_t tot;
-        uint32_t run = block_excl_scan_u32<G>(c, sh, tot);
-        for (int i = 0; i < per; ++i)
-            if (b0 + i < nwin) {
-                woff[b0 + i] = run;
-                run += wcnt[b0 + i];
+    // With room for a bin of C_tot ids per window in part, one walk scatters
+    // the ids straight into the bins; else a first walk counts the windows
+    // and a second one scatters into segments laid out by the counts.
+    const bool bins = ctot_q != ~uint64_t(0) && ctot_q <= pcap / (uint64_t)nwin;
+    if (bins) {
+        const uint32_t cap = (uint32_t)ctot_q;
+        for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+            if (ok) {
+                const int w = v >> shift;
+                const uint32_t pos = lds_add(&wfill[w], 1u);
+                if (pos < cap) gst(part + (uint64_t)w * cap + pos, v);
             }
-    }
-    G::sync();
-    for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
-        if (ok) {
-            const int w = v >> shift;
-            const uint32_t pos = woff[w] + atomicAdd(&wfill[w], 1u);
-            if (pos < pcap) gst(part + pos, v);
+        });
+        G::sync();
+        for (int i = G::tid(); i < nwin; i += G::size) {
+            wcnt[i] = min(wfill[i], cap);
+            woff[i] = (uint32_t)i * cap;
         }
-    });
+    } else {
+        // 24 ids per lane in flight (a bucket of the usual size in one round
+        // of loads), as in the other paths
+        for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+            if (ok) lds_add(&wcnt[v >> shift], 1u);
+        });
+        G::sync();
+        {
+            // exclusive scan of the window counts: thread t sums a contiguous run
+            const int per = (nwin + G::size - 1) / G::size;
+            const int b0 = G::tid() * per;
+            uint32_t c = 0;
+            for (int i = 0; i < per; ++i)
+                if (b0 + i < nwin) c += wcnt[b0 + i];
+            uint32_t tot;
+            uint32_t run = block_excl_scan_u32<G>(c, sh, tot);
+            for (int i = 0; i < per; ++i)
+                if (b0 + i < nwin) {
+                    woff[b0 + i] = run;
+                    run += wcnt[b0 + i];
+                }
+        }
+        G::sync();
+        for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
+            if (ok) {
+                const int w = v >> shift;
+                const uint32_t pos = lds_ld(woff + w) + lds_add(&wfill[w], 1u);
+                if (pos < pcap) gst(part + pos, v);
+            }
+        });
+    }
     uint32_t* count = sh;  // appended ids
-    if (G::tid() == 0) *count = 0u;
+    if (G::tid() == 0) lds_st(count, 0u);
     clear_words<G>(bits, min(dirty, area));
     dirty = 0u;
     G::sync();
     if (ts && G::tid() == 0) ts[1] = wall_clock64();
-    for (int w = 0; w < nwin; ++w) {
-        const uint32_t cnt = wcnt[w];
-        if (cnt == 0) continue;
+    // Each window's segment of part in rounds of UNR ids per thread; the
+    // next round (of this window or the next non-empty one) is loaded while
+    // the current one goes through the bitset.
+    constexpr int UNR = 8;
+    constexpr uint32_t RND = (uint32_t)G::size * UNR;
+    auto next_window = [&](int w) {
+        ++w;
+        while (w < nwin && lds_ld(wcnt + w) == 0u) ++w;
+        return w;
+    };
+    auto load_round = [&](int32_t* dst, int w, uint32_t i0) {
+        const uint32_t cnt = lds_ld(wcnt + w), base = lds_ld(woff + w);
+#pragma unroll
+        for (int r = 0; r < UNR; ++r) {
+            const uint32_t i = i0 + r * G::size + G::tid();
+            dst[r] = (i < cnt && base + i < pcap) ? gld(part + base + i) : -1;
+        }
+    };
+    int w = next_window(-1);
+    int32_t nx[UNR];
+    if (w < nwin) load_round(nx, w, 0);
+    while (w < nwin) {
+        const uint32_t cnt = lds_ld(wcnt + w);
         const int64_t w0 = (int64_t)w << shift;
         const int words = (int)((min(W, a.N - w0) + 31) / 32);
-        const uint32_t base = woff[w];
-        constexpr int UNR = 8;
         // the last round's ids stay in v: a window of one round is zeroed
         // from them afterwards
         int32_t v[UNR];
-        for (uint32_t i0 = 0; i0 < cnt; i0 += G::size * UNR) {
+        int wn = w;
+        for (uint32_t i0 = 0;; i0 += RND) {
 #pragma unroll
-            for (int r = 0; r < UNR; ++r) {
-                const uint32_t i = i0 + r * G::size + G::tid();
-                v[r] = (i < cnt && base + i < pcap) ? gld(part + base + i) : -1;
+            for (int r = 0; r < UNR; ++r) v[r] = nx[r];
+            const bool more = i0 + RND < cnt;
+            if (more) {
+                load_round(nx, w, i0 + RND);
+            } else {
+                wn = next_window(w);
+                if (wn < nwin) load_round(nx, wn, 0);
             }
+            bool fresh[UNR];
 #pragma unroll
-            for (int r = 0; r < UNR; ++r)
-                wave_append<true>(v[r] >= 0 && bitset_insert(bits, (int64_t)v[r] - w0), v[r], count, cq, ccap);
+            for (int r = 0; r < UNR; ++r) fresh[r] = v[r] >= 0 && bitset_insert(bits, (int64_t)v[r] - w0);
+            wave_append_n<UNR>(fresh, v, count, cq, ccap);
+            if (!more) break;
         }
-        G::sync();
-        // zero what the window set: the words of its ids, or the whole window
-        // when that takes fewer stores
-        if (cnt > (uint32_t)words) {
-            clear_words<G>(bits, (uint32_t)words);
-        } else if (cnt <= (uint32_t)(G::size * UNR)) {
+        // the windows share only LDS: the appends to cq and the next
+        // window's loads stay in flight across these barriers
+        G::lds_sync();
+        // zero what the window set: from the ids of its only round, else
+        // the whole window
+        if (cnt <= RND) {
 #pragma unroll
             for (int r = 0; r < UNR; ++r)
-                if (v[r] >= 0) bits[(v[r] - w0) >> 5] = 0u;
+                if (v[r] >= 0) lds_st(bits + ((v[r] - w0) >> 5), 0u);
         } else {
-            for (uint32_t i = (uint32_t)G::tid(); i < cnt; i += G::size)
-                if (base + i < pcap) bits[(gld(part + base + i) - w0) >> 5] = 0u;
+            clear_words<G>(bits, (uint32_t)words);
         }
-        G::sync();
+        G::lds_sync();
+        w = wn;
     }
-    const uint32_t nc = *count;
+    const uint32_t nc = lds_ld(count);
     G::sync();
     return nc;
 }
@@ -541,11 +643,25 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
 // unspecified order; returns min(want, n).  Radix select over the 32-bit key;
 // when more entries share the boundary key than are needed, a second radix
 // select over their ids keeps the smallest.  hist: 256 words, sh: G::size words,
-// count: one word.
-template <class G>
+// count: one word, out_id: want words, all in LDS; keys in LDS when KEYS_LDS,
+// else in global memory.
+template <class G, bool KEYS_LDS>
 __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restrict__ cq, uint32_t n, uint32_t want,
                                     uint32_t* hist, uint32_t* sh, int* s_digit, uint32_t* s_below, uint32_t* count,
                                     int32_t* out_id) {
+    if constexpr (KEYS_LDS) keys = lds_ptr(keys);
+    auto key_at = [&](uint32_t i) {
+        if constexpr (KEYS_LDS)
+            return keys[i];
+        else
+            return gld(keys + i);
+    };
+    out_id = lds_ptr(out_id);
+    hist = lds_ptr(hist);
+    sh = lds_ptr(sh);
+    s_digit = lds_ptr(s_digit);
+    s_below = lds_ptr(s_below);
+    count = lds_ptr(count);
     const uint32_t kk = min(want, n);
     if (kk == 0) return 0;
     uint32_t prefix = 0, pmask = 0, k = kk, eqc = 0;
@@ -553,7 +669,7 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
         for (int i = G::tid(); i < 256; i += G::size) hist[i] = 0;
         G::sync();
         for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) {
-            const uint32_t key = keys[i];
+            const uint32_t key = key_at(i);
             if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
         }
         G::sync();
@@ -573,7 +689,7 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
             G::sync();
             for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) {
                 const uint32_t id = (uint32_t)gld(cq + i);
-                if (keys[i] == T && (id & im) == ip) atomicAdd(&hist[(id >> shift) & 255u], 1u);
+                if (key_at(i) == T && (id & im) == ip) atomicAdd(&hist[(id >> shift) & 255u], 1u);
             }
             G::sync();
             int d;
@@ -592,7 +708,7 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
         bool take = false;
         int32_t id = 0;
         if (i < n) {
-            const uint32_t key = keys[i];
+            const uint32_t key = key_at(i);
             if (key <= T) {
                 id = gld(cq + i);
                 take = key < T || (uint32_t)id <= IT;

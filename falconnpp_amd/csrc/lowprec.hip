@@ -314,9 +314,10 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
         }
         __syncthreads();
         // the R best approximate scores (ties: smaller id)
-        rr = nc <= lds_cap ? select_smallest<Whole<TB>>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid)
-                           : select_smallest<Whole<TB>>(reinterpret_cast<const uint32_t*>(qs.part), cq, nc, (uint32_t)Rq, hist,
-                                                 sh, &s_digit, &s_below, count, rid);
+        rr = nc <= lds_cap
+                 ? select_smallest<Whole<TB>, true>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid)
+                 : select_smallest<Whole<TB>, false>(reinterpret_cast<const uint32_t*>(qs.part), cq, nc, (uint32_t)Rq,
+                                                     hist, sh, &s_digit, &s_below, count, rid);
     }
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
     // exact FP32 rescoring of the survivors, 8 lanes per row

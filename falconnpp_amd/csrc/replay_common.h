@@ -540,6 +540,7 @@ __device__ void replay_select(const ProbeGeom& g, const QueryView& qv_, uint32_t
 // gather's scoring loop.
 template <class G>
 __device__ __noinline__ void replay_query(const ReplayArgs& r, uint32_t q, unsigned char* lds, IFPair* pool) {
+    lds = lds_ptr(lds);
     const int w = G::tid() / WAVE;
     const QueryView qv_ = query_view(r.pg.qp, (int)q, r.pg.qProbes, r.pg.perTable, r.pg.PL, 0);
     const uint32_t flag = r.qflag[q];

@@ -179,8 +179,10 @@ int fpp_get_fp8_scale(fpp_index* h, float* shared);
  * FPP_TUNE_QUERY_ORDER: the queue's order, 0 = query order, 1 = by the best
  * table-0 bucket (queries sharing candidates together) in 8 runs, one per
  * XCD (default), 2 = longest first (by C_tot).  FPP_TUNE_GATHER_THREADS
- * 256/512/1024 (workgroup of the gather kernel; 256 runs two workgroups per
- * CU, the reduced-precision kernels use 256 or 512).  FPP_TUNE_PERSISTENT 0/1
+ * 0/256/512/1024 (workgroup of the gather kernel; 256 runs two workgroups per
+ * CU, the reduced-precision kernels use 256 or 512; 0, the default, picks
+ * 256 for the FP32 gather when n_points needs several dedup windows at 512,
+ * else 512).  FPP_TUNE_PERSISTENT 0/1
  * kept for callers of earlier versions: 1 sets order 2, 0 sets order 1.
  * FPP_TUNE_GRAPH 0/1: the whole query call (copy, hashing, probe selection,
  * exact replay, ordering, gather) captured once in a hipGraph and replayed
