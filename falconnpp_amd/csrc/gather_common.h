@@ -638,30 +638,16 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
     return nc;
 }
 
-// Selects the want smallest (key, id) pairs among keys[0 .. n), where the id
-// of position i is cq[i] (ids unique), and writes their ids to out_id in
-// unspecified order; returns min(want, n).  Radix select over the 32-bit key;
-// when more entries share the boundary key than are needed, a second radix
-// select over their ids keeps the smallest.  hist: 256 words, sh: G::size words,
-// count: one word, out_id: want words, all in LDS; keys in LDS when KEYS_LDS,
-// else in global memory.
-template <class G, bool KEYS_LDS>
-__device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restrict__ cq, uint32_t n, uint32_t want,
-                                    uint32_t* hist, uint32_t* sh, int* s_digit, uint32_t* s_below, uint32_t* count,
-                                    int32_t* out_id) {
-    if constexpr (KEYS_LDS) keys = lds_ptr(keys);
-    auto key_at = [&](uint32_t i) {
-        if constexpr (KEYS_LDS)
-            return keys[i];
-        else
-            return gld(keys + i);
-    };
-    out_id = lds_ptr(out_id);
-    hist = lds_ptr(hist);
-    sh = lds_ptr(sh);
-    s_digit = lds_ptr(s_digit);
-    s_below = lds_ptr(s_below);
-    count = lds_ptr(count);
+// Radix select of the want smallest (key, id) pairs among n entries, entry i
+// being (key_at(i), id_at(i)) with unique ids; writes their ids to out_id in
+// unspecified order and returns min(want, n).  Four 8-bit passes over the
+// key; when more entries share the boundary key than are needed, a second
+// radix select over their ids keeps the smallest.  hist: 256 words, sh:
+// G::size words, count: one word, out_id: want words, all in LDS.
+template <class G, class KeyAt, class IdAt>
+__device__ __forceinline__ uint32_t radix_select_smallest(KeyAt key_at, IdAt id_at, uint32_t n, uint32_t want,
+                                                          uint32_t* hist, uint32_t* sh, int* s_digit,
+                                                          uint32_t* s_below, uint32_t* count, int32_t* out_id) {
     const uint32_t kk = min(want, n);
     if (kk == 0) return 0;
     uint32_t prefix = 0, pmask = 0, k = kk, eqc = 0;
@@ -688,8 +674,9 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
             for (int i = G::tid(); i < 256; i += G::size) hist[i] = 0;
             G::sync();
             for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) {
-                const uint32_t id = (uint32_t)gld(cq + i);
-                if (key_at(i) == T && (id & im) == ip) atomicAdd(&hist[(id >> shift) & 255u], 1u);
+                if (key_at(i) != T) continue;
+                const uint32_t id = (uint32_t)id_at(i);
+                if ((id & im) == ip) atomicAdd(&hist[(id >> shift) & 255u], 1u);
             }
             G::sync();
             int d;
@@ -710,7 +697,7 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
         if (i < n) {
             const uint32_t key = key_at(i);
             if (key <= T) {
-                id = gld(cq + i);
+                id = id_at(i);
                 take = key < T || (uint32_t)id <= IT;
             }
         }
@@ -718,6 +705,119 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
     }
     G::sync();
     return kk;
+}
+
+// Selects the want smallest (key, id) pairs among keys[0 .. n), where the id
+// of position i is cq[i] (ids unique), and writes their ids to out_id in
+// unspecified order; returns min(want, n).  The keys are order keys of
+// scores (desc_key_any).  A radix select over all n keys is slow here: the
+// scores of one query cross zero and share few exponents, so the passes
+// over key bits pile their LDS atomics into a handful of bins.  Instead one
+// histogram over 256 bins spread linearly over the query's score range
+// finds the bin holding the want-th best; the keys of that bin and the
+// better ones (usually a few hundred) are copied to lst (global memory, room
+// for lcap pairs) and the radix select runs over those.  A selection whose
+// copy would not fit lcap runs over all keys.  hist: 256 words, sh: G::size
+// words, count: one word, out_id: want words, all in LDS; keys in LDS when
+// KEYS_LDS, else in global memory.
+template <class G, bool KEYS_LDS>
+__device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restrict__ cq, uint32_t n, uint32_t want,
+                                    uint32_t* hist, uint32_t* sh, int* s_digit, uint32_t* s_below, uint32_t* count,
+                                    int32_t* out_id, int32_t* lst, uint32_t lcap) {
+    if constexpr (KEYS_LDS) keys = lds_ptr(keys);
+    auto key_at = [&](uint32_t i) {
+        if constexpr (KEYS_LDS)
+            return keys[i];
+        else
+            return gld(keys + i);
+    };
+    auto cq_at = [&](uint32_t i) { return gld(cq + i); };
+    out_id = lds_ptr(out_id);
+    hist = lds_ptr(hist);
+    sh = lds_ptr(sh);
+    s_digit = lds_ptr(s_digit);
+    s_below = lds_ptr(s_below);
+    count = lds_ptr(count);
+    const uint32_t kk = min(want, n);
+    if (kk == 0) return 0;
+    if (n <= lcap || lcap < 2 * kk)
+        return radix_select_smallest<G>(key_at, cq_at, n, want, hist, sh, s_digit, s_below, count, out_id);
+    constexpr int NW = G::size / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = G::tid() / WAVE;
+    // the score range: best (smallest key) and worst
+    uint32_t mn = 0xffffffffu, mx = 0u;
+    for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) {
+        const uint32_t key = key_at(i);
+        mn = min(mn, key);
+        mx = max(mx, key);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, WAVE));
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, WAVE));
+    }
+    if (lane == 0) {
+        sh[wv] = mn;
+        sh[NW + wv] = mx;
+    }
+    G::sync();
+    uint32_t kmin = 0xffffffffu, kmax = 0u;
+#pragma unroll
+    for (int v = 0; v < NW; ++v) {
+        kmin = min(kmin, sh[v]);
+        kmax = max(kmax, sh[NW + v]);
+    }
+    G::sync();
+    const float vbest = desc_key_any_value(kmin), vworst = desc_key_any_value(kmax);
+    const float range = vbest - vworst;
+    const float scale = 256.0f / range;
+    if (!(range > 0.0f) || !(scale < __builtin_inff()))
+        return radix_select_smallest<G>(key_at, cq_at, n, want, hist, sh, s_digit, s_below, count, out_id);
+    // bin of a key: non-decreasing in the key (rounding is monotone), so the
+    // keys up to the want-th smallest all lie in bins up to its bin
+    auto bin_of = [&](uint32_t key) -> uint32_t {
+        const float x = (vbest - desc_key_any_value(key)) * scale;
+        return x < 255.0f ? (uint32_t)x : 255u;
+    };
+    for (int i = G::tid(); i < 256; i += G::size) hist[i] = 0;
+    G::sync();
+    for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) atomicAdd(&hist[bin_of(key_at(i))], 1u);
+    G::sync();
+    int d;
+    uint32_t below, eqd;
+    block_find_digit<G>(hist, kk, sh, d, below, eqd, s_digit, s_below);
+    const uint32_t m = below + eqd;
+    if (m > lcap)
+        return radix_select_smallest<G>(key_at, cq_at, n, want, hist, sh, s_digit, s_below, count, out_id);
+    // the keys of bins <= d with their ids, in lst[0 .. m) and lst[lcap ..)
+    uint32_t* lkey = reinterpret_cast<uint32_t*>(lst);
+    int32_t* lid = lst + lcap;
+    if (G::tid() == 0) *count = 0u;
+    G::sync();
+    for (uint32_t c0 = 0; c0 < n; c0 += G::size) {
+        const uint32_t i = c0 + G::tid();
+        uint32_t key = 0;
+        bool take = false;
+        if (i < n) {
+            key = key_at(i);
+            take = bin_of(key) <= (uint32_t)d;
+        }
+        const uint64_t bm = __ballot(take);
+        if (bm == 0ull) continue;
+        const int leader = (int)__ffsll((long long)bm) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = lds_add(count, (uint32_t)__popcll(bm));
+        base = __shfl(base, leader, WAVE);
+        const uint32_t pos = base + lane_rank(bm);
+        if (take && pos < m) {
+            gst(lkey + pos, key);
+            gst(lid + pos, cq_at(i));
+        }
+    }
+    G::sync();
+    return radix_select_smallest<G>([&](uint32_t i) { return gld(lkey + i); }, [&](uint32_t i) { return gld(lid + i); },
+                                    m, want, hist, sh, s_digit, s_below, count, out_id);
 }
 
 // Wave-level running top-R (R <= 64*KPL) over unique 64-bit keys, smaller is

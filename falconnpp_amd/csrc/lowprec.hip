@@ -314,10 +314,18 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
         }
         __syncthreads();
         // the R best approximate scores (ties: smaller id)
-        rr = nc <= lds_cap
-                 ? select_smallest<Whole<TB>, true>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid)
-                 : select_smallest<Whole<TB>, false>(reinterpret_cast<const uint32_t*>(qs.part), cq, nc, (uint32_t)Rq,
-                                                     hist, sh, &s_digit, &s_below, count, rid);
+        // (the short list of the selection in the query's scratch, after the
+        // keys when they are there)
+        constexpr uint64_t LIST_CAP = 4096;
+        if (nc <= lds_cap) {
+            rr = select_smallest<Whole<TB>, true>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid,
+                                                  qs.part, (uint32_t)min(qs.pcap / 2, LIST_CAP));
+        } else {
+            const uint64_t room = qs.pcap > nc ? (qs.pcap - nc) / 2 : 0;
+            rr = select_smallest<Whole<TB>, false>(reinterpret_cast<const uint32_t*>(qs.part), cq, nc, (uint32_t)Rq,
+                                                   hist, sh, &s_digit, &s_below, count, rid, qs.part + nc,
+                                                   (uint32_t)min(room, LIST_CAP));
+        }
     }
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
     // exact FP32 rescoring of the survivors, 8 lanes per row

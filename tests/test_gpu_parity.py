@@ -462,6 +462,59 @@ def test_small_queries_are_scored_exactly_in_every_precision(mode):
     assert np.array_equal(got, ref) and np.array_equal(gsc, rsc)
 
 
+def _bf16(a):
+    """float32 -> bfloat16 (round to nearest even) -> float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_bf16_refined_set_is_the_best_by_approximate_score():
+    """With k equal to the refine count R, the answer is the set the
+    approximate pass kept: the R best candidates by BF16 score (smaller id on
+    ties).  Thousands of candidates per query, so the selection goes through
+    the short list of its best score bins; the host recomputes the BF16
+    scores in float64, so the sets may differ only between candidates whose
+    scores lie within rounding of the boundary."""
+    import torch
+    X = synth.gaussian_mixture(400_000, 64, 100, seed=31)
+    n, d = X.shape
+    R, qp, nq = 80, 400, 16
+    g = E.Index(n, d)
+    g.setIndexParam(8, 64, 20, 0.05, 1, 8, 42)
+    g.build(X)
+    rng = np.random.default_rng(5)
+    Q = (X[rng.choice(n, nq, replace=False)] + np.float32(0.05) * rng.standard_normal((nq, d))).astype(np.float32)
+    g.set_qProbes(qp)
+    g.set_tuning(E.TUNE_KEEP_CANDIDATES, 1)
+    g.query(Q, 20)
+    cands = [g.last_candidates(j) for j in range(nq)]
+    g.set_tuning(E.TUNE_KEEP_CANDIDATES, 0)
+    print(f"[parity] candidates per query {min(map(len, cands))} .. {max(map(len, cands))}", flush=True)
+    assert min(map(len, cands)) > 20 * R
+    dQ = torch.from_numpy(Q).cuda()
+    out = torch.zeros((nq, R), dtype=torch.int32, device="cuda")
+    try:
+        g.set_precision(E.PREC_BF16, R)
+        g.query_device(dQ.data_ptr(), d, nq, R, out.data_ptr(), 0)
+        torch.cuda.synchronize()
+    finally:
+        g.set_precision(E.PREC_FP32, 0)
+    got = out.cpu().numpy()
+    Xb = _bf16(X - g.center())
+    for j in range(nq):
+        c = cands[j]
+        approx = Xb[c] @ _bf16(Q[j])
+        order = np.lexsort((c, -approx))
+        want = set(c[order[:R]].tolist())
+        bound = approx[order[R - 1]]
+        score = dict(zip(c.tolist(), approx.tolist()))
+        diff = set(got[j].tolist()) ^ want
+        tol = 1e-5 * (abs(bound) + np.abs(approx).max())
+        assert all(abs(score[i] - bound) <= tol for i in diff), (j, sorted(diff))
+        assert len(set(got[j].tolist())) == R
+
+
 def test_fp8_rows_keep_their_own_scales_when_magnitudes_spread():
     # rows scaled by 2^-6 .. 2^6, with their negatives so that the centre is
     # about zero: the centred rows' largest magnitudes span more than 2^8, so
