@@ -4,6 +4,8 @@
 // running top-R over 64-bit keys.
 #pragma once
 
+#include <utility>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -29,6 +31,13 @@ struct Whole {
 };
 
 // ---- block / wave helpers -------------------------------------------------
+// LDS ordering inside one wave: a wave's LDS operations complete in program
+// order, so a barrier that keeps the compiler from moving them is enough.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <class G>
 __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* sh, uint32_t& total) {
     constexpr int T = G::size;
@@ -707,6 +716,56 @@ __device__ __forceinline__ uint32_t radix_select_smallest(KeyAt key_at, IdAt id_
     return kk;
 }
 
+// radix_select_smallest by one wave over entries it holds in registers:
+// lane l's slot e is entry e * 64 + l when ok[e].  hist: 256 words of LDS of
+// this wave, count: one LDS word, out_id: LDS.  Returns the number written,
+// min(want, entries).
+template <int E>
+__device__ __forceinline__ uint32_t wave_select_smallest(const uint32_t (&key)[E], const int32_t (&id)[E],
+                                                         const bool (&ok)[E], uint32_t n, uint32_t want,
+                                                         uint32_t* hist, uint32_t* count, int32_t* out_id) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t kk = min(want, n);
+    if (kk == 0) return 0;
+    // one radix select over 32-bit values v(e) of the entries with sel(e)
+    auto radix = [&](auto&& val, auto&& sel, uint32_t k, uint32_t& eq) {
+        uint32_t prefix = 0, pmask = 0;
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            for (int i = lane; i < 256; i += WAVE) lds_st(hist + i, 0u);
+            wave_lds_sync();
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t v = val(e);
+                if (ok[e] && sel(e) && (v & pmask) == prefix) lds_add(hist + ((v >> shift) & 255u), 1u);
+            }
+            wave_lds_sync();
+            int d;
+            uint32_t below;
+            wave_find_digit(hist, k, d, below, eq);
+            wave_lds_sync();
+            prefix |= (uint32_t)d << shift;
+            pmask |= 255u << shift;
+            k -= below;
+        }
+        return std::make_pair(prefix, k);
+    };
+    uint32_t eqc = 0;
+    const auto tk = radix([&](int e) { return key[e]; }, [&](int) { return true; }, kk, eqc);
+    const uint32_t T = tk.first, k = tk.second;
+    uint32_t IT = 0xffffffffu;
+    if (eqc > k) {
+        uint32_t eqd = 0;
+        IT = radix([&](int e) { return (uint32_t)id[e]; }, [&](int e) { return key[e] == T; }, k, eqd).first;
+    }
+    if (lane == 0) lds_st(count, 0u);
+    wave_lds_sync();
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        wave_append<false>(ok[e] && (key[e] < T || (key[e] == T && (uint32_t)id[e] <= IT)), id[e], count, out_id, kk);
+    wave_lds_sync();
+    return kk;
+}
+
 // Selects the want smallest (key, id) pairs among keys[0 .. n), where the id
 // of position i is cq[i] (ids unique), and writes their ids to out_id in
 // unspecified order; returns min(want, n).  The keys are order keys of
@@ -717,13 +776,14 @@ __device__ __forceinline__ uint32_t radix_select_smallest(KeyAt key_at, IdAt id_
 // finds the bin holding the want-th best; the keys of that bin and the
 // better ones (usually a few hundred) are copied to lst (global memory, room
 // for lcap pairs) and the radix select runs over those.  A selection whose
-// copy would not fit lcap runs over all keys.  hist: 256 words, sh: G::size
-// words, count: one word, out_id: want words, all in LDS; keys in LDS when
-// KEYS_LDS, else in global memory.
+// copy would not fit lcap runs over all keys.  kmin and kmax are the smallest
+// and largest key (the caller tracks them while it writes the keys).  hist:
+// 256 words, sh: G::size words, count: one word, out_id: want words, all in
+// LDS; keys in LDS when KEYS_LDS, else in global memory.
 template <class G, bool KEYS_LDS>
 __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restrict__ cq, uint32_t n, uint32_t want,
                                     uint32_t* hist, uint32_t* sh, int* s_digit, uint32_t* s_below, uint32_t* count,
-                                    int32_t* out_id, int32_t* lst, uint32_t lcap) {
+                                    int32_t* out_id, int32_t* lst, uint32_t lcap, uint32_t kmin, uint32_t kmax) {
     if constexpr (KEYS_LDS) keys = lds_ptr(keys);
     auto key_at = [&](uint32_t i) {
         if constexpr (KEYS_LDS)
@@ -742,33 +802,28 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
     if (kk == 0) return 0;
     if (n <= lcap || lcap < 2 * kk)
         return radix_select_smallest<G>(key_at, cq_at, n, want, hist, sh, s_digit, s_below, count, out_id);
-    constexpr int NW = G::size / WAVE;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = G::tid() / WAVE;
-    // the score range: best (smallest key) and worst
-    uint32_t mn = 0xffffffffu, mx = 0u;
-    for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) {
-        const uint32_t key = key_at(i);
-        mn = min(mn, key);
-        mx = max(mx, key);
-    }
+    // the keys in rounds of U per thread, all read before the first is used
+    // (the passes would otherwise wait out a load's latency per key);
+    // f(i, key, valid) is reached by the whole group the same number of times
+    constexpr int U = 8;
+    auto for_keys = [&](auto&& f) {
+        for (uint32_t c0 = 0; c0 < n; c0 += (uint32_t)U * G::size) {
+            uint32_t kv[U];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, WAVE));
-        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, WAVE));
-    }
-    if (lane == 0) {
-        sh[wv] = mn;
-        sh[NW + wv] = mx;
-    }
-    G::sync();
-    uint32_t kmin = 0xffffffffu, kmax = 0u;
+            for (int u = 0; u < U; ++u) {
+                const uint32_t i = c0 + (uint32_t)(u * G::size + G::tid());
+                kv[u] = i < n ? key_at(i) : 0u;
+            }
 #pragma unroll
-    for (int v = 0; v < NW; ++v) {
-        kmin = min(kmin, sh[v]);
-        kmax = max(kmax, sh[NW + v]);
-    }
-    G::sync();
+            for (int u = 0; u < U; ++u) {
+                const uint32_t i = c0 + (uint32_t)(u * G::size + G::tid());
+                f(i, kv[u], i < n);
+            }
+        }
+    };
+    // [kmin, kmax]: the caller's range of the keys, best (smallest) to worst
     const float vbest = desc_key_any_value(kmin), vworst = desc_key_any_value(kmax);
     const float range = vbest - vworst;
     const float scale = 256.0f / range;
@@ -782,12 +837,24 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
     };
     for (int i = G::tid(); i < 256; i += G::size) hist[i] = 0;
     G::sync();
-    for (uint32_t i = (uint32_t)G::tid(); i < n; i += G::size) atomicAdd(&hist[bin_of(key_at(i))], 1u);
+    for_keys([&](uint32_t, uint32_t key, bool ok) {
+        if (ok) atomicAdd(&hist[bin_of(key)], 1u);
+    });
     G::sync();
-    int d;
-    uint32_t below, eqd;
-    block_find_digit<G>(hist, kk, sh, d, below, eqd, s_digit, s_below);
-    const uint32_t m = below + eqd;
+    // the bin of the want-th best, by one wave
+    if (wv == 0) {
+        int dd;
+        uint32_t bb, ee;
+        wave_find_digit(hist, kk, dd, bb, ee);
+        if (lane == 0) {
+            *s_digit = dd;
+            sh[0] = bb + ee;
+        }
+    }
+    G::sync();
+    const int d = *s_digit;
+    const uint32_t m = sh[0];
+    G::sync();
     if (m > lcap)
         return radix_select_smallest<G>(key_at, cq_at, n, want, hist, sh, s_digit, s_below, count, out_id);
     // the keys of bins <= d with their ids, in lst[0 .. m) and lst[lcap ..)
@@ -795,16 +862,10 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
     int32_t* lid = lst + lcap;
     if (G::tid() == 0) *count = 0u;
     G::sync();
-    for (uint32_t c0 = 0; c0 < n; c0 += G::size) {
-        const uint32_t i = c0 + G::tid();
-        uint32_t key = 0;
-        bool take = false;
-        if (i < n) {
-            key = key_at(i);
-            take = bin_of(key) <= (uint32_t)d;
-        }
+    for_keys([&](uint32_t i, uint32_t key, bool ok) {
+        const bool take = ok && bin_of(key) <= (uint32_t)d;
         const uint64_t bm = __ballot(take);
-        if (bm == 0ull) continue;
+        if (bm == 0ull) return;
         const int leader = (int)__ffsll((long long)bm) - 1;
         uint32_t base = 0;
         if (lane == leader) base = lds_add(count, (uint32_t)__popcll(bm));
@@ -814,8 +875,28 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
             gst(lkey + pos, key);
             gst(lid + pos, cq_at(i));
         }
-    }
+    });
     G::sync();
+    // a list of up to 64 * LE entries: one wave selects from registers,
+    // with no workgroup barrier per pass
+    constexpr int LE = 16;
+    if (m <= (uint32_t)(WAVE * LE)) {
+        if (wv == 0) {
+            uint32_t key[LE];
+            int32_t id[LE];
+            bool ok[LE];
+#pragma unroll
+            for (int e = 0; e < LE; ++e) {
+                const uint32_t i = (uint32_t)(e * WAVE + lane);
+                ok[e] = i < m;
+                key[e] = ok[e] ? gld(lkey + i) : 0u;
+                id[e] = ok[e] ? gld(lid + i) : 0;
+            }
+            wave_select_smallest<LE>(key, id, ok, m, want, hist, count, out_id);
+        }
+        G::sync();
+        return kk;
+    }
     return radix_select_smallest<G>([&](uint32_t i) { return gld(lkey + i); }, [&](uint32_t i) { return gld(lid + i); },
                                     m, want, hist, sh, s_digit, s_below, count, out_id);
 }

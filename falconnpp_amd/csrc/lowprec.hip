@@ -250,6 +250,7 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
             const int32_t v = gld(cq + min(i, last));
             idn[h] = i < nc ? v : -1;
         }
+        uint32_t kmn = 0xffffffffu, kmx = 0u;  // this lane's range of the keys
         for (uint32_t t0 = (uint32_t)wv * SPAN; t0 < nc; t0 += stride) {
             int32_t id[NT];
             float rs[NT];
@@ -305,12 +306,31 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
                 if constexpr (FMT == FP8) sc *= (lp.uscale > 0.0f ? lp.uscale : __shfl(rs[h], 2 * c, WAVE)) * qscale;
                 const uint32_t i = t0 + h * 8 + c;
                 if (r16 < 2 && i < nc) {
+                    const uint32_t key = desc_key_any(sc);
+                    kmn = min(kmn, key);
+                    kmx = max(kmx, key);
                     if (nc <= lds_cap)
-                        bits[i] = desc_key_any(sc);
+                        bits[i] = key;
                     else
-                        gst(reinterpret_cast<uint32_t*>(qs.part) + i, desc_key_any(sc));
+                        gst(reinterpret_cast<uint32_t*>(qs.part) + i, key);
                 }
             }
+        }
+        // the range of the keys, for the selection
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            kmn = min(kmn, (uint32_t)__shfl_xor((int)kmn, off, WAVE));
+            kmx = max(kmx, (uint32_t)__shfl_xor((int)kmx, off, WAVE));
+        }
+        if (lane == 0) {
+            red[wv] = kmn;
+            red[NW + wv] = kmx;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            kmn = min(kmn, red[w]);
+            kmx = max(kmx, red[NW + w]);
         }
         __syncthreads();
         // the R best approximate scores (ties: smaller id)
@@ -319,12 +339,12 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
         constexpr uint64_t LIST_CAP = 4096;
         if (nc <= lds_cap) {
             rr = select_smallest<Whole<TB>, true>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid,
-                                                  qs.part, (uint32_t)min(qs.pcap / 2, LIST_CAP));
+                                                  qs.part, (uint32_t)min(qs.pcap / 2, LIST_CAP), kmn, kmx);
         } else {
             const uint64_t room = qs.pcap > nc ? (qs.pcap - nc) / 2 : 0;
             rr = select_smallest<Whole<TB>, false>(reinterpret_cast<const uint32_t*>(qs.part), cq, nc, (uint32_t)Rq,
                                                    hist, sh, &s_digit, &s_below, count, rid, qs.part + nc,
-                                                   (uint32_t)min(room, LIST_CAP));
+                                                   (uint32_t)min(room, LIST_CAP), kmn, kmx);
         }
     }
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();

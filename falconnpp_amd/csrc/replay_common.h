@@ -15,12 +15,6 @@ namespace {
 using namespace gather;
 
 // ---- exact replay of a flagged query's probing sequence -------------------
-// LDS ordering inside one wave: a wave's LDS operations complete in program
-// order, so a barrier that keeps the compiler from moving them is enough.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // Rotation of the query for one layer into x (F floats of the wave's LDS):
 // three rounds of sign flips and the unnormalised FWHT with the stages in
