@@ -94,7 +94,7 @@ struct Index {
     size_t ws_budget = size_t(8) << 30;
     DevBuf<int32_t> out_ids;  // host-pointer query calls
     std::vector<QueryParams> hqp;  // per-query probe counts and k (batched serving)
-    int gather_threads = 0;        // 0: chosen per call (gather_workgroup_threads)
+    int gather_threads = 0;        // 0: chosen per call (gather_shape)
     int order_mode = 1;            // ORDER_* below
     bool keep_candidates = false;  // every query's candidates kept after the call (inspection)
     int replay_mode = 0;           // tie replay: 0 per call, 1 separate kernels, 2 inside the gather
@@ -634,15 +634,24 @@ QueryLayout query_layout(int nq, int L, int PL, uint64_t nprobes, bool batched, 
     return w;
 }
 
-// Workgroup size of the gather: the tuned one, or automatically 256 for the
-// FP32 gather when even a workgroup alone on its CU needs several dedup
-// windows (two workgroups per CU then overlap one query's candidate
-// collection with another's scoring, which outweighs their smaller windows),
-// else 512.
-int gather_workgroup_threads(const Index& I, bool lowp) {
-    if (I.gather_threads) return I.gather_threads;
-    if (lowp) return 512;
-    return I.n_points > gather_window_bits(I.n_points, 512 + 256 + 2048, 1) ? 256 : 512;
+// Launch shape of the gather: workgroup size and workgroups per CU.  The
+// tuned size, or automatically: when even a workgroup alone on its CU needs
+// several dedup windows, two workgroups per CU (one query's candidate
+// collection then overlaps another's scoring, which outweighs their smaller
+// windows): 256 threads for the FP32 gather, 512 with half the rows in
+// flight per wave for BF16/FP8; else one 512-thread workgroup per CU.
+// 256-thread workgroups always run two per CU.
+struct GatherShape {
+    int threads;
+    int per_cu;
+};
+GatherShape gather_shape(const Index& I, bool lowp, int refine) {
+    if (I.gather_threads == 256) return {256, 2};
+    if (I.gather_threads) return {lowp ? 512 : I.gather_threads, 1};
+    const int64_t extra = lowp ? lowprec_extra_words(refine, 512) : 512 + 256 + 2048;
+    const bool windows = I.n_points > gather_window_bits(I.n_points, extra, 1);
+    if (!windows) return {512, 1};
+    return lowp ? GatherShape{512, 2} : GatherShape{256, 2};
 }
 
 // One batch of queries.  k is the number of neighbours, or with per-query
@@ -720,10 +729,9 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         else
             ensure_lowprec(I, st);
     }
-    // 256-thread workgroups run two per CU (dedup window sized for half the
-    // LDS), larger ones one per CU
-    const int tsel = gather_workgroup_threads(I, lowp);
-    const int gthreads = lowp ? (tsel <= 256 ? 256 : 512) : tsel;
+    // (two workgroups per CU size their dedup windows for half the LDS)
+    const GatherShape shape = gather_shape(I, lowp, lowp ? I.refine_count(k) : 0);
+    const int gthreads = shape.threads;
     GatherArgs ga;
     ga.N = I.n_points;
     ga.D = I.n_features;
@@ -733,8 +741,8 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     if (batched) ga.ldo = k;
     ga.window_bits = gather_window_bits(I.n_points,
                                         lowp ? lowprec_extra_words(I.refine_count(k), gthreads)
-                                             : (int64_t)tsel + 256 + 2048,
-                                        gthreads <= 256 ? 2 : 1);
+                                             : (int64_t)gthreads + 256 + 2048,
+                                        shape.per_cu);
     LowPrecArgs lp;
     if (lowp) {
         lp.format = I.precision;
@@ -761,7 +769,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     const bool in_gather = !keep && rlds > 0 && I.replay_mode != 1;
     const size_t glds =
         std::max(lowp ? gather_lp_lds(ga, lp) : gather_f32_lds(ga, gthreads), in_gather ? rlds : size_t(0));
-    unsigned grid = persistent_grid(nq, glds, gthreads);
+    unsigned grid = persistent_grid(nq, glds, shape.per_cu);
     // per-workgroup scratch: the unique candidates (at most min(C_tot, N)) and
     // the window scatter or score area: with several dedup windows, a bin of
     // C_tot per window when the budget holds them for every workgroup (the
@@ -941,7 +949,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         if (ev) I.mark(5, st);
         if (lowp)
             launch_gather_score_lp(a, lp, rp, glds, grid, probes, I.bucket_off.get(), I.ids.get(), I.Xc.get(), Qpad,
-                                   I.ldx, d_out, d_scores, st);
+                                   I.ldx, d_out, d_scores, shape.per_cu, st);
         else
             launch_gather_score_f32(a, rp, glds, grid, probes, I.bucket_off.get(), I.ids.get(), I.Xc.get(), Qpad,
                                     I.ldx, d_out, d_scores, gthreads, st);
@@ -1280,9 +1288,11 @@ void autotune(Index& I, hipStream_t st, Run run, int32_t* chosen) {
     const bool profiling = I.profiling, stamps = I.phase_stamps;
     I.profiling = I.phase_stamps = false;
     // workgroup size applies to the FP32 gather; the reduced-precision
-    // gather runs 256 or 512
+    // gather runs 256 or 512; 0 is the automatic launch shape (for the
+    // reduced-precision gather the only way to two 512-thread workgroups
+    // per CU)
     const std::vector<int> threads =
-        I.precision == 0 ? std::vector<int>{256, 512, 1024} : std::vector<int>{256, 512};
+        I.precision == 0 ? std::vector<int>{0, 256, 512, 1024} : std::vector<int>{0, 256, 512};
     float best = 0.0f;
     int bt = I.gather_threads, bo = I.order_mode;
     try {
@@ -1322,7 +1332,7 @@ void autotune(Index& I, hipStream_t st, Run run, int32_t* chosen) {
     I.gather_threads = bt;
     I.order_mode = bo;
     if (chosen) {
-        chosen[0] = bt;
+        chosen[0] = bt ? bt : gather_shape(I, I.precision != 0, I.refine_count(1)).threads;
         chosen[1] = bo;
         chosen[2] = bo == ORDER_LONGEST ? 1 : 0;
         chosen[3] = (int32_t)(best / 2.0f * 1000.0f);  // microseconds per batch

@@ -240,9 +240,9 @@ void launch_pad_rows_init(const float* src, int64_t lds, int64_t rows, int D, fl
                           hipStream_t st);
 // p[0 .. n) = v (a kernel, so it can be captured in a graph).
 void launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
-// Workgroups of a persistent launch over nitems: every CU holds as many as
-// fit in its LDS (and two waves per SIMD), no more than there are items.
-unsigned persistent_grid(int nitems, size_t lds_bytes, int threads);
+// Workgroups of a persistent launch over nitems: per_cu on every CU, fewer
+// when their LDS does not fit, no more than there are items.
+unsigned persistent_grid(int nitems, size_t lds_bytes, int per_cu);
 // Sizes of the buckets of a built index (synchronises st): hist[s] = number
 // of buckets holding s < SIZE_BINS ids; big = the sizes of the others.
 // Returns false when there were more than 65536 of those (big incomplete).
@@ -302,7 +302,7 @@ size_t gather_lp_lds(const GatherArgs& a, const LowPrecArgs& lp);
 void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, const ReplayArgs& rp, size_t lds,
                             unsigned grid, const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids,
                             const float* Xc, const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores,
-                            hipStream_t st);
+                            int per_cu, hipStream_t st);
 
 // ---- FP64 brute force (bf.hip), verification only -------------------------
 void launch_bf64(const float* X, int64_t ldx, int64_t N, int D, const float* Q, int64_t ldq, int nq, int k,

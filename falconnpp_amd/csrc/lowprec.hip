@@ -104,14 +104,17 @@ constexpr int lp_chunks(int fmt, int kt) {
     const int c = fmt == BF16 ? kt / 2 : kt / 4;
     return c < 1 ? 1 : c;
 }
-// Tiles of 8 candidates per wave step: about 16 KB of row data in flight per
-// wave (a tile reads 1 KB per chunk), at least one tile.
-constexpr int lp_tiles(int fmt, int kt) {
-    const int t = 16 / lp_chunks(fmt, kt);
+// Waves per SIMD of a launch: TB threads, PERCU workgroups per CU.
+constexpr int lp_waves_per_simd(int tb, int percu) { return tb * percu / (4 * WAVE); }
+// Tiles of 8 candidates per wave step (a tile reads 1 KB per chunk): about
+// 16 KB of row data in flight per wave at two waves per SIMD, 8 KB at four
+// (their registers: 128 per wave), at least one tile.
+constexpr int lp_tiles(int fmt, int kt, int waves) {
+    const int t = (32 / waves) / lp_chunks(fmt, kt);
     return t < 1 ? 1 : t;
 }
 
-template <int TB, int FMT, int KT>
+template <int TB, int PERCU, int FMT, int KT>
 __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const LowPrecArgs& lp,
                                                         const ReplayArgs& rp,
                                                         const uint32_t* __restrict__ probes,
@@ -237,7 +240,7 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
         // query's scratch segment
         const uint32_t lds_cap = (uint32_t)Wwords;
         if (nc <= lds_cap) dirty = max(dirty, nc);
-        constexpr int NT = lp_tiles(FMT, KT);
+        constexpr int NT = lp_tiles(FMT, KT, lp_waves_per_simd(TB, PERCU));
         constexpr uint32_t SPAN = 8 * NT;
         const uint32_t stride = (uint32_t)NW * SPAN;
         // candidate ids one step ahead; every load is issued (clamped to the
@@ -394,8 +397,9 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
 
 // Persistent: each workgroup takes queries from the work queue until it is
 // drained.
-template <int TB, int FMT, int KT>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gather_score_lp(GatherArgs a, LowPrecArgs lp, ReplayArgs rp,
+template <int TB, int PERCU, int FMT, int KT>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(lp_waves_per_simd(TB, PERCU),
+                                                                   lp_waves_per_simd(TB, PERCU)))) void k_gather_score_lp(GatherArgs a, LowPrecArgs lp, ReplayArgs rp,
                                                         const uint32_t* __restrict__ probes,
                                                         const uint64_t* __restrict__ bucket_off,
                                                         const int32_t* __restrict__ ids, const float* __restrict__ Xc,
@@ -408,19 +412,19 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     for (;;) {
         const int q = claim_query(a, &s_q);
         if (q < 0) break;
-        k_gather_score_lp_one<TB, FMT, KT>(a, lp, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, q,
+        k_gather_score_lp_one<TB, PERCU, FMT, KT>(a, lp, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, q,
                                            dirty);
         __syncthreads();
     }
 }
 
-template <int TB, int FMT>
+template <int TB, int PERCU, int FMT>
 void dispatch_lp_kt(const GatherArgs& a, const LowPrecArgs& lp, const ReplayArgs& rp, unsigned grid, size_t lds,
                     const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids, const float* Xc,
                     const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores, hipStream_t st) {
     const int64_t kt = lp.ldl / 32;
 #define FPP_LP(KT)                                                                                            \
-    k_gather_score_lp<TB, FMT, KT><<<grid, TB, lds, st>>>(a, lp, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, \
+    k_gather_score_lp<TB, PERCU, FMT, KT><<<grid, TB, lds, st>>>(a, lp, rp, probes, bucket_off, ids, Xc, Q, ldq, out_ids, \
                                                           out_scores)
     if (kt <= 4)
         FPP_LP(4);
@@ -433,14 +437,14 @@ void dispatch_lp_kt(const GatherArgs& a, const LowPrecArgs& lp, const ReplayArgs
 #undef FPP_LP
 }
 
-template <int TB>
+template <int TB, int PERCU>
 void dispatch_lp_fmt(const GatherArgs& a, const LowPrecArgs& lp, const ReplayArgs& rp, unsigned grid, size_t lds,
                      const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids, const float* Xc,
                      const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores, hipStream_t st) {
     if (lp.format == BF16)
-        dispatch_lp_kt<TB, BF16>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
+        dispatch_lp_kt<TB, PERCU, BF16>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
     else
-        dispatch_lp_kt<TB, FP8>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
+        dispatch_lp_kt<TB, PERCU, FP8>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
 }
 
 }  // namespace
@@ -490,20 +494,25 @@ size_t gather_lp_lds(const GatherArgs& a, const LowPrecArgs& lp) {
 void launch_gather_score_lp(const GatherArgs& a, const LowPrecArgs& lp, const ReplayArgs& rp, size_t lds,
                             unsigned grid, const uint32_t* probes, const uint64_t* bucket_off, const int32_t* ids,
                             const float* Xc, const float* Q, int64_t ldq, int32_t* out_ids, float* out_scores,
-                            hipStream_t st) {
+                            int per_cu, hipStream_t st) {
     FPP_REQUIRE(lp.format == BF16 || lp.format == FP8, FPP_EINVAL, "unknown scoring format");
     FPP_REQUIRE(lp.refine >= 1 && lp.refine <= 4 * WAVE, FPP_EINVAL, "refine count must be in [1, 256]");
     FPP_REQUIRE(a.k >= 1 && a.k <= lp.refine, FPP_EINVAL, "n_neighbors must not exceed the refine count");
     FPP_REQUIRE(lp.ldl % 32 == 0 && lp.ldl / 32 <= 32, FPP_EINVAL, "n_features must be <= 1024");
     FPP_REQUIRE(lp.threads == 256 || lp.threads == 512, FPP_EINVAL, "low-precision workgroups are 256 or 512 threads");
+    FPP_REQUIRE(per_cu == 1 || per_cu == 2, FPP_EINTERNAL, "one or two low-precision workgroups per CU");
     FPP_REQUIRE(a.heads && a.ncand && a.cand && a.part && a.nqueues >= 1, FPP_EINTERNAL, "gather scratch missing");
     FPP_REQUIRE(lds >= gather_lp_lds(a, lp) && lds <= 160 * 1024, FPP_EINVAL,
                 "LDS budget exceeded (refine count or qProbes too large)");
     if (a.nitems == 0 || grid == 0) return;
+    // 256 threads run two workgroups per CU; 512 one, or two with half the
+    // rows in flight per wave
     if (lp.threads == 256)
-        dispatch_lp_fmt<256>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
+        dispatch_lp_fmt<256, 2>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
+    else if (per_cu == 2)
+        dispatch_lp_fmt<512, 2>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
     else
-        dispatch_lp_fmt<512>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
+        dispatch_lp_fmt<512, 1>(a, lp, rp, grid, lds, probes, bucket_off, ids, Xc, Q, ldq, out_ids, out_scores, st);
     FPP_HIP(hipGetLastError());
 }
 

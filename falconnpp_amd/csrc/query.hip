@@ -576,8 +576,6 @@ __global__ __launch_bounds__(CF_T) void k_compact_flagged(int nq, const uint32_t
     if (threadIdx.x == 0) *count = s_base;
 }
 
-// Sort key for longest-first order: C_tot descending, tie-flagged queries
-// (replayed in the gather, C_tot not known yet) first.
 // Is query q small (served one per wave)?  See SmallPick.
 __device__ __forceinline__ bool small_query(const SmallPick& sp, const uint32_t* __restrict__ qflag, int q) {
     if (sp.max_ctot == 0 || qflag[q]) return false;
@@ -593,6 +591,8 @@ __device__ __forceinline__ void count_small(bool small, uint32_t* count) {
         atomicAdd(count, (uint32_t)__popcll(m));
 }
 
+// Sort key for longest-first order: C_tot descending, tie-flagged queries
+// (replayed in the gather, C_tot not known yet) first.
 __global__ void k_ctot_keys(const uint64_t* __restrict__ ctot, const uint32_t* __restrict__ qflag, int nq,
                             uint32_t* __restrict__ key, SmallPick sp) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1315,16 +1315,16 @@ void launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st) {
     FPP_HIP(hipGetLastError());
 }
 
-unsigned persistent_grid(int nitems, size_t lds_bytes, int threads) {
+unsigned persistent_grid(int nitems, size_t lds_bytes, int per_cu) {
     static int cus = 0;
     if (!cus) {
         int dev = 0;
         FPP_HIP(hipGetDevice(&dev));
         FPP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
-    // resident workgroups per CU: LDS, and two waves per SIMD
-    const int per_cu = std::max<int>(1, std::min<int>(threads <= 256 ? 2 : 1, (int)(160 * 1024 / (lds_bytes + 1024))));
-    return (unsigned)std::max(1, std::min(nitems, cus * per_cu));
+    // resident workgroups per CU: as asked, as far as their LDS fits
+    const int fit = std::max<int>(1, std::min<int>(per_cu, (int)(160 * 1024 / (lds_bytes + 1024))));
+    return (unsigned)std::max(1, std::min(nitems, cus * fit));
 }
 
 uint64_t count_empty_probes(const uint32_t* probes, int64_t n, const uint64_t* bucket_off, uint32_t* scratch2,

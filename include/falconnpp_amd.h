@@ -180,9 +180,10 @@ int fpp_get_fp8_scale(fpp_index* h, float* shared);
  * table-0 bucket (queries sharing candidates together) in 8 runs, one per
  * XCD (default), 2 = longest first (by C_tot).  FPP_TUNE_GATHER_THREADS
  * 0/256/512/1024 (workgroup of the gather kernel; 256 runs two workgroups per
- * CU, the reduced-precision kernels use 256 or 512; 0, the default, picks
- * 256 for the FP32 gather when n_points needs several dedup windows at 512,
- * else 512).  FPP_TUNE_PERSISTENT 0/1
+ * CU, the reduced-precision kernels use 256 or 512; 0, the default, runs
+ * two workgroups per CU when n_points needs several dedup windows even for
+ * one 512-thread workgroup per CU: 256 threads for the FP32 gather, 512 with
+ * half the rows in flight per wave for BF16/FP8; else one of 512).  FPP_TUNE_PERSISTENT 0/1
  * kept for callers of earlier versions: 1 sets order 2, 0 sets order 1.
  * FPP_TUNE_GRAPH 0/1: the whole query call (copy, hashing, probe selection,
  * exact replay, ordering, gather) captured once in a hipGraph and replayed
