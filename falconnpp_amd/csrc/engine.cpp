@@ -753,6 +753,7 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
         lp.x8 = I.X8.get();
         lp.scale = I.s8.get();
         lp.uscale = I.precision == 2 ? I.s8_shared : 0.0f;
+        lp.scale_in_row = I.precision == 2 && fp8_scale_in_row(I.ldl, I.n_features, I.s8_shared) ? 1 : 0;
         lp.threads = gthreads;
     }
     const bool keep = I.keep_candidates && !fixed;

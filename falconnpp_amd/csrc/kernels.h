@@ -292,7 +292,12 @@ struct LowPrecArgs {
     const float* scale;      // FP8 per-row scale
     float uscale = 0.0f;     // FP8 scale shared by every row (0: per-row scales)
     int threads = 512;       // workgroup size, 256 or 512
+    int scale_in_row = 0;    // FP8 per-row scale in the row's last four bytes (fp8_scale_in_row)
 };
+// Whether an FP8 copy keeps each row's scale in the row's last four bytes
+// (per-row scales and at least four bytes of padding): scoring then reads
+// one line per candidate instead of the line and a separate scale.
+bool fp8_scale_in_row(int64_t ldl, int D, float uscale);
 int64_t lowprec_extra_words(int refine, int threads);
 void launch_quant_bf16(const float* Xc, int64_t ldx, int64_t N, int D, int64_t ldl, uint16_t* Xb, hipStream_t st);
 // returns the scale shared by every row, or 0 when rows keep their own

@@ -71,9 +71,10 @@ __global__ __launch_bounds__(256) void k_amax_range(const float* __restrict__ Xc
 }
 
 // One wave per row: scale = uscale when given, else amax / 240; codes =
-// e4m3(x / scale), zero padded.
+// e4m3(x / scale), zero padded; with in_row the row's last four bytes hold
+// its scale (FP32) instead of padding.
 __global__ __launch_bounds__(256) void k_quant_fp8(const float* __restrict__ Xc, int64_t ldx, int64_t N, int D,
-                                                   int64_t ldl, float uscale, uint8_t* __restrict__ X8,
+                                                   int64_t ldl, float uscale, bool in_row, uint8_t* __restrict__ X8,
                                                    float* __restrict__ scale) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t n = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
@@ -93,6 +94,7 @@ __global__ __launch_bounds__(256) void k_quant_fp8(const float* __restrict__ Xc,
         int w = 0;
         w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], w, false);
         w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);
+        if (in_row && d0 == ldl - 4) w = __float_as_int(s);
         *reinterpret_cast<int*>(X8 + n * ldl + d0) = w;
     }
 }
@@ -264,7 +266,8 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
                 const int32_t v = gld(cq + min(i, last));
                 idn[h] = i < nc ? v : -1;
                 if constexpr (FMT == FP8)
-                    rs[h] = lp.uscale > 0.0f ? lp.uscale : (id[h] < 0 ? 0.0f : gld(lp.scale + id[h]));
+                    rs[h] = lp.uscale > 0.0f || lp.scale_in_row ? lp.uscale
+                                                                : (id[h] < 0 ? 0.0f : gld(lp.scale + id[h]));
             }
             f32x4 acc[NT];
     #pragma unroll
@@ -280,6 +283,18 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
                     av[ch][h] = *reinterpret_cast<const int4*>(xbase + row * rb + boff[ch]);
                 }
             __builtin_amdgcn_sched_barrier(0);
+            if constexpr (FMT == FP8) {
+                // scale in the row: its last four bytes, read by lane (g16 3,
+                // odd r16) with the last chunk; they enter the MFMA as zero
+                if (lp.scale_in_row) {
+                    const bool holder = g16 == 3 && (r16 & 1);
+    #pragma unroll
+                    for (int h = 0; h < NT; ++h) {
+                        rs[h] = holder ? __int_as_float(av[KC - 1][h].w) : rs[h];
+                        av[KC - 1][h].w = holder ? 0 : av[KC - 1][h].w;
+                    }
+                }
+            }
     #pragma unroll
             for (int ch = 0; ch < KC; ++ch)
     #pragma unroll
@@ -306,7 +321,9 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
                 const float s0 = v0 + __shfl_xor(v0, 1, WAVE);
                 const float s1 = v1 + __shfl_xor(v1, 1, WAVE);
                 float sc = (r16 & 1) ? s1 : s0;
-                if constexpr (FMT == FP8) sc *= (lp.uscale > 0.0f ? lp.uscale : __shfl(rs[h], 2 * c, WAVE)) * qscale;
+                if constexpr (FMT == FP8)
+                    sc *= (lp.uscale > 0.0f ? lp.uscale : __shfl(rs[h], lp.scale_in_row ? 49 + 2 * c : 2 * c, WAVE)) *
+                          qscale;
                 const uint32_t i = t0 + h * 8 + c;
                 if (r16 < 2 && i < nc) {
                     const uint32_t key = desc_key_any(sc);
@@ -449,6 +466,8 @@ void dispatch_lp_fmt(const GatherArgs& a, const LowPrecArgs& lp, const ReplayArg
 
 }  // namespace
 
+bool fp8_scale_in_row(int64_t ldl, int D, float uscale) { return uscale <= 0.0f && ldl - D >= 4; }
+
 // LDS words beyond the bitset and window counters: scan scratch, reduction,
 // histogram, survivors (exact keys and ids), append counter.
 int64_t lowprec_extra_words(int refine, int threads) { return threads + 64 + 256 + 3 * (int64_t)refine + 2; }
@@ -480,7 +499,8 @@ float launch_quant_fp8(const float* Xc, int64_t ldx, int64_t N, int D, int64_t l
     std::memcpy(&lo, &r[1], 4);
     const float uscale =
         (!per_row && hi > 0.0f && r[1] != 0xffffffffu && hi <= 256.0f * lo) ? hi / FP8_TARGET : 0.0f;
-    k_quant_fp8<<<(unsigned)ceil_div(N, 4), 256, 0, st>>>(Xc, ldx, N, D, ldl, uscale, X8, scale);
+    k_quant_fp8<<<(unsigned)ceil_div(N, 4), 256, 0, st>>>(Xc, ldx, N, D, ldl, uscale,
+                                                          fp8_scale_in_row(ldl, D, uscale), X8, scale);
     FPP_HIP(hipGetLastError());
     return uscale;
 }

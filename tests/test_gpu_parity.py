@@ -515,13 +515,15 @@ def test_bf16_refined_set_is_the_best_by_approximate_score():
         assert len(set(got[j].tolist())) == R
 
 
-def test_fp8_rows_keep_their_own_scales_when_magnitudes_spread():
+@pytest.mark.parametrize("d", [128, 100])
+def test_fp8_rows_keep_their_own_scales_when_magnitudes_spread(d):
     # rows scaled by 2^-6 .. 2^6, with their negatives so that the centre is
     # about zero: the centred rows' largest magnitudes span more than 2^8, so
-    # each row keeps its own FP8 scale (the cases above share one)
+    # each row keeps its own FP8 scale (the cases above share one); at d=100
+    # an FP8 row has 28 bytes of padding and keeps its scale there
     import torch
     rng = np.random.default_rng(21)
-    A = synth.gaussian_mixture(6000, 128, 60, seed=13)
+    A = synth.gaussian_mixture(6000, d, 60, seed=13)
     A = A * np.exp2(rng.integers(-6, 7, len(A))).astype(np.float32)[:, None]
     X = np.concatenate([A, -A])[rng.permutation(2 * len(A))].astype(np.float32)
     n, d = X.shape
