@@ -110,6 +110,11 @@ __device__ __forceinline__ uint32_t lds_cas(uint32_t* p, uint32_t expected, uint
                                          __HIP_MEMORY_SCOPE_WORKGROUP);
     return expected;
 }
+// Sixteen bytes at a 16-byte aligned global address, one global_load_dwordx4
+// (HIP's int4 is a class whose copy goes through a generic reference, which
+// turns the load into a flat one).
+typedef int32_t i32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ i32x4_t gld_16(const void* p) { return *(const FPP_GLOBAL i32x4_t*)p; }
 // Four consecutive int32 at a 4-byte aligned address, one load.
 typedef int32_t i32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 __device__ __forceinline__ i32x4_a4 gld_i32x4(const int32_t* p) {

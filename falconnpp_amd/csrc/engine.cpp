@@ -766,25 +766,26 @@ void do_query(Index& I, const float* dQ, int64_t ldq, int nq, int k, int32_t* d_
     rp.hg = hg;
     rp.pg = pg;
     const int64_t wpq = probe_replay_work_bytes(pg);
-    const size_t rlds = replay_lds_bytes(rp, gthreads);
+    const size_t rlds = replay_lds_bytes(rp, gthreads, shape.per_cu);
     const bool in_gather = !keep && rlds > 0 && I.replay_mode != 1;
     const size_t glds =
         std::max(lowp ? gather_lp_lds(ga, lp) : gather_f32_lds(ga, gthreads), in_gather ? rlds : size_t(0));
     unsigned grid = persistent_grid(nq, glds, shape.per_cu);
     // per-workgroup scratch: the unique candidates (at most min(C_tot, N)) and
-    // the window scatter or score area: with several dedup windows, a bin of
-    // C_tot per window when the budget holds them for every workgroup (the
-    // ids are then scattered by one walk over the buckets instead of two),
-    // else C_tot
+    // the window scatter or score area (C_tot).  With several dedup windows
+    // the scatter is a bin per window, filled by one walk over the buckets; a
+    // query whose ids overflow a bin takes two walks instead.  The area is
+    // grown towards a bin of C_tot per window (no overflow possible) as far
+    // as the budget allows for every workgroup.
     const uint64_t bound = std::max<uint64_t>(1, ctot_bound(I, (uint64_t)qp_max));
     const bool one_window = I.n_points <= ga.window_bits;
     const uint64_t slot_cand = std::min<uint64_t>(bound, (uint64_t)I.n_points);
     uint64_t slot_part = one_window ? slot_cand : bound;
     if (!one_window && !fixed && !keep) {
         const uint64_t nwin = (uint64_t)window_count(I.n_points, ga.window_bits);
-        const uint64_t binned = nwin * bound;
-        if ((size_t)grid * (slot_cand + binned) * 4 + (in_gather ? (size_t)grid * wpq : 0) <= I.ws_budget)
-            slot_part = binned;
+        const size_t fixed_part = in_gather ? (size_t)grid * wpq : 0;
+        const size_t room = I.ws_budget > fixed_part ? (I.ws_budget - fixed_part) / ((size_t)grid * 4) : 0;
+        if (room > slot_cand) slot_part = std::max<uint64_t>(bound, std::min<uint64_t>(nwin * bound, room - slot_cand));
     }
     // separate replay kernels: one work area for each of the first rcap
     // flagged queries, and rover persistent workgroups for any beyond

@@ -23,6 +23,8 @@ struct Whole {
     static constexpr int size = TB;
     __device__ static __forceinline__ int tid() { return (int)threadIdx.x; }
     __device__ static __forceinline__ void sync() { __syncthreads(); }
+    // barrier that also returns whether any thread's p was nonzero
+    __device__ static __forceinline__ int sync_or(int p) { return __syncthreads_or(p); }
     __device__ static __forceinline__ void lds_sync() {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_s_barrier();
@@ -465,10 +467,10 @@ __device__ __forceinline__ void clear_words(uint32_t* bits, uint32_t n) {
 //    order, so cq is ascending; the area is left dirty.
 //  * Otherwise (W is then a power of two): the bucket ids are scattered by
 //    window into part (capacity pcap, at least C_tot: duplicates
-//    included): into a bin of C_tot ids per window by one walk when pcap
-//    holds them all, else into segments laid out by a first, counting walk;
-//    each window's ids go through a test-and-set, first occurrences
-//    appended, and the words it touched are zeroed again.
+//    included): by one walk into a bin of pcap / nwin ids per window, or,
+//    when a window overflows its bin, into segments laid out by a first,
+//    counting walk; each window's ids go through a test-and-set, first
+//    occurrences appended, and the words it touched are zeroed again.
 // Callers rank by (score, id), which does not depend on the order of cq.
 // wc holds window_counter_words(N, W) words, sh at least G::size words.
 template <class G>
@@ -527,12 +529,16 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         wfill[i] = 0u;
     }
     G::sync();
-    // With room for a bin of C_tot ids per window in part, one walk scatters
-    // the ids straight into the bins; else a first walk counts the windows
-    // and a second one scatters into segments laid out by the counts.
-    const bool bins = ctot_q != ~uint64_t(0) && ctot_q <= pcap / (uint64_t)nwin;
+    // One walk scatters the ids straight into a bin per window: C_tot ids
+    // each when part holds that (no window can overflow; the bins stay as
+    // close together as the query allows), else pcap / nwin.  A window that
+    // overflows its bin sends the query to the counted layout: a first walk
+    // counts the windows and a second one scatters into segments laid out by
+    // the counts.
+    const uint64_t capw = min<uint64_t>(ctot_q, pcap / (uint64_t)nwin);
+    bool bins = capw >= 1;
     if (bins) {
-        const uint32_t cap = (uint32_t)ctot_q;
+        const uint32_t cap = (uint32_t)min<uint64_t>(capw, 0xffffffffu / (uint64_t)nwin);
         for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {
             if (ok) {
                 const int w = v >> shift;
@@ -541,11 +547,20 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
             }
         });
         G::sync();
+        int over = 0;
+        for (int i = G::tid(); i < nwin; i += G::size) over |= wfill[i] > cap ? 1 : 0;
+        bins = G::sync_or(over) == 0;
         for (int i = G::tid(); i < nwin; i += G::size) {
-            wcnt[i] = min(wfill[i], cap);
-            woff[i] = (uint32_t)i * cap;
+            if (bins) {
+                wcnt[i] = wfill[i];
+                woff[i] = (uint32_t)i * cap;
+            } else {
+                wfill[i] = 0u;
+            }
         }
-    } else {
+        G::sync();
+    }
+    if (!bins) {
         // 24 ids per lane in flight (a bucket of the usual size in one round
         // of loads), as in the other paths
         for_each_bucket_id<G, 24>(pr, bucket_off, ids, np, [&](int32_t v, bool ok) {

@@ -153,9 +153,10 @@ struct ReplayArgs {
     int twaves = 0;        // waves that rebuild tables
     int merge_staged = 0;  // lists staged in LDS for the merge
 };
-// LDS the replay needs inside a gather workgroup of `threads` (0 when a
-// workgroup of that size cannot hold one table's wave); fills r's LDS fields.
-size_t replay_lds_bytes(ReplayArgs& r, int threads);
+// LDS the replay needs inside a gather workgroup of `threads`, per_cu of
+// them sharing a CU (0 when a workgroup cannot hold one table's wave); fills
+// r's LDS fields.
+size_t replay_lds_bytes(ReplayArgs& r, int threads, int per_cu);
 int64_t probe_replay_work_bytes(const ProbeGeom& g);
 
 // One persistent gather launch: every workgroup takes queries from a work

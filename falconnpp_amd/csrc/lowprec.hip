@@ -116,6 +116,177 @@ constexpr int lp_tiles(int fmt, int kt, int waves) {
     return t < 1 ? 1 : t;
 }
 
+// Approximate scores of the candidates cq[0 .. nc) of one query, for the
+// calling wave's steps (candidates first .. first + 8 NT - 1, then every
+// stride), as order keys by candidate position: keys_lds[i] when
+// keys_in_lds, else keys_glb[i].  Returns this lane's smallest key in the low
+// half and its largest in the high half.  Out of line, so that its registers
+// are allocated for the scoring loop alone (inside the gather kernel the
+// loop spilled at its top, and the reload's wait drained every load and
+// store in flight on each step).
+//
+// A candidate enters the MFMA as two half-rows: tile row 2c + h holds half h
+// of candidate c's current 128-byte chunk and column n of the B operand holds
+// the query's half n of that chunk (columns 2..15 are zero), so candidate c's
+// score is the sum over chunks of C[2c][0] + C[2c+1][1].  Lane (g16, r16)
+// loads the 16 bytes at 64 h + 16 g16 of the chunk of candidate r16 / 2 (h =
+// r16 % 2): one load instruction reads the whole 128-byte lines of 8
+// candidates.  Bytes past the row end re-read its last 32 bytes against a
+// zero query (adding exactly 0).
+template <int FMT, int KT, int NT>
+__device__ __noinline__ uint64_t lp_score_keys(const float* __restrict__ qf32, int D, float qscale,
+                                              const uint8_t* __restrict__ xbase, int64_t rb, float uscale,
+                                              const float* __restrict__ scale, int scale_in_row,
+                                              const int32_t* __restrict__ cq, uint32_t nc, uint32_t first,
+                                              uint32_t stride, uint32_t* keys_lds, uint32_t* __restrict__ keys_glb,
+                                              bool keys_in_lds) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int g16 = lane >> 4;  // k-group of this lane in an MFMA operand
+    const int r16 = lane & 15;  // row (A) / column (B) of this lane
+    constexpr int KC = lp_chunks(FMT, KT);
+    // B operand: lane (g16, n = r16) holds query elements 8 g16 .. 8 g16 + 7
+    // of half n of each chunk (FP8: two k-steps per chunk half, s = 0, 1)
+    bf16x8 qb[KC];
+    long q8[KC][2];
+    const float inv = 1.0f / qscale;
+#pragma unroll
+    for (int ch = 0; ch < KC; ++ch) {
+#pragma unroll
+        for (int s = 0; s < (FMT == BF16 ? 1 : 2); ++s) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int d = FMT == BF16 ? 64 * ch + 32 * r16 + 8 * g16 + j
+                                          : 128 * ch + 64 * r16 + 16 * g16 + 8 * s + j;
+                v[j] = (r16 < 2 && d < D) ? qf32[d] : 0.0f;
+            }
+            if constexpr (FMT == BF16) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) qb[ch][j] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(v[j]));
+            } else {
+                int lo = 0, hi = 0;
+                lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, lo, false);
+                lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, lo, true);
+                hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4] * inv, v[5] * inv, hi, false);
+                hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6] * inv, v[7] * inv, hi, true);
+                q8[ch][s] = (long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+            }
+        }
+    }
+    // byte offset of this lane's 16 bytes in each chunk, clamped into the row
+    const int c8 = r16 >> 1;
+    int64_t boff[KC];
+#pragma unroll
+    for (int ch = 0; ch < KC; ++ch) {
+        const int64_t o = 128 * ch + 64 * (r16 & 1) + 16 * g16;
+        boff[ch] = o < rb ? o : rb - 32 + (o & 16);
+    }
+    // NT tiles of 8 candidates per wave step
+    constexpr uint32_t SPAN = 8 * NT;
+    // candidate ids one step ahead, as the step's SPAN ids in order over
+    // the lanes (IPL per lane, one coalesced load each; a tile's lane
+    // takes its candidate's id by a lane shuffle), which keeps IPL
+    // registers live across the step rather than NT; every load is issued
+    // (clamped to the last candidate) and the id replaced by -1 past the
+    // end
+    constexpr int IPL = (int)((SPAN + WAVE - 1) / WAVE);
+    const uint32_t last = nc - 1;
+    int32_t idn[IPL];
+#pragma unroll
+    for (int j = 0; j < IPL; ++j) {
+        const uint32_t i = first + (uint32_t)(j * WAVE + lane);
+        const int32_t v = gld(cq + min(i, last));
+        idn[j] = i < nc && (uint32_t)(j * WAVE + lane) < SPAN ? v : -1;
+    }
+    uint32_t kmn = 0xffffffffu, kmx = 0u;  // this lane's range of the keys
+    for (uint32_t t0 = first; t0 < nc; t0 += stride) {
+        int32_t idv[IPL];
+#pragma unroll
+        for (int j = 0; j < IPL; ++j) {
+            idv[j] = idn[j];
+            const uint32_t i = t0 + stride + (uint32_t)(j * WAVE + lane);
+            const int32_t v = gld(cq + min(i, last));
+            idn[j] = i < nc && (uint32_t)(j * WAVE + lane) < SPAN ? v : -1;
+        }
+        // candidate h * 8 + c8 of the step: lane (h % 8) * 8 + c8 of
+        // register h / 8
+        int32_t id[NT];
+        float rs[NT];
+#pragma unroll
+        for (int h = 0; h < NT; ++h) {
+            id[h] = __shfl(idv[h / 8], (h % 8) * 8 + c8, WAVE);
+            if constexpr (FMT == FP8)
+                rs[h] = uscale > 0.0f || scale_in_row ? uscale : (id[h] < 0 ? 0.0f : gld(scale + id[h]));
+        }
+        f32x4 acc[NT];
+#pragma unroll
+        for (int h = 0; h < NT; ++h) acc[h] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // no branches: all KC * NT loads of a step are in flight before the
+        // first MFMA waits
+        i32x4_t av[KC][NT];
+#pragma unroll
+        for (int ch = 0; ch < KC; ++ch)
+#pragma unroll
+            for (int h = 0; h < NT; ++h) {
+                const int64_t row = id[h] < 0 ? 0 : id[h];
+                av[ch][h] = gld_16(xbase + row * rb + boff[ch]);
+            }
+        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (FMT == FP8) {
+            // scale in the row: its last four bytes, read by lane (g16 3,
+            // odd r16) with the last chunk; they enter the MFMA as zero
+            if (scale_in_row) {
+                const bool holder = g16 == 3 && (r16 & 1);
+#pragma unroll
+                for (int h = 0; h < NT; ++h) {
+                    rs[h] = holder ? __int_as_float(av[KC - 1][h].w) : rs[h];
+                    av[KC - 1][h].w = holder ? 0 : av[KC - 1][h].w;
+                }
+            }
+        }
+#pragma unroll
+        for (int ch = 0; ch < KC; ++ch)
+#pragma unroll
+            for (int h = 0; h < NT; ++h) {
+                if constexpr (FMT == BF16) {
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[ch][h]), qb[ch],
+                                                                      acc[h], 0, 0, 0);
+                } else {
+                    const long lo = (long)(((uint64_t)(uint32_t)av[ch][h].y << 32) | (uint32_t)av[ch][h].x);
+                    const long hi = (long)(((uint64_t)(uint32_t)av[ch][h].w << 32) | (uint32_t)av[ch][h].z);
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(lo, q8[ch][0], acc[h], 0, 0, 0);
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(hi, q8[ch][1], acc[h], 0, 0, 0);
+                }
+            }
+        // accumulator i of lane (g16, n) holds C[4 g16 + i][n]: candidate
+        // 2 g16 is acc[0] of column 0 + acc[1] of column 1, candidate
+        // 2 g16 + 1 is acc[2] of column 0 + acc[3] of column 1; lane
+        // (g16, n < 2) stores candidate 2 g16 + n
+        const int c = 2 * g16 + (r16 & 1);
+#pragma unroll
+        for (int h = 0; h < NT; ++h) {
+            const float v0 = (r16 & 1) ? acc[h][1] : acc[h][0];
+            const float v1 = (r16 & 1) ? acc[h][3] : acc[h][2];
+            const float s0 = v0 + __shfl_xor(v0, 1, WAVE);
+            const float s1 = v1 + __shfl_xor(v1, 1, WAVE);
+            float sc = (r16 & 1) ? s1 : s0;
+            if constexpr (FMT == FP8)
+                sc *= (uscale > 0.0f ? uscale : __shfl(rs[h], scale_in_row ? 49 + 2 * c : 2 * c, WAVE)) * qscale;
+            const uint32_t i = t0 + h * 8 + c;
+            if (r16 < 2 && i < nc) {
+                const uint32_t key = desc_key_any(sc);
+                kmn = min(kmn, key);
+                kmx = max(kmx, key);
+                if (keys_in_lds)
+                    lds_st(keys_lds + i, key);
+                else
+                    gst(keys_glb + i, key);
+            }
+        }
+    }
+    return ((uint64_t)kmx << 32) | kmn;
+}
+
 template <int TB, int PERCU, int FMT, int KT>
 __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const LowPrecArgs& lp,
                                                         const ReplayArgs& rp,
@@ -198,144 +369,17 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
             qscale = m > 0.0f ? m / FP8_TARGET : 1.0f;
             __syncthreads();
         }
-        // B operand: lane (g16, n = r16) holds query elements 8 g16 .. 8 g16 + 7
-        // of half n of each chunk (FP8: two k-steps per chunk half, s = 0, 1)
-        bf16x8 qb[KC];
-        long q8[KC][2];
-        const float inv = 1.0f / qscale;
-    #pragma unroll
-        for (int ch = 0; ch < KC; ++ch) {
-    #pragma unroll
-            for (int s = 0; s < (FMT == BF16 ? 1 : 2); ++s) {
-                float v[8];
-    #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int d = FMT == BF16 ? 64 * ch + 32 * r16 + 8 * g16 + j
-                                              : 128 * ch + 64 * r16 + 16 * g16 + 8 * s + j;
-                    v[j] = (r16 < 2 && d < a.D) ? qf32[d] : 0.0f;
-                }
-                if constexpr (FMT == BF16) {
-    #pragma unroll
-                    for (int j = 0; j < 8; ++j) qb[ch][j] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(v[j]));
-                } else {
-                    int lo = 0, hi = 0;
-                    lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, lo, false);
-                    lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, lo, true);
-                    hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4] * inv, v[5] * inv, hi, false);
-                    hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6] * inv, v[7] * inv, hi, true);
-                    q8[ch][s] = (long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
-                }
-            }
-        }
-        // byte offset of this lane's 16 bytes in each chunk, clamped into the row
-        const int c8 = r16 >> 1;
-        int64_t boff[KC];
-    #pragma unroll
-        for (int ch = 0; ch < KC; ++ch) {
-            const int64_t o = 128 * ch + 64 * (r16 & 1) + 16 * g16;
-            boff[ch] = o < rb ? o : rb - 32 + (o & 16);
-        }
         const uint8_t* xbase = FMT == BF16 ? reinterpret_cast<const uint8_t*>(lp.xb) : lp.x8;
-        // approximate scores, NT tiles of 8 candidates per wave step (the ids of
-        // the next step fetched one step ahead), as sort keys by candidate
-        // position: in LDS over the dedup bitset when they fit, else in the
-        // query's scratch segment
+        // approximate scores as sort keys by candidate position: in LDS over
+        // the dedup bitset when they fit, else in the query's scratch segment
         const uint32_t lds_cap = (uint32_t)Wwords;
         if (nc <= lds_cap) dirty = max(dirty, nc);
         constexpr int NT = lp_tiles(FMT, KT, lp_waves_per_simd(TB, PERCU));
         constexpr uint32_t SPAN = 8 * NT;
-        const uint32_t stride = (uint32_t)NW * SPAN;
-        // candidate ids one step ahead; every load is issued (clamped to the
-        // last candidate) and the id replaced by -1 past the end
-        const uint32_t last = nc - 1;
-        int32_t idn[NT];
-    #pragma unroll
-        for (int h = 0; h < NT; ++h) {
-            const uint32_t i = (uint32_t)wv * SPAN + h * 8 + c8;
-            const int32_t v = gld(cq + min(i, last));
-            idn[h] = i < nc ? v : -1;
-        }
-        uint32_t kmn = 0xffffffffu, kmx = 0u;  // this lane's range of the keys
-        for (uint32_t t0 = (uint32_t)wv * SPAN; t0 < nc; t0 += stride) {
-            int32_t id[NT];
-            float rs[NT];
-    #pragma unroll
-            for (int h = 0; h < NT; ++h) {
-                id[h] = idn[h];
-                const uint32_t i = t0 + stride + h * 8 + c8;
-                const int32_t v = gld(cq + min(i, last));
-                idn[h] = i < nc ? v : -1;
-                if constexpr (FMT == FP8)
-                    rs[h] = lp.uscale > 0.0f || lp.scale_in_row ? lp.uscale
-                                                                : (id[h] < 0 ? 0.0f : gld(lp.scale + id[h]));
-            }
-            f32x4 acc[NT];
-    #pragma unroll
-            for (int h = 0; h < NT; ++h) acc[h] = f32x4{0.f, 0.f, 0.f, 0.f};
-            // no branches: all KC * NT loads of a step are in flight before the
-            // first MFMA waits
-            int4 av[KC][NT];
-    #pragma unroll
-            for (int ch = 0; ch < KC; ++ch)
-    #pragma unroll
-                for (int h = 0; h < NT; ++h) {
-                    const int64_t row = id[h] < 0 ? 0 : id[h];
-                    av[ch][h] = *reinterpret_cast<const int4*>(xbase + row * rb + boff[ch]);
-                }
-            __builtin_amdgcn_sched_barrier(0);
-            if constexpr (FMT == FP8) {
-                // scale in the row: its last four bytes, read by lane (g16 3,
-                // odd r16) with the last chunk; they enter the MFMA as zero
-                if (lp.scale_in_row) {
-                    const bool holder = g16 == 3 && (r16 & 1);
-    #pragma unroll
-                    for (int h = 0; h < NT; ++h) {
-                        rs[h] = holder ? __int_as_float(av[KC - 1][h].w) : rs[h];
-                        av[KC - 1][h].w = holder ? 0 : av[KC - 1][h].w;
-                    }
-                }
-            }
-    #pragma unroll
-            for (int ch = 0; ch < KC; ++ch)
-    #pragma unroll
-                for (int h = 0; h < NT; ++h) {
-                    if constexpr (FMT == BF16) {
-                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[ch][h]), qb[ch],
-                                                                          acc[h], 0, 0, 0);
-                    } else {
-                        const long lo = (long)(((uint64_t)(uint32_t)av[ch][h].y << 32) | (uint32_t)av[ch][h].x);
-                        const long hi = (long)(((uint64_t)(uint32_t)av[ch][h].w << 32) | (uint32_t)av[ch][h].z);
-                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(lo, q8[ch][0], acc[h], 0, 0, 0);
-                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(hi, q8[ch][1], acc[h], 0, 0, 0);
-                    }
-                }
-            // accumulator i of lane (g16, n) holds C[4 g16 + i][n]: candidate
-            // 2 g16 is acc[0] of column 0 + acc[1] of column 1, candidate
-            // 2 g16 + 1 is acc[2] of column 0 + acc[3] of column 1; lane
-            // (g16, n < 2) stores candidate 2 g16 + n
-            const int c = 2 * g16 + (r16 & 1);
-    #pragma unroll
-            for (int h = 0; h < NT; ++h) {
-                const float v0 = (r16 & 1) ? acc[h][1] : acc[h][0];
-                const float v1 = (r16 & 1) ? acc[h][3] : acc[h][2];
-                const float s0 = v0 + __shfl_xor(v0, 1, WAVE);
-                const float s1 = v1 + __shfl_xor(v1, 1, WAVE);
-                float sc = (r16 & 1) ? s1 : s0;
-                if constexpr (FMT == FP8)
-                    sc *= (lp.uscale > 0.0f ? lp.uscale : __shfl(rs[h], lp.scale_in_row ? 49 + 2 * c : 2 * c, WAVE)) *
-                          qscale;
-                const uint32_t i = t0 + h * 8 + c;
-                if (r16 < 2 && i < nc) {
-                    const uint32_t key = desc_key_any(sc);
-                    kmn = min(kmn, key);
-                    kmx = max(kmx, key);
-                    if (nc <= lds_cap)
-                        bits[i] = key;
-                    else
-                        gst(reinterpret_cast<uint32_t*>(qs.part) + i, key);
-                }
-            }
-        }
+        const uint64_t kr = lp_score_keys<FMT, KT, NT>(qf32, a.D, qscale, xbase, rb, lp.uscale, lp.scale, lp.scale_in_row,
+                                                       cq, nc, (uint32_t)wv * SPAN, (uint32_t)NW * SPAN, bits,
+                                                       reinterpret_cast<uint32_t*>(qs.part), nc <= lds_cap);
+        uint32_t kmn = (uint32_t)kr, kmx = (uint32_t)(kr >> 32);
         // the range of the keys, for the selection
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {

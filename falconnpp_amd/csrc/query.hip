@@ -1340,8 +1340,10 @@ uint64_t count_empty_probes(const uint32_t* probes, int64_t n, const uint64_t* b
     return c;
 }
 
-size_t replay_lds_bytes(ReplayArgs& r, int threads) {
-    constexpr size_t budget = 150 * 1024;
+size_t replay_lds_bytes(ReplayArgs& r, int threads, int per_cu) {
+    // the workgroup's share of the CU's LDS, less room for the kernels' own
+    // words, so that the replay never costs the gather a resident workgroup
+    const size_t budget = std::min<size_t>(150 * 1024, 160 * 1024 / (size_t)std::max(1, per_cu) - 2048);
     const int64_t tlds = exact_table_lds_bytes(r.hg.F, r.pg.PL, r.pg.perTable);
     if (tlds > (int64_t)budget) return 0;
     const size_t staged = (size_t)merge_lds_bytes(r.pg.L, r.pg.perTable, true);
@@ -1402,7 +1404,7 @@ void launch_ctot_keys(const uint64_t* ctot, const uint32_t* qflag, int nq, uint3
 
 // extra_words: LDS words the kernel needs beyond the bitset and window counters.
 int64_t gather_window_bits(int64_t N, int64_t extra_words, int per_cu) {
-    const int64_t fixed = (extra_words + 64) * 4;  // + static LDS margin
+    const int64_t fixed = (extra_words + 64) * 4 + 1024;  // + the kernels' own LDS words
     const int64_t budget = 160 * 1024 / std::max(1, per_cu) - 1024 - fixed;
     const int64_t single = (N + 31) / 32 * 32;
     if (single / 8 + 8 <= budget) return single;
