@@ -794,17 +794,28 @@ __device__ __forceinline__ uint32_t wave_select_smallest(const uint32_t (&key)[E
 // copy would not fit lcap runs over all keys.  kmin and kmax are the smallest
 // and largest key (the caller tracks them while it writes the keys).  hist:
 // 256 words, sh: G::size words, count: one word, out_id: want words, all in
-// LDS; keys in LDS when KEYS_LDS, else in global memory.
-template <class G, bool KEYS_LDS>
-__device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restrict__ cq, uint32_t n, uint32_t want,
-                                    uint32_t* hist, uint32_t* sh, int* s_digit, uint32_t* s_below, uint32_t* count,
-                                    int32_t* out_id, int32_t* lst, uint32_t lcap, uint32_t kmin, uint32_t kmax) {
-    if constexpr (KEYS_LDS) keys = lds_ptr(keys);
+// LDS.  KEYS: 0, the keys in LDS; 2, in global memory; 1, in global memory
+// with their top 16 bits in LDS (keys16): the histogram and the copy read
+// those (a key's top bits order like the key, so its bin is still monotone)
+// and only the copied keys are read whole.
+template <class G, int KEYS>
+__device__ uint32_t select_smallest(const uint32_t* keys, const uint16_t* keys16, const int32_t* __restrict__ cq,
+                                    uint32_t n, uint32_t want, uint32_t* hist, uint32_t* sh, int* s_digit,
+                                    uint32_t* s_below, uint32_t* count, int32_t* out_id, int32_t* lst, uint32_t lcap,
+                                    uint32_t kmin, uint32_t kmax) {
+    if constexpr (KEYS == 0) keys = lds_ptr(keys);
     auto key_at = [&](uint32_t i) {
-        if constexpr (KEYS_LDS)
+        if constexpr (KEYS == 0)
             return keys[i];
         else
             return gld(keys + i);
+    };
+    // the key, or with KEYS 1 its top 16 bits, for the bins
+    auto coarse_at = [&](uint32_t i) {
+        if constexpr (KEYS == 1)
+            return (uint32_t)lds_ld(keys16 + i) << 16;
+        else
+            return key_at(i);
     };
     auto cq_at = [&](uint32_t i) { return gld(cq + i); };
     out_id = lds_ptr(out_id);
@@ -829,7 +840,7 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t i = c0 + (uint32_t)(u * G::size + G::tid());
-                kv[u] = i < n ? key_at(i) : 0u;
+                kv[u] = i < n ? coarse_at(i) : 0u;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -839,7 +850,8 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
         }
     };
     // [kmin, kmax]: the caller's range of the keys, best (smallest) to worst
-    const float vbest = desc_key_any_value(kmin), vworst = desc_key_any_value(kmax);
+    // (with KEYS 1 the best one's top bits, so that no bin is below 0)
+    const float vbest = desc_key_any_value(KEYS == 1 ? kmin & 0xffff0000u : kmin), vworst = desc_key_any_value(kmax);
     const float range = vbest - vworst;
     const float scale = 256.0f / range;
     if (!(range > 0.0f) || !(scale < __builtin_inff()))
@@ -848,7 +860,7 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
     // keys up to the want-th smallest all lie in bins up to its bin
     auto bin_of = [&](uint32_t key) -> uint32_t {
         const float x = (vbest - desc_key_any_value(key)) * scale;
-        return x < 255.0f ? (uint32_t)x : 255u;
+        return x > 0.0f ? (x < 255.0f ? (uint32_t)x : 255u) : 0u;
     };
     for (int i = G::tid(); i < 256; i += G::size) hist[i] = 0;
     G::sync();
@@ -887,7 +899,7 @@ __device__ uint32_t select_smallest(const uint32_t* keys, const int32_t* __restr
         base = __shfl(base, leader, WAVE);
         const uint32_t pos = base + lane_rank(bm);
         if (take && pos < m) {
-            gst(lkey + pos, key);
+            gst(lkey + pos, KEYS == 1 ? key_at(i) : key);
             gst(lid + pos, cq_at(i));
         }
     });

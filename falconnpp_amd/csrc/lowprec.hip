@@ -118,8 +118,9 @@ constexpr int lp_tiles(int fmt, int kt, int waves) {
 
 // Approximate scores of the candidates cq[0 .. nc) of one query, for the
 // calling wave's steps (candidates first .. first + 8 NT - 1, then every
-// stride), as order keys by candidate position: keys_lds[i] when
-// keys_in_lds, else keys_glb[i].  Returns this lane's smallest key in the low
+// stride), as order keys by candidate position: keys_lds[i] (keys_mode 0),
+// else keys_glb[i] and with keys_mode 1 also the key's top 16 bits at
+// (uint16_t*)keys_lds + i.  Returns this lane's smallest key in the low
 // half and its largest in the high half.  Out of line, so that its registers
 // are allocated for the scoring loop alone (inside the gather kernel the
 // loop spilled at its top, and the reload's wait drained every load and
@@ -139,7 +140,7 @@ __device__ __noinline__ uint64_t lp_score_keys(const float* __restrict__ qf32, i
                                               const float* __restrict__ scale, int scale_in_row,
                                               const int32_t* __restrict__ cq, uint32_t nc, uint32_t first,
                                               uint32_t stride, uint32_t* keys_lds, uint32_t* __restrict__ keys_glb,
-                                              bool keys_in_lds) {
+                                              int keys_mode) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int g16 = lane >> 4;  // k-group of this lane in an MFMA operand
     const int r16 = lane & 15;  // row (A) / column (B) of this lane
@@ -277,10 +278,12 @@ __device__ __noinline__ uint64_t lp_score_keys(const float* __restrict__ qf32, i
                 const uint32_t key = desc_key_any(sc);
                 kmn = min(kmn, key);
                 kmx = max(kmx, key);
-                if (keys_in_lds)
+                if (keys_mode == 0) {
                     lds_st(keys_lds + i, key);
-                else
+                } else {
                     gst(keys_glb + i, key);
+                    if (keys_mode == 1) lds_st(reinterpret_cast<uint16_t*>(keys_lds) + i, (uint16_t)(key >> 16));
+                }
             }
         }
     }
@@ -371,14 +374,18 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
         }
         const uint8_t* xbase = FMT == BF16 ? reinterpret_cast<const uint8_t*>(lp.xb) : lp.x8;
         // approximate scores as sort keys by candidate position: in LDS over
-        // the dedup bitset when they fit, else in the query's scratch segment
+        // the dedup bitset when they fit (mode 0), else in the query's
+        // scratch segment, with their top 16 bits in LDS when those fit
+        // (mode 1, the selection then reads the whole keys of a few hundred
+        // only)
         const uint32_t lds_cap = (uint32_t)Wwords;
-        if (nc <= lds_cap) dirty = max(dirty, nc);
+        const int keys_mode = nc <= lds_cap ? 0 : (nc <= 2 * lds_cap ? 1 : 2);
+        if (keys_mode != 2) dirty = max(dirty, keys_mode == 0 ? nc : (nc + 1) / 2);
         constexpr int NT = lp_tiles(FMT, KT, lp_waves_per_simd(TB, PERCU));
         constexpr uint32_t SPAN = 8 * NT;
         const uint64_t kr = lp_score_keys<FMT, KT, NT>(qf32, a.D, qscale, xbase, rb, lp.uscale, lp.scale, lp.scale_in_row,
                                                        cq, nc, (uint32_t)wv * SPAN, (uint32_t)NW * SPAN, bits,
-                                                       reinterpret_cast<uint32_t*>(qs.part), nc <= lds_cap);
+                                                       reinterpret_cast<uint32_t*>(qs.part), keys_mode);
         uint32_t kmn = (uint32_t)kr, kmx = (uint32_t)(kr >> 32);
         // the range of the keys, for the selection
 #pragma unroll
@@ -401,14 +408,19 @@ __device__ __forceinline__ void k_gather_score_lp_one(const GatherArgs& a, const
         // (the short list of the selection in the query's scratch, after the
         // keys when they are there)
         constexpr uint64_t LIST_CAP = 4096;
-        if (nc <= lds_cap) {
-            rr = select_smallest<Whole<TB>, true>(bits, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid,
-                                                  qs.part, (uint32_t)min(qs.pcap / 2, LIST_CAP), kmn, kmx);
+        const uint32_t* kg = reinterpret_cast<const uint32_t*>(qs.part);
+        const uint16_t* k16 = reinterpret_cast<const uint16_t*>(bits);
+        if (keys_mode == 0) {
+            rr = select_smallest<Whole<TB>, 0>(bits, k16, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count, rid,
+                                               qs.part, (uint32_t)min(qs.pcap / 2, LIST_CAP), kmn, kmx);
         } else {
             const uint64_t room = qs.pcap > nc ? (qs.pcap - nc) / 2 : 0;
-            rr = select_smallest<Whole<TB>, false>(reinterpret_cast<const uint32_t*>(qs.part), cq, nc, (uint32_t)Rq,
-                                                   hist, sh, &s_digit, &s_below, count, rid, qs.part + nc,
-                                                   (uint32_t)min(room, LIST_CAP), kmn, kmx);
+            const uint32_t lcap = (uint32_t)min(room, LIST_CAP);
+            rr = keys_mode == 1
+                     ? select_smallest<Whole<TB>, 1>(kg, k16, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count,
+                                                     rid, qs.part + nc, lcap, kmn, kmx)
+                     : select_smallest<Whole<TB>, 2>(kg, k16, cq, nc, (uint32_t)Rq, hist, sh, &s_digit, &s_below, count,
+                                                     rid, qs.part + nc, lcap, kmn, kmx);
         }
     }
     if (ts && threadIdx.x == 0) ts[3] = wall_clock64();
