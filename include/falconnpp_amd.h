@@ -200,8 +200,10 @@ int fpp_get_fp8_scale(fpp_index* h, float* shared);
  * own across the device), 2 = inside the gather, by the workgroup that claims
  * the query, before it gathers it (flagged queries head the queue).
  * FPP_TUNE_WORKSPACE_MB: bound, in MiB, of the per-workgroup candidate
- * scratch of a query call (default 8192); a tighter bound runs the gather on
- * fewer workgroups, with the same results.  FPP_TUNE_SMALL_QUERIES 0/1
+ * scratch of a query call (default 8192); a tighter bound first gives the
+ * dedup windows smaller bins (a query whose ids overflow one takes two walks
+ * over its buckets instead of one), then runs the gather on fewer
+ * workgroups, with the same results.  FPP_TUNE_SMALL_QUERIES 0/1
  * (default 1): a query whose probed buckets hold at most 512 ids in all, that
  * needs no exact tie replay and asks for at most 64 neighbours is gathered
  * and scored by one wave instead of a workgroup (with a sorted query order,
