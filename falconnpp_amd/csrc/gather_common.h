@@ -597,65 +597,79 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
     dirty = 0u;
     G::sync();
     if (ts && G::tid() == 0) ts[1] = wall_clock64();
-    // Each window's segment of part in rounds of UNR ids per thread; the
-    // next round (of this window or the next non-empty one) is loaded while
-    // the current one goes through the bitset.
+    // Each window's segment of part in rounds of UNR ids per thread, the
+    // rounds of all windows in one sequence.  Two rounds are in flight while
+    // one goes through the bitset: a round's loads are issued two rounds
+    // before their ids are inserted, so a window's wait for its bin spans two
+    // rounds of work rather than one.  The three buffers rotate through an
+    // unrolled loop (a register copy would wait for the loads).
     constexpr int UNR = 8;
     constexpr uint32_t RND = (uint32_t)G::size * UNR;
+    struct Round {
+        int w;        // window (nwin: past the last one)
+        uint32_t i0;  // first index of the round in the window's segment
+    };
     auto next_window = [&](int w) {
         ++w;
         while (w < nwin && lds_ld(wcnt + w) == 0u) ++w;
         return w;
     };
-    auto load_round = [&](int32_t* dst, int w, uint32_t i0) {
-        const uint32_t cnt = lds_ld(wcnt + w), base = lds_ld(woff + w);
+    auto after = [&](Round c) -> Round {
+        if (c.w >= nwin) return c;
+        if (c.i0 + RND < lds_ld(wcnt + c.w)) return Round{c.w, c.i0 + RND};
+        return Round{next_window(c.w), 0u};
+    };
+    // ids of round c, -1 where there is none (every round past the last
+    // window issues no load)
+    auto issue = [&](int32_t (&dst)[UNR], Round c) {
+        const bool live = c.w < nwin;
+        const uint32_t cnt = live ? lds_ld(wcnt + c.w) : 0u, base = live ? lds_ld(woff + c.w) : 0u;
 #pragma unroll
         for (int r = 0; r < UNR; ++r) {
-            const uint32_t i = i0 + r * G::size + G::tid();
+            const uint32_t i = c.i0 + r * G::size + G::tid();
             dst[r] = (i < cnt && base + i < pcap) ? gld(part + base + i) : -1;
         }
     };
-    int w = next_window(-1);
-    int32_t nx[UNR];
-    if (w < nwin) load_round(nx, w, 0);
-    while (w < nwin) {
-        const uint32_t cnt = lds_ld(wcnt + w);
-        const int64_t w0 = (int64_t)w << shift;
-        const int words = (int)((min(W, a.N - w0) + 31) / 32);
-        // the last round's ids stay in v: a window of one round is zeroed
-        // from them afterwards
-        int32_t v[UNR];
-        int wn = w;
-        for (uint32_t i0 = 0;; i0 += RND) {
+    auto process = [&](Round c, const int32_t (&v)[UNR]) {
+        const uint32_t cnt = lds_ld(wcnt + c.w);
+        const int64_t w0 = (int64_t)c.w << shift;
+        bool fresh[UNR];
 #pragma unroll
-            for (int r = 0; r < UNR; ++r) v[r] = nx[r];
-            const bool more = i0 + RND < cnt;
-            if (more) {
-                load_round(nx, w, i0 + RND);
-            } else {
-                wn = next_window(w);
-                if (wn < nwin) load_round(nx, wn, 0);
-            }
-            bool fresh[UNR];
-#pragma unroll
-            for (int r = 0; r < UNR; ++r) fresh[r] = v[r] >= 0 && bitset_insert(bits, (int64_t)v[r] - w0);
-            wave_append_n<UNR>(fresh, v, count, cq, ccap);
-            if (!more) break;
-        }
-        // the windows share only LDS: the appends to cq and the next
-        // window's loads stay in flight across these barriers
+        for (int r = 0; r < UNR; ++r) fresh[r] = v[r] >= 0 && bitset_insert(bits, (int64_t)v[r] - w0);
+        wave_append_n<UNR>(fresh, v, count, cq, ccap);
+        if (c.i0 + RND < cnt) return;
+        // the window's last round.  The windows share only LDS: the appends
+        // to cq and the next rounds' loads stay in flight across these
+        // barriers
         G::lds_sync();
-        // zero what the window set: from the ids of its only round, else
-        // the whole window
+        // zero what the window set: from the ids of its only round, else the
+        // whole window
         if (cnt <= RND) {
 #pragma unroll
             for (int r = 0; r < UNR; ++r)
                 if (v[r] >= 0) lds_st(bits + ((v[r] - w0) >> 5), 0u);
         } else {
-            clear_words<G>(bits, (uint32_t)words);
+            clear_words<G>(bits, (uint32_t)((min(W, a.N - w0) + 31) / 32));
         }
         G::lds_sync();
-        w = wn;
+    };
+    Round r0{next_window(-1), 0u};
+    Round r1 = after(r0), r2 = after(r1);
+    int32_t b0[UNR], b1[UNR], b2[UNR];
+    issue(b0, r0);
+    issue(b1, r1);
+    while (r0.w < nwin) {
+        issue(b2, r2);
+        process(r0, b0);
+        r0 = after(r2);
+        if (r1.w >= nwin) break;
+        issue(b0, r0);
+        process(r1, b1);
+        r1 = after(r0);
+        if (r2.w >= nwin) break;
+        issue(b1, r1);
+        process(r2, b2);
+        r2 = after(r1);
     }
     const uint32_t nc = lds_ld(count);
     G::sync();
