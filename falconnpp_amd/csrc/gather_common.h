@@ -671,6 +671,9 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
         process(r2, b2);
         r2 = after(r1);
     }
+    // every wave's appends (count and cq) are ordered before this read by
+    // the last window's lds_sync (count was zeroed before a full barrier when
+    // there is no window at all)
     const uint32_t nc = lds_ld(count);
     G::sync();
     return nc;
