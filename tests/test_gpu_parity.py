@@ -29,6 +29,17 @@ def _sym_int_grid(n, d, lo, hi, seed):
     return np.concatenate([A, -A])[rng.permutation(n // 2 * 2)]
 
 
+def _far_block(X, lo, hi, seed):
+    """Rows lo..hi replaced by a tight cluster around one direction: queries
+    drawn from the other rows find no candidate in that id range, queries
+    drawn from it find tens of thousands there."""
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal(X.shape[1]).astype(np.float32)
+    u /= np.linalg.norm(u)
+    X[lo:hi] = u + np.float32(0.05) * rng.standard_normal((hi - lo, X.shape[1])).astype(np.float32)
+    return X
+
+
 CASES = {
     # name: (data, L, n_proj, bucketLimit, alpha, iProbes, qProbes, k, nq)
     "sphere_d100_p1": (lambda: synth.lognormal_sphere(20000, 100, 0.3, seed=11), 10, 128, 20, 0.01, 1, 100, 20, 60),
@@ -63,6 +74,11 @@ CASES = {
     # probes over 16 tables of buckets holding ~1270 ids each): the scatter by
     # window holds every bucket id
     "ctot_over_n_windows": (lambda: synth.gaussian_mixture(1_300_000, 16, 300, seed=26), 16, 16, 30, 1.0, 1, 2000, 20, 6),
+    # a tight cluster at ids [2^19, 2^20) of 1.6M: an empty window between a
+    # query's non-empty ones, windows of one round and of tens, and a query
+    # small enough for the open-addressing set, in one batch
+    "windows_far_block": (lambda: _far_block(synth.gaussian_mixture(1_600_000, 16, 300, seed=27), 1 << 19, 1 << 20, 28),
+                          4, 16, 30, 0.05, 1, 40, 20, 12),
 }
 
 # probe selection each case must take (engine stats: 0 pool of every pair,
@@ -184,6 +200,13 @@ def test_query_probes_candidates_topk(name):
         assert st["probe_path"] == PROBE_PATH[name]
     if name == "ctot_over_n_windows":
         assert all(o.query_debug(q, k)["ctot"] > len(X) for q in Q[:2])
+    if name == "windows_far_block":
+        # some query skips a window of 2^19 ids (the FP32 gather's window at
+        # two workgroups per CU) between its first and last candidate
+        def gap(c):
+            w = np.unique(c.astype(np.int64) >> 19)
+            return len(w) > 0 and w[-1] - w[0] + 1 > len(w)
+        assert any(gap(g.last_candidates(j)) for j in range(len(Q)))
 
 
 @pytest.mark.parametrize("k", [10, 64, 65, 200])
