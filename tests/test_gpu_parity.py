@@ -379,6 +379,31 @@ def test_low_precision_recall_and_exact_scores(case, mode, min_recall):
         assert np.all(got[j, m:] == 0) and np.all(scores[j, m:] == 0)
 
 
+@pytest.mark.parametrize("mode", [E.PREC_BF16, E.PREC_FP8])
+def test_low_precision_candidates_match_oracle_over_windows(mode):
+    """The BF16/FP8 gathers collect with their own launch shape (512-thread
+    workgroups, rounds of 4096 ids); over dedup windows that are empty,
+    of one round and of tens, their candidate lists equal the oracle's."""
+    import torch
+    X, Q, g, o = _built("windows_far_block")
+    qp, k = CASES["windows_far_block"][6], CASES["windows_far_block"][7]
+    g.set_qProbes(qp)
+    o.set_qProbes(qp)
+    d = X.shape[1]
+    dQ = torch.from_numpy(np.ascontiguousarray(Q)).cuda()
+    out = torch.zeros((len(Q), k), dtype=torch.int32, device="cuda")
+    sc = torch.zeros((len(Q), k), dtype=torch.float32, device="cuda")
+    try:
+        g.set_precision(mode, 4 * k)
+        g.query_device(dQ.data_ptr(), d, len(Q), k, out.data_ptr(), sc.data_ptr())
+        torch.cuda.synchronize()
+        got = [g.last_candidates(j) for j in range(len(Q))]
+    finally:
+        g.set_precision(E.PREC_FP32, 0)
+    for j, q in enumerate(Q):
+        assert np.array_equal(got[j], o.query_debug(q, k)["cands"]), j
+
+
 # ---- small queries: one wave per query ----------------------------------
 
 SMALL_CASES = [
