@@ -603,7 +603,8 @@ __device__ uint32_t collect_candidates(const GatherArgs& a, int np, const uint32
     // before their ids are inserted, so a window's wait for its bin spans two
     // rounds of work rather than one.  The three buffers rotate through an
     // unrolled loop (a register copy would wait for the loads).
-    constexpr int UNR = 8;
+    // 4096 ids a round at every workgroup size up to 512 threads
+    constexpr int UNR = G::size <= 256 ? 16 : 8;
     constexpr uint32_t RND = (uint32_t)G::size * UNR;
     struct Round {
         int w;        // window (nwin: past the last one)
